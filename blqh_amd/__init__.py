@@ -1,0 +1,245 @@
+"""blqh_amd -- MI355X exact-diagonalisation engine for bilayer quantum Hall
+systems on the torus (Python side of the C ABI in include/blqh.h).
+
+The classes mirror the reference's class API (qiongz/BLQH):
+``Basis``  ~ ``basis`` (BLQH_without_spin/basis.h:25-56),
+``Hamiltonian`` ~ ``lhamil`` set_hamil / H (lanczos_hamiltonian.h:11-75),
+``Hamiltonian.lanczos`` ~ coeff_explicit_update + diag +
+eigenstates_reconstruction + ground_state_energy.
+
+All compute runs in libblqh.so (HIP kernels for gfx950).  Importing this
+package does not need a GPU; calling into it does, and it raises when the
+library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libblqh.so")
+CLI_PATH = os.path.join(HERE, "bin", "blqh")
+
+BLQH_OK = 0
+
+
+class BlqhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"blqh error {code}: {msg}")
+        self.code = code
+
+
+class Sector(C.Structure):
+    _fields_ = [("nspecies", C.c_int32), ("nphi", C.c_int32), ("nel", C.c_int32), ("nel_up", C.c_int32),
+                ("nel_su", C.c_int32), ("J", C.c_int32), ("K", C.c_int32)]
+
+
+class Params(C.Structure):
+    _fields_ = [("lx", C.c_double), ("ly", C.c_double), ("nLL", C.c_int32), ("d", C.c_double),
+                ("delta_sas", C.c_double), ("delta_v", C.c_double), ("delta_z", C.c_double),
+                ("theta_b", C.c_double), ("theta_x", C.c_double), ("theta_y", C.c_double)]
+
+
+class LanczosOpts(C.Structure):
+    _fields_ = [("lam", C.c_int32), ("seed", C.c_uint32), ("tol", C.c_double), ("check_every", C.c_int32),
+                ("store_basis", C.c_int32), ("start", C.c_void_p)]
+
+
+class LanczosResult(C.Structure):
+    _fields_ = [("E0", C.c_double), ("E_rq", C.c_double), ("dE", C.c_double), ("iterations", C.c_int32),
+                ("converged", C.c_int32), ("passes", C.c_int32), ("seconds", C.c_double)]
+
+
+# name -> (restype, argtypes); every symbol here must exist in libblqh.so
+_SIGS = {
+    "blqh_last_error": (C.c_char_p, []),
+    "blqh_init": (C.c_int, [C.c_int]),
+    "blqh_basis_create": (C.c_int, [C.POINTER(Sector), C.POINTER(C.c_void_p)]),
+    "blqh_basis_size": (C.c_int64, [C.c_void_p]),
+    "blqh_basis_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "blqh_basis_destroy": (None, [C.c_void_p]),
+    "blqh_hamil_build": (C.c_int, [C.c_void_p, C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "blqh_hamil_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "blqh_hamil_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "blqh_hamil_destroy": (None, [C.c_void_p]),
+    "blqh_spmv": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "blqh_spmv_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "blqh_lanczos": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.POINTER(LanczosResult), C.c_void_p,
+                               C.c_void_p, C.c_void_p]),
+    "blqh_dense_ground_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]),
+    "blqh_start_vector": (C.c_int, [C.c_int64, C.c_uint32, C.c_void_p]),
+    "blqh_tridiag_lowest": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
+    "blqh_profile_enable": (C.c_int, [C.c_int]),
+    "blqh_profile_read": (C.c_int, [C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "blqh_profile_reset": (C.c_int, []),
+}
+
+
+def profile_enable(on=True):
+    _check(lib().blqh_profile_enable(1 if on else 0))
+
+
+def profile_reset():
+    _check(lib().blqh_profile_reset())
+
+
+def profile_read():
+    """(launches, summed kernel ms, summed algorithmic bytes) of the fused Lanczos SpMV."""
+    n = C.c_int64()
+    ms = C.c_double()
+    b = C.c_double()
+    _check(lib().blqh_profile_read(C.byref(n), C.byref(ms), C.byref(b)))
+    return n.value, ms.value, b.value
+
+_lib = None
+
+
+def lib():
+    """Load libblqh.so; raise if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BlqhError(-3, f"{LIB_PATH} is missing; run `python -m blqh_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != BLQH_OK:
+        raise BlqhError(rc, lib().blqh_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def geometry(nphi, gamma=1.0):
+    """ly = sqrt(2 pi nphi / gamma), lx = gamma ly (BLQH_without_spin/main.cpp:20-21)."""
+    ly = np.sqrt(nphi * 2.0 * np.pi / gamma)
+    return ly * gamma, ly
+
+
+def params(nphi, gamma=1.0, nLL=0, d=0.0, delta_sas=0.0, delta_v=0.0, delta_z=0.0, theta_b=0.0, theta_x=0.0,
+           theta_y=0.0):
+    lx, ly = geometry(nphi, gamma)
+    return Params(lx, ly, nLL, d, delta_sas, delta_v, delta_z, theta_b, theta_x, theta_y)
+
+
+class Basis:
+    """(J,K) sector basis: orbit representatives ``id`` (ascending) and
+    periods ``period`` (the reference's id[] and basis_C[])."""
+
+    def __init__(self, nphi, nel, nel_up=-1, J=-1, K=-1, nspecies=2, nel_su=-1):
+        self.sector = Sector(nspecies, nphi, nel, nel_up, nel_su, J, K)
+        h = C.c_void_p()
+        _check(lib().blqh_basis_create(C.byref(self.sector), C.byref(h)))
+        self.h = h
+        self.n = lib().blqh_basis_size(self.h)
+        self._id = None
+        self._period = None
+
+    def _export(self):
+        self._id = np.empty(self.n, np.uint64)
+        self._period = np.empty(self.n, np.int16)
+        _check(lib().blqh_basis_export(self.h, _p(self._id), _p(self._period)))
+
+    @property
+    def id(self):
+        if self._id is None:
+            self._export()
+        return self._id
+
+    @property
+    def period(self):
+        if self._period is None:
+            self._export()
+        return self._period
+
+    def __len__(self):
+        return self.n
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().blqh_basis_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Hamiltonian:
+    """Hamiltonian of one sector in CSR on the GPU (lhamil::set_hamil)."""
+
+    def __init__(self, basis, par):
+        self.basis = basis
+        self.par = par
+        h = C.c_void_p()
+        _check(lib().blqh_hamil_build(basis.h, C.byref(par), C.byref(h)))
+        self.h = h
+        n = C.c_int64()
+        nnz = C.c_int64()
+        _check(lib().blqh_hamil_info(self.h, C.byref(n), C.byref(nnz)))
+        self.n, self.nnz = n.value, nnz.value
+
+    def export(self):
+        rowptr = np.empty(self.n + 1, np.int64)
+        col = np.empty(self.nnz, np.int32)
+        val = np.empty(self.nnz, np.complex128)
+        _check(lib().blqh_hamil_export(self.h, _p(rowptr), _p(col), _p(val)))
+        return rowptr, col, val
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, np.complex128)
+        y = np.empty_like(x)
+        _check(lib().blqh_spmv(self.h, _p(x), _p(y)))
+        return y
+
+    def lanczos(self, lam=200, seed=0, tol=1e-10, check_every=1, store_basis=-1, start=None, want_psi=True):
+        opts = LanczosOpts(lam, seed, tol, check_every, store_basis, None)
+        keep = None
+        if start is not None:
+            keep = np.ascontiguousarray(start, np.complex128)
+            opts.start = keep.ctypes.data
+        res = LanczosResult()
+        psi = np.empty(self.n, np.complex128) if want_psi else None
+        a = np.zeros(lam)
+        b = np.zeros(lam + 1)
+        _check(lib().blqh_lanczos(self.h, C.byref(opts), C.byref(res), _p(psi) if want_psi else None, _p(a), _p(b)))
+        out = {k: getattr(res, k) for k, _ in LanczosResult._fields_}
+        out.update(psi=psi, alpha=a[:res.iterations], beta=b[:res.iterations + 1])
+        return out
+
+    def dense_ground_state(self):
+        e0 = C.c_double()
+        ep = C.c_double()
+        v = np.empty(self.n, np.complex128)
+        _check(lib().blqh_dense_ground_state(self.h, C.byref(e0), C.byref(ep), _p(v)))
+        return e0.value, ep.value, v
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().blqh_hamil_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def start_vector(n, seed):
+    out = np.empty(n, np.complex128)
+    _check(lib().blqh_start_vector(n, seed, _p(out)))
+    return out
+
+
+def tridiag_lowest(alpha, offdiag):
+    alpha = np.ascontiguousarray(alpha, np.float64)
+    offdiag = np.ascontiguousarray(offdiag, np.float64)
+    e0 = C.c_double()
+    v = np.empty(len(alpha))
+    _check(lib().blqh_tridiag_lowest(len(alpha), _p(alpha), _p(offdiag), C.byref(e0), _p(v)))
+    return e0.value, v

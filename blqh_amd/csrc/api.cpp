@@ -1,0 +1,216 @@
+// api.cpp -- the C ABI (include/blqh.h): argument checks, error codes and
+// the thread-local error message; no exception crosses the boundary.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+void blqh_basis_build_impl(const blqh_sector& s, blqh_basis* B);
+void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil* H);
+void blqh_spmv_device_impl(blqh_hamil* H, const void* x, void* y);
+void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_result& r, double* psi, double* alpha,
+                       double* beta);
+void blqh_dense_ground_state_impl(const blqh_hamil* H, double* e0, double* e_printed, double* vec);
+namespace blqh {
+void start_vector(int64_t n, uint32_t seed, std::vector<double2>& v);
+double tridiag_lowest(int m, const double* a, const double* b, double* vec);
+}  // namespace blqh
+
+namespace {
+thread_local std::string g_err;
+std::mutex g_init_mu;
+blqh::Runtime g_rt;
+bool g_rt_ok = false;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return BLQH_OK;
+    } catch (const blqh::Error& e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(BLQH_ENOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(BLQH_EINTERNAL, e.what());
+    }
+}
+
+void init_device(int dev) {
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (g_rt_ok) {
+        BLQH_REQUIRE(dev < 0 || dev == g_rt.device, BLQH_EINVAL, "library already bound to another device");
+        return;
+    }
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) throw blqh::Error(BLQH_EDEVICE, "no HIP device available");
+    if (dev < 0) dev = 0;
+    BLQH_REQUIRE(dev < count, BLQH_EINVAL, "device index out of range");
+    BLQH_HIP(hipSetDevice(dev));
+    hipDeviceProp_t prop;
+    BLQH_HIP(hipGetDeviceProperties(&prop, dev));
+    g_rt.device = dev;
+    g_rt.num_cus = prop.multiProcessorCount;
+    BLQH_HIP(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
+    g_rt_ok = true;
+}
+}  // namespace
+
+namespace blqh {
+Runtime& runtime() {
+    if (!g_rt_ok) init_device(-1);
+    return g_rt;
+}
+hipStream_t stream() { return runtime().stream; }
+}  // namespace blqh
+
+extern "C" {
+
+const char* blqh_last_error(void) { return g_err.c_str(); }
+
+int blqh_init(int device) {
+    return guarded([&] { init_device(device); });
+}
+
+int blqh_basis_create(const blqh_sector* s, blqh_basis** out) {
+    if (!s || !out) return fail(BLQH_EINVAL, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        blqh::runtime();
+        auto* B = new blqh_basis;
+        try {
+            blqh_basis_build_impl(*s, B);
+        } catch (...) {
+            delete B;
+            throw;
+        }
+        *out = B;
+    });
+}
+
+int64_t blqh_basis_size(const blqh_basis* b) {
+    if (!b) return fail(BLQH_EINVAL, "null basis");
+    return b->n;
+}
+
+int blqh_basis_export(const blqh_basis* b, uint64_t* id, int16_t* period) {
+    if (!b) return fail(BLQH_EINVAL, "null basis");
+    if (id) std::memcpy(id, b->h_id.data(), b->h_id.size() * sizeof(uint64_t));
+    if (period) std::memcpy(period, b->h_period.data(), b->h_period.size() * sizeof(int16_t));
+    return BLQH_OK;
+}
+
+void blqh_basis_destroy(blqh_basis* b) { delete b; }
+
+int blqh_hamil_build(const blqh_basis* b, const blqh_params* p, blqh_hamil** out) {
+    if (!b || !p || !out) return fail(BLQH_EINVAL, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        auto* H = new blqh_hamil;
+        try {
+            blqh_hamil_build_impl(b, *p, H);
+        } catch (...) {
+            delete H;
+            throw;
+        }
+        *out = H;
+    });
+}
+
+int blqh_hamil_info(const blqh_hamil* h, int64_t* nrows, int64_t* nnz) {
+    if (!h) return fail(BLQH_EINVAL, "null hamiltonian");
+    if (nrows) *nrows = h->n;
+    if (nnz) *nnz = h->nnz;
+    return BLQH_OK;
+}
+
+int blqh_hamil_export(const blqh_hamil* h, int64_t* rowptr, int32_t* col, double* val) {
+    if (!h) return fail(BLQH_EINVAL, "null hamiltonian");
+    return guarded([&] {
+        hipStream_t st = blqh::stream();
+        if (rowptr) BLQH_HIP(hipMemcpyAsync(rowptr, h->rowptr.get(), (h->n + 1) * 8, hipMemcpyDeviceToHost, st));
+        if (col && h->nnz) BLQH_HIP(hipMemcpyAsync(col, h->col.get(), h->nnz * 4, hipMemcpyDeviceToHost, st));
+        if (val && h->nnz) BLQH_HIP(hipMemcpyAsync(val, h->val.get(), h->nnz * 16, hipMemcpyDeviceToHost, st));
+        BLQH_HIP(hipStreamSynchronize(st));
+    });
+}
+
+void blqh_hamil_destroy(blqh_hamil* h) { delete h; }
+
+int blqh_spmv(const blqh_hamil* h, const double* x, double* y) {
+    if (!h || !x || !y) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] {
+        hipStream_t st = blqh::stream();
+        blqh::DevBuf<double2> dx(h->n), dy(h->n);
+        BLQH_HIP(hipMemcpyAsync(dx.get(), x, h->n * 16, hipMemcpyHostToDevice, st));
+        blqh_spmv_device_impl(const_cast<blqh_hamil*>(h), dx.get(), dy.get());
+        BLQH_HIP(hipMemcpyAsync(y, dy.get(), h->n * 16, hipMemcpyDeviceToHost, st));
+        BLQH_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int blqh_spmv_device(const blqh_hamil* h, const void* x, void* y) {
+    if (!h || !x || !y) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] { blqh_spmv_device_impl(const_cast<blqh_hamil*>(h), x, y); });
+}
+
+int blqh_lanczos(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_result* r, double* psi, double* alpha,
+                 double* beta) {
+    if (!h || !o || !r) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] {
+        std::memset(r, 0, sizeof(*r));
+        blqh_lanczos_impl(const_cast<blqh_hamil*>(h), *o, *r, psi, alpha, beta);
+    });
+}
+
+int blqh_dense_ground_state(const blqh_hamil* h, double* e0, double* e_printed, double* vec) {
+    if (!h) return fail(BLQH_EINVAL, "null hamiltonian");
+    return guarded([&] { blqh_dense_ground_state_impl(h, e0, e_printed, vec); });
+}
+
+int blqh_profile_enable(int on) {
+    blqh::profile().on = on != 0;
+    return BLQH_OK;
+}
+
+int blqh_profile_read(int64_t* launches, double* kernel_ms, double* bytes) {
+    const blqh::Profile& p = blqh::profile();
+    if (launches) *launches = p.launches;
+    if (kernel_ms) *kernel_ms = p.ms;
+    if (bytes) *bytes = p.bytes;
+    return BLQH_OK;
+}
+
+int blqh_profile_reset(void) {
+    blqh::Profile& p = blqh::profile();
+    p.launches = 0;
+    p.ms = 0;
+    p.bytes = 0;
+    return BLQH_OK;
+}
+
+int blqh_start_vector(int64_t n, uint32_t seed, double* out) {
+    if (n < 0 || !out) return fail(BLQH_EINVAL, "bad argument");
+    return guarded([&] {
+        std::vector<double2> v;
+        blqh::start_vector(n, seed, v);
+        std::memcpy(out, v.data(), n * sizeof(double2));
+    });
+}
+
+int blqh_tridiag_lowest(int32_t m, const double* alpha, const double* offdiag, double* e0, double* vec) {
+    if (m < 1 || !alpha || (m > 1 && !offdiag) || !e0) return fail(BLQH_EINVAL, "bad argument");
+    return guarded([&] { *e0 = blqh::tridiag_lowest(m, alpha, offdiag, vec); });
+}
+
+}  // extern "C"

@@ -1,0 +1,362 @@
+// basis.hip -- sector basis on the GPU.
+//
+// Pipeline (reference: basis::init, BLQH_without_spin/basis.cpp:61-120 and
+// BLQH_with_spin/basis.cpp:74-141):
+//  1. host: insert the sector words into a std::unordered_map<unsigned long,
+//     long> in the reference's depth-first order and read back the map's
+//     iteration order.  The reference keeps, for every translation orbit, the
+//     member that comes last in that order (it erases a word while iterating
+//     whenever another orbit member is still present), so this order is the
+//     only input the representative choice needs.
+//  2. device: combinatorial rank of every word -> its position in that order.
+//  3. device: survivor test per word (no orbit member later in the order),
+//     spin-up filter, compaction in rank order = ascending word order (the
+//     reference's sort of id[]).
+//  4. device: orbit periods (basis_C) and the lookup table used by the
+//     Hamiltonian build: for every sector word w, the representative row j
+//     and the shift q with inv_translate(w, q) == id[j].
+#include <algorithm>
+#include <chrono>
+#include <unordered_map>
+
+#include "engine.hpp"
+
+namespace blqh {
+
+RankSpec make_rank_spec(const blqh_sector& s, bool with_spin_constraint) {
+    RankSpec r{};
+    r.nbits = s.nspecies * s.nphi;
+    r.nphi = s.nphi;
+    r.nel = s.nel;
+    r.R = s.J >= 0 ? s.nphi : 1;
+    r.Jt = s.J >= 0 ? s.J : 0;
+    r.maskA = s.nel_up >= 0 ? lowbits(r.nbits / 2) : 0;
+    r.nA = s.nel_up >= 0 ? s.nel_up : 0;
+    if (with_spin_constraint && s.nspecies == 4 && s.nel_su >= 0) {
+        r.maskB = lowbits(s.nphi) | (lowbits(s.nphi) << (2 * s.nphi));
+        r.nB = s.nel_su;
+    } else {
+        r.maskB = 0;
+        r.nB = 0;
+    }
+    r.dK = s.nel + 1;
+    r.dR = r.R;
+    r.dA = r.maskA ? r.nA + 1 : 1;
+    r.dB = r.maskB ? r.nB + 1 : 1;
+    r.sA = r.dB;
+    r.sR = (int64_t)r.dA * r.sA;
+    r.sK = (int64_t)r.dR * r.sR;
+    r.sP = (int64_t)r.dK * r.sK;
+    return r;
+}
+
+// cnt[p][k][r][a][b]: subsets of [0,p) with k elements, residual r, a in A,
+// b in B.  Entries are saturated to 32 bits; the rank/unrank walks only read
+// entries bounded by the sector size, which is checked to fit.
+std::vector<uint32_t> make_rank_table(const RankSpec& s, int64_t* total) {
+    const int64_t sz = (int64_t)(s.nbits + 1) * s.sP;
+    std::vector<int64_t> c(sz, 0);
+    auto sat_add = [](int64_t x, int64_t y) { int64_t z = x + y; return z < x ? INT64_MAX : z; };
+    c[rs_index(s, 0, 0, 0, 0, 0)] = 1;
+    for (int p = 0; p < s.nbits; ++p) {
+        const int inA = (s.maskA >> p) & 1, inB = (s.maskB >> p) & 1;
+        for (int k = 0; k < s.dK; ++k)
+            for (int r = 0; r < s.dR; ++r)
+                for (int a = 0; a < s.dA; ++a)
+                    for (int b = 0; b < s.dB; ++b) {
+                        int64_t v = c[rs_index(s, p, k, r, a, b)];
+                        if (k > 0 && a - inA >= 0 && b - inB >= 0) {
+                            int rr = (r - p % s.nphi) % s.R;
+                            if (rr < 0) rr += s.R;
+                            v = sat_add(v, c[rs_index(s, p, k - 1, rr, a - inA, b - inB)]);
+                        }
+                        c[rs_index(s, p + 1, k, r, a, b)] = v;
+                    }
+    }
+    *total = c[rs_index(s, s.nbits, s.nel, s.Jt, s.nA, s.nB)];
+    std::vector<uint32_t> out(sz);
+    for (int64_t i = 0; i < sz; ++i) out[i] = c[i] > 0xFFFFFFFFLL ? 0xFFFFFFFFu : (uint32_t)c[i];
+    return out;
+}
+
+}  // namespace blqh
+
+namespace {
+
+using namespace blqh;
+
+constexpr int kBlock = 256;
+
+int grid_for(int64_t n, int block = kBlock) {
+    int64_t g = ceil_div(n, block);
+    return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 1 << 20);
+}
+
+// ---- scan ------------------------------------------------------------------
+constexpr int kScanBlock = 1024;
+
+__global__ void k_block_sums(const int32_t* __restrict__ in, int64_t n, int64_t* __restrict__ sums) {
+    __shared__ int64_t sh[kScanBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    int64_t v = i < n ? in[i] : 0;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int w = 0; w < kScanBlock / 64; ++w) t += sh[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+// single-block exclusive scan of the block sums (in place), total to *tot
+__global__ void k_scan_sums(int64_t* __restrict__ sums, int64_t nb, int64_t* __restrict__ tot) {
+    __shared__ int64_t carry;
+    __shared__ int64_t sh[kScanBlock];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < nb; base += kScanBlock) {
+        const int64_t i = base + threadIdx.x;
+        int64_t v = i < nb ? sums[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < kScanBlock; o <<= 1) {
+            int64_t t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) sums[i] = carry + sh[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == kScanBlock - 1) carry += sh[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *tot = carry;
+}
+
+__global__ void k_block_scan(const int32_t* __restrict__ in, int64_t n, const int64_t* __restrict__ offs,
+                             int64_t* __restrict__ out) {
+    __shared__ int64_t sh[kScanBlock];
+    const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    const int64_t v = i < n ? in[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < kScanBlock; o <<= 1) {
+        int64_t t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i < n) out[i] = offs[blockIdx.x] + sh[threadIdx.x] - v;
+}
+
+__global__ void k_copy_total(const int64_t* __restrict__ tot, int64_t* __restrict__ dst) { *dst = *tot; }
+
+// ---- basis kernels -----------------------------------------------------------
+
+// posOf[rank(order[p])] = p
+__global__ void k_scatter_pos(const uint64_t* __restrict__ order, int64_t M, RankSpec s,
+                              const uint32_t* __restrict__ cnt, int32_t* __restrict__ posOf) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < M; p += (int64_t)gridDim.x * blockDim.x)
+        posOf[rank_word(s, cnt, order[p])] = (int32_t)p;
+}
+
+// flags[r] = 1 when word r is the orbit member latest in the hash order (or
+// when no reduction is requested) and passes the spin-up filter.
+__global__ void k_survivor(int64_t M, RankSpec s, const uint32_t* __restrict__ cnt,
+                           const int32_t* __restrict__ posOf, Geo g, int reduce, uint64_t spin_mask, int nel_su,
+                           int32_t* __restrict__ flags) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < M; r += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t c = unrank_word(s, cnt, r);
+        bool keep = !reduce || is_survivor(g, s, cnt, posOf, c, posOf[r]);
+        if (keep && nel_su >= 0 && popc(c & spin_mask) != nel_su) keep = false;
+        flags[r] = keep ? 1 : 0;
+    }
+}
+
+__global__ void k_compact(int64_t M, RankSpec s, const uint32_t* __restrict__ cnt, const int32_t* __restrict__ flags,
+                          const int64_t* __restrict__ offs, uint64_t* __restrict__ id) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < M; r += (int64_t)gridDim.x * blockDim.x)
+        if (flags[r]) id[offs[r]] = unrank_word(s, cnt, r);
+}
+
+// basis_C: smallest n in [1, nphi) with inv_translate(id, n) == id, else nphi.
+__global__ void k_period(int64_t n, const uint64_t* __restrict__ id, Geo g, int16_t* __restrict__ period) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t c = id[j];
+        int per = g.nphi;
+        for (int k = 1; k < g.nphi; ++k) {
+            int sg;
+            if (inv_translate(g, c, k, sg) == c) { per = k; break; }
+        }
+        period[j] = (int16_t)per;
+    }
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// lut[rank(translate(id[j], q))] = j << 5 | q for the orbit members in the sector
+__global__ void k_lut(int64_t n, const uint64_t* __restrict__ id, const int16_t* __restrict__ period, Geo g,
+                      RankSpec s, const uint32_t* __restrict__ cnt, int no_reduction, int32_t* __restrict__ lut) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t c = id[j];
+        const int D = no_reduction ? 1 : period[j];
+        for (int q = 0; q < D; ++q) {
+            int sg;
+            const uint64_t y = translate(g, c, q, sg);
+            if (!in_sector(s, y)) continue;
+            lut[rank_word(s, cnt, y)] = lut_pack(j, q);
+        }
+    }
+}
+
+// ---- host helpers ------------------------------------------------------------
+
+// The reference's depth-first insertion order (generate / generate_all_density /
+// generate_fixed_density): electron k scans its allowed orbital window upward,
+// the first nel_up electrons in the upper layer, the rest in the lower one.
+template <class F>
+void dfs_words(const blqh_sector& s, F&& emit) {
+    const int nbits = s.nspecies * s.nphi, W = nbits / 2, ne = s.nel;
+    if (ne == 0) { if (s.J <= 0) emit(0ULL); return; }
+    std::vector<int> lo(ne), hi(ne), pos(ne);
+    for (int k = 0; k < ne; ++k) {
+        if (s.nel_up >= 0 && k < s.nel_up) { lo[k] = 0; hi[k] = W - (s.nel_up - k - 1); }
+        else if (s.nel_up >= 0) { lo[k] = W; hi[k] = nbits - (ne - k - 1); }
+        else { lo[k] = 0; hi[k] = nbits - (ne - k - 1); }
+    }
+    std::vector<uint64_t> word(ne + 1, 0);
+    std::vector<int> js(ne + 1, 0);
+    int k = 0;
+    pos[0] = lo[0];
+    while (k >= 0) {
+        if (pos[k] >= hi[k]) { if (--k >= 0) ++pos[k]; continue; }
+        const uint64_t c = word[k] | (1ULL << pos[k]);
+        const int j = js[k] + pos[k] % s.nphi;
+        if (k + 1 == ne) {
+            if (s.J < 0 || j % s.nphi == s.J) emit(c);
+            ++pos[k];
+        } else {
+            word[k + 1] = c;
+            js[k + 1] = j;
+            pos[k + 1] = std::max(pos[k] + 1, lo[k + 1]);
+            ++k;
+        }
+    }
+}
+
+void validate_sector(const blqh_sector& s) {
+    BLQH_REQUIRE(s.nspecies == 2 || s.nspecies == 4, BLQH_EINVAL, "nspecies must be 2 or 4");
+    BLQH_REQUIRE(s.nphi >= 2 && s.nphi <= 31 && s.nspecies * s.nphi <= 64, BLQH_EINVAL,
+                 "nphi must satisfy 2 <= nphi <= 31 and nspecies*nphi <= 64");
+    BLQH_REQUIRE(s.nel >= 1 && s.nel <= s.nspecies * s.nphi, BLQH_EINVAL, "nel out of range");
+    BLQH_REQUIRE(s.nel_up <= s.nel, BLQH_EINVAL, "nel_up must not exceed nel");
+    BLQH_REQUIRE(s.nel_up < 0 || (s.nel_up <= s.nspecies * s.nphi / 2 && s.nel - s.nel_up <= s.nspecies * s.nphi / 2),
+                 BLQH_EINVAL, "layer occupation exceeds the layer");
+    BLQH_REQUIRE(s.J < s.nphi && s.K < s.nphi, BLQH_EINVAL, "J and K must be < nphi");
+    BLQH_REQUIRE(s.nel_su < 0 || s.nspecies == 4, BLQH_EINVAL, "nel_su needs nspecies = 4");
+}
+
+}  // namespace
+
+namespace blqh {
+
+// Iteration order of the reference's std::unordered_map<unsigned long, long>
+// after inserting the map sector (no spin-up filter yet) in depth-first order.
+std::vector<uint64_t> hash_iteration_order(const blqh_sector& s) {
+    std::unordered_map<unsigned long, long> set;
+    dfs_words(s, [&](uint64_t c) { set[(unsigned long)c] = (long)c; });
+    std::vector<uint64_t> order;
+    order.reserve(set.size());
+    for (const auto& kv : set) order.push_back((uint64_t)kv.first);
+    return order;
+}
+
+void exclusive_scan(const int32_t* in, int64_t* out, int64_t n, hipStream_t st) {
+    const int64_t nb = std::max<int64_t>(1, ceil_div(n, kScanBlock));
+    DevBuf<int64_t> sums(nb + 1);
+    hipLaunchKernelGGL(k_block_sums, dim3((unsigned)nb), dim3(kScanBlock), 0, st, in, n, sums.get());
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, st, sums.get(), nb, sums.get() + nb);
+    hipLaunchKernelGGL(k_block_scan, dim3((unsigned)nb), dim3(kScanBlock), 0, st, in, n, sums.get(), out);
+    hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, st, sums.get() + nb, out + n);
+    BLQH_HIP(hipGetLastError());
+    BLQH_HIP(hipStreamSynchronize(st));  // sums is freed on return
+}
+
+}  // namespace blqh
+
+using namespace blqh;
+
+// Build a basis; called through blqh_basis_create.
+void blqh_basis_build_impl(const blqh_sector& s, blqh_basis* B) {
+    validate_sector(s);
+    hipStream_t st = stream();
+    B->sec = s;
+    B->geo = Geo{s.nspecies, s.nphi, s.nel};
+    const bool reduce = s.K >= 0;
+    const bool spin_filter = s.nspecies == 4 && s.nel_su >= 0;
+
+    // the words the reference inserts into its map: no spin-up constraint yet
+    RankSpec smap = make_rank_spec(s, false);
+    int64_t Mmap = 0;
+    std::vector<uint32_t> tmap = make_rank_table(smap, &Mmap);
+    BLQH_REQUIRE(Mmap > 0, BLQH_EINVAL, "empty sector");
+    BLQH_REQUIRE(Mmap < (int64_t)INT32_MAX, BLQH_ERANGE, "sector too large (more than 2^31 words)");
+    DevBuf<uint32_t> dmap(tmap.size());
+    BLQH_HIP(hipMemcpyAsync(dmap.get(), tmap.data(), tmap.size() * 4, hipMemcpyHostToDevice, st));
+
+    auto t0 = std::chrono::steady_clock::now();
+    DevBuf<int32_t> posOf;
+    if (reduce) {
+        // host: the map's iteration order after inserting in depth-first order
+        std::vector<uint64_t> order = hash_iteration_order(s);
+        BLQH_REQUIRE((int64_t)order.size() == Mmap, BLQH_EINTERNAL, "enumeration size mismatch");
+        B->seconds_host = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        DevBuf<uint64_t> dorder(Mmap);
+        posOf.alloc(Mmap);
+        BLQH_HIP(hipMemcpyAsync(dorder.get(), order.data(), Mmap * 8, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_scatter_pos, dim3(grid_for(Mmap)), dim3(kBlock), 0, st, dorder.get(), Mmap, smap,
+                           dmap.get(), posOf.get());
+        BLQH_HIP(hipGetLastError());
+        BLQH_HIP(hipStreamSynchronize(st));
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    DevBuf<int32_t> flags(Mmap);
+    DevBuf<int64_t> offs(Mmap + 1);
+    const uint64_t spin_mask = lowbits(s.nphi) | (lowbits(s.nphi) << (2 * s.nphi));
+    hipLaunchKernelGGL(k_survivor, dim3(grid_for(Mmap)), dim3(kBlock), 0, st, Mmap, smap, dmap.get(), posOf.get(),
+                       B->geo, reduce ? 1 : 0, spin_mask, spin_filter ? s.nel_su : -1, flags.get());
+    BLQH_HIP(hipGetLastError());
+    exclusive_scan(flags.get(), offs.get(), Mmap, st);
+    int64_t n = 0;
+    BLQH_HIP(hipMemcpy(&n, offs.get() + Mmap, 8, hipMemcpyDeviceToHost));
+    BLQH_REQUIRE(n > 0, BLQH_EINVAL, "empty basis");
+    BLQH_REQUIRE(n < (1LL << 26), BLQH_ERANGE, "basis larger than 2^26 states is not supported");
+    B->n = n;
+    B->id.alloc(n);
+    B->period.alloc(n);
+    hipLaunchKernelGGL(k_compact, dim3(grid_for(Mmap)), dim3(kBlock), 0, st, Mmap, smap, dmap.get(), flags.get(),
+                       offs.get(), B->id.get());
+    hipLaunchKernelGGL(k_period, dim3(grid_for(n)), dim3(kBlock), 0, st, n, B->id.get(), B->geo, B->period.get());
+    BLQH_HIP(hipGetLastError());
+
+    // lookup table over the (spin-constrained) sector
+    B->spec = make_rank_spec(s, true);
+    std::vector<uint32_t> tsec = make_rank_table(B->spec, &B->M);
+    B->cnt.alloc(tsec.size());
+    BLQH_HIP(hipMemcpyAsync(B->cnt.get(), tsec.data(), tsec.size() * 4, hipMemcpyHostToDevice, st));
+    B->lut.alloc(B->M);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(B->M)), dim3(kBlock), 0, st, B->lut.get(), B->M, -1);
+    hipLaunchKernelGGL(k_lut, dim3(grid_for(n)), dim3(kBlock), 0, st, n, B->id.get(), B->period.get(), B->geo,
+                       B->spec, B->cnt.get(), reduce ? 0 : 1, B->lut.get());
+    BLQH_HIP(hipGetLastError());
+    B->h_id.resize(n);
+    B->h_period.resize(n);
+    BLQH_HIP(hipMemcpyAsync(B->h_id.data(), B->id.get(), n * 8, hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipMemcpyAsync(B->h_period.data(), B->period.get(), n * 2, hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    B->seconds_device = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+}

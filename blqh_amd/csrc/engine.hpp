@@ -1,0 +1,79 @@
+// engine.hpp -- internal types shared by the basis, Hamiltonian and Lanczos
+// stages of the blqh library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <vector>
+
+#include "bits.hpp"
+#include "common.hpp"
+#include "rowterms.hpp"
+
+struct blqh_basis {
+    blqh_sector sec{};
+    blqh::Geo geo{};
+    blqh::RankSpec spec{};           // the sector (with the spin-up constraint)
+    blqh::DevBuf<uint32_t> cnt;      // rank table of spec
+    int64_t M = 0;                   // sector words
+    int64_t n = 0;                   // basis states (orbit representatives)
+    blqh::DevBuf<uint64_t> id;       // representatives, ascending
+    blqh::DevBuf<int16_t> period;    // orbit period (basis_C)
+    blqh::DevBuf<int32_t> lut;       // sector word rank -> (row << 6 | q << 1 | sign bit), -1 = none
+    std::vector<uint64_t> h_id;
+    std::vector<int16_t> h_period;
+    double seconds_host = 0, seconds_device = 0;
+};
+
+struct blqh_hamil {
+    const blqh_basis* basis = nullptr;
+    blqh_params par{};
+    int64_t n = 0, nnz = 0;
+    blqh::DevBuf<int64_t> rowptr;
+    blqh::DevBuf<int32_t> col;
+    blqh::DevBuf<double2> val;
+    int spmv_lpr = 0;                // lanes per row chosen by the autotuner (0 = not tuned)
+    double seconds_build = 0;
+};
+
+namespace blqh {
+
+RankSpec make_rank_spec(const blqh_sector& s, bool with_spin_constraint);
+std::vector<uint32_t> make_rank_table(const RankSpec& spec, int64_t* total);
+void exclusive_scan(const int32_t* in, int64_t* out, int64_t n, hipStream_t st);  // out has n+1 entries
+std::vector<uint64_t> hash_iteration_order(const blqh_sector& s);
+
+// Event timing of the fused Lanczos SpMV (blqh_profile_*).
+struct Profile {
+    bool on = false;
+    int64_t launches = 0;
+    double ms = 0, bytes = 0;
+    std::vector<hipEvent_t> ev;  // start/stop pairs, reused across solves
+};
+Profile& profile();
+
+// Host-side tables of one Hamiltonian (Coulomb table, background energy,
+// translation phases, twist phases, tunnelling hop list).
+struct HostTables {
+    std::vector<double> C;
+    std::vector<double2> FT, tw;
+    std::vector<HopTerm> hops;
+    double Ec = 0;
+};
+HostTables host_tables(const blqh_sector& s, const blqh_params& p);
+
+// Row context with the basis' device pointers (tables left null).
+inline RowCtx make_row_ctx(const blqh_basis* B) {
+    RowCtx c{};
+    c.g = B->geo;
+    c.spec = B->spec;
+    c.M = B->M;
+    c.K = B->sec.K;
+    c.spinless = B->sec.nspecies == 2;
+    c.twist_sel = B->sec.nspecies == 2 ? 1 : 2;
+    c.period = B->period.get();
+    c.lut = B->lut.get();
+    c.cnt = B->cnt.get();
+    return c;
+}
+
+}  // namespace blqh

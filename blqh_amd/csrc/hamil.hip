@@ -1,0 +1,392 @@
+// hamil.hip -- Hamiltonian build into CSR on the GPU.
+//
+// Reference: lhamil::init_Coulomb_matrix + peer_set_hamil + set_hamil,
+// BLQH_without_spin/lanczos_hamiltonian.cpp:37-385 (spinless) and
+// BLQH_with_spin/lanczos_hamiltonian.cpp:37-954 (spinful).
+//
+// Row i of H (one wavefront per row):
+//   for every translation ql < D_l of id[i], every occupied pair (n < m) and
+//   every shift t in (-nphi, nphi): n -> nt = n + t, m -> mt = m - t (both
+//   wrapped inside their species block); when both targets are empty the
+//   target word R0 is looked up through its combinatorial rank in the sector
+//   table, which gives the representative row j, the shift qr with
+//   inv_translate(R0, qr) = id[j] and that translation's sign.  The element
+//   FT[ql][qr] * V[alpha][|dm|][|t|] * sign / sqrt(D_l D_r) is accumulated
+//   into an LDS hash keyed by j.  Tunnelling terms are one-body hops between
+//   the layers; the diagonal (background, charging, bias, Zeeman) comes last.
+// Entries with |h_ij| > 1e-10 are kept, columns ascending.
+//
+// The build runs twice over the rows: a count pass (row lengths), an
+// exclusive scan into rowptr, and a fill pass that writes columns and values.
+// Each row is owned by exactly one wavefront, so its LDS accumulation order is
+// fixed and both passes produce the same entries; the fill pass checks this.
+// Rows whose distinct columns overflow the small hash are redone by a second
+// kernel instance with a large hash.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+
+#include "engine.hpp"
+#include "rowterms.hpp"
+
+namespace blqh {
+
+// Coulomb tables, background energy and translation phases.
+// BLQH_without_spin/lanczos_hamiltonian.cpp:37-85 (theta_x enters as +2 theta_x);
+// BLQH_with_spin/lanczos_hamiltonian.cpp:37-84 (as -theta_x).
+static double vq(const blqh_params& p, int alpha, int qx, int qy) {
+    const double q = std::sqrt(qx * qx / (p.lx * p.lx) + qy * qy / (p.ly * p.ly)) * 2.0 * M_PI;
+    if (alpha == 1)
+        return 2.0 * M_PI / q * std::exp(-q * q / 2.0 - q * p.d) * std::pow(1.0 - std::exp(-q * q / 2.0), p.nLL * 2);
+    return 2.0 * M_PI / q * std::exp(-q * q / 2.0) * std::pow(1.0 - std::exp(-q * q / 2.0), p.nLL * 2);
+}
+
+HostTables host_tables(const blqh_sector& s, const blqh_params& p) {
+    HostTables H;
+    const int nphi = s.nphi;
+    const double lx = p.lx, ly = p.ly;
+    H.Ec = -2.0;
+    for (int i = 0; i < nphi; ++i)
+        for (int j = 0; j < nphi; ++j)
+            if (!(i == 0 && j == 0)) H.Ec += std::erfc(std::sqrt(M_PI * (i * i * lx / ly + j * j * ly / lx)));
+    H.Ec /= std::sqrt(lx * ly);
+    H.C.assign(2 * nphi * nphi, 0.0);
+    for (int sx = 0; sx < nphi; ++sx)
+        for (int qy = 0; qy < nphi; ++qy) {
+            double V = 0;
+            for (int qx = -nphi; qx <= nphi; ++qx)
+                if (!(qx == 0 && qy == 0)) V += 2.0 * vq(p, 0, qx, qy) * std::cos(2.0 * M_PI * sx * qx / nphi) / (2.0 * lx * ly);
+            H.C[sx * nphi + qy] = V;
+            V = 0;
+            const double arg = s.nspecies == 2 ? 2.0 * M_PI * sx + 2 * p.theta_x : 2.0 * M_PI * sx - p.theta_x;
+            const int q0 = (int)(-10 * nphi / (p.d + 0.01));
+            for (int qx = q0; qx < 10 * nphi / (p.d + 0.01); ++qx)
+                if (!(qx == 0 && qy == 0)) V += 2.0 * vq(p, 1, qx, qy) * std::cos(arg * qx / nphi) / (2.0 * lx * ly);
+            H.C[nphi * nphi + sx * nphi + qy] = V;
+        }
+    H.FT.resize(nphi * nphi);
+    for (int a = 0; a < nphi; ++a)
+        for (int b = 0; b < nphi; ++b)
+            H.FT[a * nphi + b] = make_double2(std::cos(2.0 * M_PI * (a - b) * s.K / nphi), std::sin(2.0 * M_PI * (a - b) * s.K / nphi));
+    H.tw.resize(2 * nphi - 1);
+    for (int t = -nphi + 1; t < nphi; ++t) {
+        const double ang = s.nspecies == 2 ? 2 * p.theta_y * t / nphi : p.theta_y * t / nphi;
+        H.tw[t + nphi - 1] = make_double2(std::cos(ang), std::sin(ang));
+    }
+    if (p.delta_sas > 0) {
+        if (s.nspecies == 2) {
+            // BLQH_without_spin/lanczos_hamiltonian.cpp:295-322
+            for (int n = 0; n < nphi; ++n) H.hops.push_back({n, n + nphi, n, n + nphi, -0.5 * p.delta_sas, 1, 0, 0});
+        } else {
+            // BLQH_with_spin/lanczos_hamiltonian.cpp:718-898: same-spin hops appear in two
+            // loops with -Delta_SAS/4 each (and theta_B phases), spin-flip hops once with -Delta_SAS/2.
+            const double tb = p.theta_b, d = p.d;
+            for (int n = 0; n < nphi; ++n)
+                H.hops.push_back({n, n + 2 * nphi, n, n + 2 * nphi, -0.25 * p.delta_sas, std::cos(tb * n * d), std::sin(tb * n * d), 1});
+            for (int n = 2 * nphi; n < 3 * nphi; ++n) {
+                const int nt = n - 2 * nphi;
+                H.hops.push_back({n, nt, nt, n, -0.25 * p.delta_sas, std::cos(tb * nt * d), -std::sin(tb * nt * d), 1});
+            }
+            for (int n = nphi; n < 2 * nphi; ++n)
+                H.hops.push_back({n, n + 2 * nphi, n, n + 2 * nphi, -0.25 * p.delta_sas, std::cos(tb * (n % nphi) * d),
+                                  std::sin(tb * (n % nphi) * d), 1});
+            for (int n = 3 * nphi; n < 4 * nphi; ++n)
+                H.hops.push_back({n, n - 2 * nphi, n - 2 * nphi, n, -0.25 * p.delta_sas, std::cos(tb * (n % nphi) * d),
+                                  -std::sin(tb * (n % nphi) * d), 1});
+            for (int n = 0; n < nphi; ++n) H.hops.push_back({n, n + 3 * nphi, n, n + 3 * nphi, -0.5 * p.delta_sas, 1, 0, 0});
+            for (int n = nphi; n < 2 * nphi; ++n) H.hops.push_back({n, n + nphi, n, n + nphi, -0.5 * p.delta_sas, 1, 0, 0});
+        }
+    }
+    return H;
+}
+
+}  // namespace blqh
+
+namespace {
+
+using namespace blqh;
+
+struct BuildArgs {
+    RowCtx ctx;                  // C / FT / tw point to global copies; the kernel restages them in LDS
+    int64_t nrows;               // rows to process (size of row_list or n)
+    const int32_t* row_list;     // optional row list (overflow pass)
+    const uint64_t* id;
+    const HopTerm* hops;
+    int nhop;
+    double Ecn, d, dv, dz;
+    int32_t* nnz_out;         // count pass
+    const int64_t* rowptr;    // fill pass
+    int32_t* col;
+    double2* val;
+    int32_t* row_counter;
+    int32_t* ovf_count;
+    int32_t* ovf_rows;
+    int32_t* err;             // fill/count mismatch counter
+};
+
+template <int CAP>
+__device__ __forceinline__ bool hash_add(int* keys, double* vre, double* vim, int j, double re, double im) {
+    constexpr int LOG = __builtin_ctz(CAP);
+    unsigned h = ((unsigned)j * 2654435761u) >> (32 - LOG);
+    for (int probe = 0; probe < CAP; ++probe) {
+        int k = __atomic_load_n(&keys[h], __ATOMIC_RELAXED);
+        if (k == -1) {
+            const int old = atomicCAS(&keys[h], -1, j);
+            k = old == -1 ? j : old;
+        }
+        if (k == j) {
+            atomicAdd(&vre[h], re);
+            atomicAdd(&vim[h], im);
+            return true;
+        }
+        h = (h + 1) & (CAP - 1);
+    }
+    return false;
+}
+
+// MODE 0: count, MODE 1: fill
+template <int CAP, int MODE>
+__global__ void __launch_bounds__(64) k_build(BuildArgs A) {
+    extern __shared__ double smem[];
+    __shared__ int keys[CAP];
+    __shared__ double vre[CAP], vim[CAP];
+    __shared__ unsigned long long sbuf[MODE == 1 ? CAP : 1];
+    __shared__ int occ[64];
+    __shared__ int s_row, s_ovf;
+
+    const int lane = threadIdx.x;
+    const Geo g = A.ctx.g;
+    const int nphi = g.nphi, nbits = g.S * nphi, nel = g.nel;
+    const int T = 2 * nphi - 1;
+    // stage the tables in LDS
+    double* sC = smem;                                   // 2 nphi^2
+    double2* sFT = (double2*)(sC + 2 * nphi * nphi);     // nphi^2
+    double2* sTW = sFT + nphi * nphi;                    // T
+    unsigned short* sPair = (unsigned short*)(sTW + T);  // npairs
+    const int npairs = nel * (nel - 1) / 2;
+    for (int x = lane; x < 2 * nphi * nphi; x += 64) sC[x] = A.ctx.C[x];
+    for (int x = lane; x < nphi * nphi; x += 64) sFT[x] = A.ctx.FT[x];
+    for (int x = lane; x < T; x += 64) sTW[x] = A.ctx.tw[x];
+    for (int x = lane; x < npairs; x += 64) sPair[x] = pair_decode(nel, x);
+    RowCtx ctx = A.ctx;
+    ctx.C = sC;
+    ctx.FT = sFT;
+    ctx.tw = sTW;
+    __syncthreads();
+
+    for (;;) {
+        if (lane == 0) s_row = atomicAdd(A.row_counter, 1);
+        __syncthreads();
+        const int64_t r = s_row;
+        if (r >= A.nrows) break;
+        const int64_t i = A.row_list ? A.row_list[r] : r;
+        for (int x = lane; x < CAP; x += 64) { keys[x] = -1; vre[x] = 0.0; vim[x] = 0.0; }
+        if (lane == 0) s_ovf = 0;
+        __syncthreads();
+        bool ovf = false;
+        const uint64_t L0 = A.id[i];
+        const int Dl = ctx.K < 0 ? 1 : ctx.period[i];
+        for (int ql = 0; ql < Dl; ++ql) {
+            int signl;
+            const uint64_t L = translate(g, L0, ql, signl);
+            if (lane < nbits && ((L >> lane) & 1ULL)) occ[popc(L & lowbits(lane))] = lane;
+            __syncthreads();
+            const int ncand = npairs * T;
+            for (int cnd = lane; cnd < ncand; cnd += 64) {
+                const int pr = cnd / T;
+                const int t = cnd - pr * T - (nphi - 1);
+                const unsigned short pw = sPair[pr];
+                int j;
+                double re, im;
+                if (pair_term(ctx, L, signl, Dl, ql, occ[pw & 0xff], occ[pw >> 8], t, j, re, im))
+                    if (!hash_add<CAP>(keys, vre, vim, j, re, im)) ovf = true;
+            }
+            for (int h = lane; h < A.nhop; h += 64) {
+                int j;
+                double re, im;
+                if (hop_term(ctx, L, signl, Dl, ql, A.hops[h], j, re, im))
+                    if (!hash_add<CAP>(keys, vre, vim, j, re, im)) ovf = true;
+            }
+            __syncthreads();
+        }
+        // diagonal
+        if (lane == 0) {
+            const double dg = diag_term(g, ctx.spinless != 0, L0, A.Ecn, A.d, A.dv, A.dz);
+            if (!hash_add<CAP>(keys, vre, vim, (int)i, dg, 0.0)) ovf = true;
+        }
+        if (ovf) s_ovf = 1;  // benign race: every writer stores 1
+        __syncthreads();
+        if (s_ovf) {
+            if (MODE == 0 && lane == 0) {
+                A.nnz_out[i] = 0;
+                const int k = atomicAdd(A.ovf_count, 1);
+                A.ovf_rows[k] = (int32_t)i;
+            }
+            if (MODE == 1 && lane == 0) atomicAdd(A.err, 1);
+            __syncthreads();
+            continue;
+        }
+        if (MODE == 0) {
+            int c = 0;
+            for (int x = lane; x < CAP; x += 64)
+                if (keys[x] >= 0 && hypot(vre[x], vim[x]) > 1e-10) ++c;
+            for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+            if (lane == 0) A.nnz_out[i] = c;
+        } else {
+            // compact (key, slot) pairs: lane owns a contiguous chunk of slots
+            constexpr int CH = CAP / 64;
+            int c = 0;
+            for (int x = lane * CH; x < lane * CH + CH; ++x)
+                if (keys[x] >= 0 && hypot(vre[x], vim[x]) > 1e-10) ++c;
+            int pre = c;  // inclusive scan over lanes
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(pre, o, 64);
+                if (lane >= o) pre += y;
+            }
+            const int E = __shfl(pre, 63, 64);
+            int w = pre - c;
+            for (int x = lane * CH; x < lane * CH + CH; ++x)
+                if (keys[x] >= 0 && hypot(vre[x], vim[x]) > 1e-10)
+                    sbuf[w++] = ((unsigned long long)(unsigned)keys[x] << 32) | (unsigned)x;
+            int P = 1;
+            while (P < E) P <<= 1;
+            for (int x = E + lane; x < P; x += 64) sbuf[x] = ~0ULL;
+            __syncthreads();
+            // bitonic sort of sbuf[0, P)
+            for (int k = 2; k <= P; k <<= 1)
+                for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                    for (int x = lane; x < P; x += 64) {
+                        const int y = x ^ jj;
+                        if (y > x) {
+                            const unsigned long long u = sbuf[x], v = sbuf[y];
+                            const bool up = (x & k) == 0;
+                            if ((u > v) == up) { sbuf[x] = v; sbuf[y] = u; }
+                        }
+                    }
+                    __syncthreads();
+                }
+            const int64_t base = A.rowptr[i];
+            const int64_t len = A.rowptr[i + 1] - base;
+            if (len != E) {
+                if (lane == 0) atomicAdd(A.err, 1);
+            } else {
+                for (int x = lane; x < E; x += 64) {
+                    const unsigned long long u = sbuf[x];
+                    const int slot = (int)(u & 0xffffffffu);
+                    A.col[base + x] = (int32_t)(u >> 32);
+                    A.val[base + x] = make_double2(vre[slot], vim[slot]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+constexpr int kSmall = 512, kLarge = 4096;
+
+size_t table_smem(const Geo& g) {
+    const int nphi = g.nphi, T = 2 * nphi - 1, np = g.nel * (g.nel - 1) / 2;
+    return sizeof(double) * 2 * nphi * nphi + sizeof(double2) * (nphi * nphi + T) + sizeof(unsigned short) * np + 16;
+}
+
+template <class T>
+T* upload(DevBuf<T>& buf, const std::vector<T>& v, hipStream_t st) {
+    buf.alloc(std::max<size_t>(v.size(), 1));
+    if (!v.empty()) BLQH_HIP(hipMemcpyAsync(buf.get(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return buf.get();
+}
+
+template <int CAP, int MODE>
+void launch_build(const BuildArgs& A, int64_t rows, size_t smem, hipStream_t st, int ncu) {
+    if (rows <= 0) return;
+    const int64_t blocks = std::min<int64_t>(rows, (int64_t)ncu * 16);
+    BLQH_HIP(hipMemsetAsync(A.row_counter, 0, sizeof(int32_t), st));
+    hipLaunchKernelGGL((k_build<CAP, MODE>), dim3((unsigned)blocks), dim3(64), smem, st, A);
+    BLQH_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil* H) {
+    using namespace blqh;
+    hipStream_t st = stream();
+    const int ncu = runtime().num_cus;
+    const blqh_sector& s = B->sec;
+    BLQH_REQUIRE(p.lx > 0 && p.ly > 0, BLQH_EINVAL, "lx and ly must be positive");
+    BLQH_REQUIRE(s.nel <= 64 && s.nel * (s.nel - 1) / 2 < 65536, BLQH_EINVAL, "too many electrons");
+    auto t0 = std::chrono::steady_clock::now();
+    H->basis = B;
+    H->par = p;
+    H->n = B->n;
+    HostTables T = host_tables(s, p);
+    DevBuf<double> dC;
+    DevBuf<double2> dFT, dTW;
+    DevBuf<HopTerm> dHop;
+    BuildArgs A{};
+    A.ctx = make_row_ctx(B);
+    A.ctx.C = upload(dC, T.C, st);
+    A.ctx.FT = upload(dFT, T.FT, st);
+    A.ctx.tw = upload(dTW, T.tw, st);
+    A.nrows = B->n;
+    A.row_list = nullptr;
+    A.id = B->id.get();
+    A.hops = upload(dHop, T.hops, st);
+    A.nhop = (int)T.hops.size();
+    A.Ecn = T.Ec * s.nel;
+    A.d = p.d;
+    A.dv = p.delta_v;
+    A.dz = p.delta_z;
+    DevBuf<int32_t> nnz(B->n), ctr(4), ovf_rows(B->n);
+    BLQH_HIP(hipMemsetAsync(ctr.get(), 0, 4 * sizeof(int32_t), st));
+    A.row_counter = ctr.get();
+    A.ovf_count = ctr.get() + 1;
+    A.err = ctr.get() + 2;
+    A.ovf_rows = ovf_rows.get();
+    A.nnz_out = nnz.get();
+    const size_t smem = table_smem(B->geo);
+    // count
+    launch_build<kSmall, 0>(A, B->n, smem, st, ncu);
+    int32_t h_ctr[4];
+    BLQH_HIP(hipMemcpyAsync(h_ctr, ctr.get(), sizeof(h_ctr), hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    const int32_t novf = h_ctr[1];
+    BuildArgs Ao = A;
+    DevBuf<int32_t> ovf2(std::max(novf, 1)), ctr2(4);
+    if (novf > 0) {
+        // rows that overflowed the small hash: recount with the large one
+        BLQH_HIP(hipMemsetAsync(ctr2.get(), 0, 4 * sizeof(int32_t), st));
+        Ao.row_list = ovf_rows.get();
+        Ao.nrows = novf;
+        Ao.row_counter = ctr2.get();
+        Ao.ovf_count = ctr2.get() + 1;
+        Ao.err = ctr2.get() + 2;
+        Ao.ovf_rows = ovf2.get();
+        launch_build<kLarge, 0>(Ao, novf, smem, st, ncu);
+        BLQH_HIP(hipMemcpyAsync(h_ctr, ctr2.get(), sizeof(h_ctr), hipMemcpyDeviceToHost, st));
+        BLQH_HIP(hipStreamSynchronize(st));
+        BLQH_REQUIRE(h_ctr[1] == 0, BLQH_ERANGE, "a Hamiltonian row has more than 4096 distinct columns");
+    }
+    H->rowptr.alloc(B->n + 1);
+    exclusive_scan(nnz.get(), H->rowptr.get(), B->n, st);
+    BLQH_HIP(hipMemcpy(&H->nnz, H->rowptr.get() + B->n, 8, hipMemcpyDeviceToHost));
+    H->col.alloc(std::max<int64_t>(H->nnz, 1));
+    H->val.alloc(std::max<int64_t>(H->nnz, 1));
+    // fill
+    A.rowptr = Ao.rowptr = H->rowptr.get();
+    A.col = Ao.col = H->col.get();
+    A.val = Ao.val = H->val.get();
+    BLQH_HIP(hipMemsetAsync(ctr.get(), 0, 4 * sizeof(int32_t), st));
+    launch_build<kSmall, 1>(A, B->n, smem, st, ncu);  // overflow rows are skipped (counted in err)
+    if (novf > 0) {
+        BLQH_HIP(hipMemsetAsync(ctr2.get(), 0, 4 * sizeof(int32_t), st));
+        launch_build<kLarge, 1>(Ao, novf, smem, st, ncu);
+    }
+    int32_t e1[4] = {0, 0, 0, 0}, e2[4] = {0, 0, 0, 0};
+    BLQH_HIP(hipMemcpyAsync(e1, ctr.get(), sizeof(e1), hipMemcpyDeviceToHost, st));
+    if (novf > 0) BLQH_HIP(hipMemcpyAsync(e2, ctr2.get(), sizeof(e2), hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    // the small-hash fill pass reports exactly the overflow rows as errors
+    BLQH_REQUIRE(e1[2] == novf && e2[2] == 0, BLQH_EINTERNAL, "count and fill passes disagree");
+    H->seconds_build = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}

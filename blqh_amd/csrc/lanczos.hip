@@ -1,0 +1,553 @@
+// lanczos.hip -- complex FP64 CSR SpMV and the Lanczos ground-state solver.
+//
+// Reference: Mat::operator* (BLQH_without_spin/matrix.cpp:244-267),
+// coeff_explicit_update / diag / eigenstates_reconstruction /
+// ground_state_energy (BLQH_without_spin/lanczos_hamiltonian.cpp:453-656).
+//
+// Vectors are kept unnormalised: v_j = W_j / beta_j.  One Lanczos step is two
+// kernels:
+//   K1  u = (H W_j)/beta_j - beta_j W_{j-1}/beta_{j-1},  alpha_j = Re<v_j|u>
+//   K2  W_{j+1} = u - alpha_j v_j,  beta_{j+1} = |W_{j+1}|
+// Both reduce through per-block partials summed in block order by the last
+// block to finish, so every scalar is bit-reproducible: a replayed pass 3
+// regenerates exactly the pass-1 vectors.  When the Krylov vectors fit in
+// HBM they are kept (W_j are written in place) and the ground state is one
+// weighted sum over them instead of a replay.
+// The reconstructed state is psi = sum_l c_l conj(v_l) as in the reference
+// (lanczos_hamiltonian.cpp:624-640), normalised; E_rq = Re<psi|H psi>.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+
+#include "engine.hpp"
+
+namespace {
+
+using namespace blqh;
+
+constexpr int kThreads = 256;
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+// streaming (non-temporal) load of a double2: matrix data is read once per
+// SpMV and should not evict the gathered vector from L2 / MALL
+__device__ __forceinline__ double2 ntload(const double2* p) {
+    const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+    return make_double2(v.x, v.y);
+}
+
+// ---- deterministic grid reduction -------------------------------------------
+// Block partial -> partials[blockIdx]; the last block (ticket) sums them in
+// order and hands the total to `fin`.
+template <class Fin>
+__device__ __forceinline__ void grid_reduce(double v, double* partials, unsigned* ticket, Fin fin) {
+    __shared__ double sh[kThreads / 64];
+    __shared__ bool last;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
+        partials[blockIdx.x] = t;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double t = 0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) t += __builtin_nontemporal_load(&partials[b]);
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+        fin(s);
+        *ticket = 0;
+    }
+}
+
+struct Scal {  // device scalars per iteration
+    double* alpha;  // [cap]
+    double* beta;   // [cap+1], beta[0] = 1
+    double* invb;   // [cap+1], invb[0] = 1
+};
+
+// ---- SpMV ---------------------------------------------------------------------
+template <int LPR>
+__device__ __forceinline__ double2 row_dot(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                           const double2* __restrict__ val, const double2* __restrict__ x,
+                                           int64_t row, int lane) {
+    const int64_t s = rowptr[row], e = rowptr[row + 1];
+    double re = 0.0, im = 0.0;
+    for (int64_t k = s + lane; k < e; k += LPR) {
+        const double2 v = ntload(&val[k]);
+        const int c = __builtin_nontemporal_load(&col[k]);
+        const double2 xv = x[c];
+        re += v.x * xv.x - v.y * xv.y;
+        im += v.x * xv.y + v.y * xv.x;
+    }
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) {
+        re += __shfl_xor(re, o, LPR);
+        im += __shfl_xor(im, o, LPR);
+    }
+    return make_double2(re, im);
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(kThreads) k_spmv(int64_t n, const int64_t* __restrict__ rowptr,
+                                                    const int32_t* __restrict__ col, const double2* __restrict__ val,
+                                                    const double2* __restrict__ x, double2* __restrict__ y) {
+    constexpr int RPB = kThreads / LPR;
+    const int lane = threadIdx.x % LPR;
+    for (int64_t row = (int64_t)blockIdx.x * RPB + threadIdx.x / LPR; row < n; row += (int64_t)gridDim.x * RPB) {
+        const double2 r = row_dot<LPR>(rowptr, col, val, x, row, lane);
+        if (lane == 0) y[row] = r;
+    }
+}
+
+// K1: u = invb[j] (H W) - (j > 0 ? beta[j] invb[j-1] Wp : 0); alpha[j] = invb[j] Re<W|u>
+template <int LPR>
+__global__ void __launch_bounds__(kThreads) k_lz1(int64_t n, const int64_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ col, const double2* __restrict__ val,
+                                                   const double2* __restrict__ W, const double2* __restrict__ Wp,
+                                                   double2* __restrict__ u, Scal sc, int j, double* partials,
+                                                   unsigned* ticket) {
+    constexpr int RPB = kThreads / LPR;
+    const int lane = threadIdx.x % LPR;
+    const double ib = sc.invb[j];
+    const double cp = j > 0 ? sc.beta[j] * sc.invb[j - 1] : 0.0;
+    double acc = 0.0;
+    for (int64_t row = (int64_t)blockIdx.x * RPB + threadIdx.x / LPR; row < n; row += (int64_t)gridDim.x * RPB) {
+        const double2 r = row_dot<LPR>(rowptr, col, val, W, row, lane);
+        if (lane == 0) {
+            double ur = ib * r.x, ui = ib * r.y;
+            if (j > 0) {
+                const double2 p = Wp[row];
+                ur -= cp * p.x;
+                ui -= cp * p.y;
+            }
+            u[row] = make_double2(ur, ui);
+            const double2 w = W[row];
+            acc += w.x * ur + w.y * ui;
+        }
+    }
+    grid_reduce(acc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
+}
+
+// K2: Wn = u - alpha[j] invb[j] W; beta[j+1] = |Wn|; optional psi += coef[j] invb[j] conj(W)
+__global__ void __launch_bounds__(kThreads) k_lz2(int64_t n, const double2* __restrict__ u,
+                                                   const double2* __restrict__ W, double2* __restrict__ Wn, Scal sc,
+                                                   int j, double2* __restrict__ psi, const double* __restrict__ coef,
+                                                   double* partials, unsigned* ticket) {
+    const double a = sc.alpha[j] * sc.invb[j];
+    const double cj = psi ? coef[j] * sc.invb[j] : 0.0;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const double2 w = W[i], x = u[i];
+        const double2 r = make_double2(x.x - a * w.x, x.y - a * w.y);
+        Wn[i] = r;
+        acc += r.x * r.x + r.y * r.y;
+        if (psi) {
+            double2 p = psi[i];
+            p.x += cj * w.x;
+            p.y -= cj * w.y;
+            psi[i] = p;
+        }
+    }
+    grid_reduce(acc, partials, ticket, [&](double s) {
+        const double b = sqrt(s);
+        sc.beta[j + 1] = b;
+        sc.invb[j + 1] = 1.0 / b;
+    });
+}
+
+// psi += coef[l] invb[l] conj(W_l) for one vector (last replayed vector)
+__global__ void k_accum(int64_t n, const double2* __restrict__ W, double c, double2* __restrict__ psi) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const double2 w = W[i];
+        double2 p = psi[i];
+        p.x += c * w.x;
+        p.y -= c * w.y;
+        psi[i] = p;
+    }
+}
+
+// psi = sum_l coef[l] conj(W_l), W_l stored with stride ld
+__global__ void __launch_bounds__(kThreads) k_combine(int64_t n, int m, const double2* __restrict__ Wb, int64_t ld,
+                                                       const double* __restrict__ coef, double2* __restrict__ psi) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        double re = 0.0, im = 0.0;
+        for (int l = 0; l < m; ++l) {
+            const double2 w = ntload(&Wb[(int64_t)l * ld + i]);
+            re += coef[l] * w.x;
+            im -= coef[l] * w.y;
+        }
+        psi[i] = make_double2(re, im);
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_norm2(int64_t n, const double2* __restrict__ x, double* out,
+                                                     double* partials, unsigned* ticket) {
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const double2 v = x[i];
+        acc += v.x * v.x + v.y * v.y;
+    }
+    grid_reduce(acc, partials, ticket, [&](double s) { *out = s; });
+}
+
+__global__ void k_scale(int64_t n, double2* __restrict__ x, const double* __restrict__ nrm2) {
+    const double s = 1.0 / sqrt(*nrm2);
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        double2 v = x[i];
+        v.x *= s;
+        v.y *= s;
+        x[i] = v;
+    }
+}
+
+// Re<x|y>
+__global__ void __launch_bounds__(kThreads) k_redot(int64_t n, const double2* __restrict__ x,
+                                                     const double2* __restrict__ y, double* out, double* partials,
+                                                     unsigned* ticket) {
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const double2 a = x[i], b = y[i];
+        acc += a.x * b.x + a.y * b.y;
+    }
+    grid_reduce(acc, partials, ticket, [&](double s) { *out = s; });
+}
+
+// ---- launch helpers -----------------------------------------------------------
+int vec_grid(int64_t n) {
+    const int ncu = runtime().num_cus;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kThreads), (int64_t)ncu * 8));
+}
+
+int spmv_grid(int64_t n, int lpr) {
+    const int ncu = runtime().num_cus;
+    const int64_t rpb = kThreads / lpr;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, rpb), (int64_t)ncu * 16));
+}
+
+#define BLQH_LPR_SWITCH(lpr, KERNEL, ...)                      \
+    switch (lpr) {                                             \
+        case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;   \
+        case 8: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;   \
+        case 16: hipLaunchKernelGGL(KERNEL<16>, __VA_ARGS__); break; \
+        case 32: hipLaunchKernelGGL(KERNEL<32>, __VA_ARGS__); break; \
+        default: hipLaunchKernelGGL(KERNEL<64>, __VA_ARGS__); break; \
+    }
+
+void launch_spmv(const blqh_hamil* H, int lpr, const double2* x, double2* y, hipStream_t st) {
+    BLQH_LPR_SWITCH(lpr, k_spmv, dim3(spmv_grid(H->n, lpr)), dim3(kThreads), 0, st, H->n, H->rowptr.get(),
+                    H->col.get(), H->val.get(), x, y);
+}
+
+// Pick lanes-per-row by timing each variant on this matrix.
+int autotune_lpr(blqh_hamil* H) {
+    if (H->spmv_lpr) return H->spmv_lpr;
+    hipStream_t st = stream();
+    const double mean_row = H->n ? (double)H->nnz / H->n : 0;
+    DevBuf<double2> x(H->n), y(H->n);
+    BLQH_HIP(hipMemsetAsync(x.get(), 0, H->n * sizeof(double2), st));
+    hipEvent_t e0, e1;
+    BLQH_HIP(hipEventCreate(&e0));
+    BLQH_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_lpr = 32;
+    const int reps = H->nnz > 50000000 ? 2 : 5;
+    for (int lpr : {4, 8, 16, 32, 64}) {
+        if (lpr > 4 * mean_row + 8 && lpr > 8) continue;  // too wide for these rows
+        launch_spmv(H, lpr, x.get(), y.get(), st);        // warm
+        BLQH_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) launch_spmv(H, lpr, x.get(), y.get(), st);
+        BLQH_HIP(hipEventRecord(e1, st));
+        BLQH_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        BLQH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; best_lpr = lpr; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    H->spmv_lpr = best_lpr;
+    return best_lpr;
+}
+
+}  // namespace
+
+// ---- host: start vector and tridiagonal eigensolver ------------------------------
+namespace blqh {
+
+// std::mt19937(seed); entry i takes its imaginary part from draw 2i and its
+// real part from draw 2i+1 (the evaluation order g++ gives to the two rng()
+// calls in the reference's complex<double>(re, im) constructor), each
+// draw/max - 0.5; normalised by the running sum of |z|^2.
+void start_vector(int64_t n, uint32_t seed, std::vector<double2>& v) {
+    std::mt19937 rng(seed);
+    v.resize(n);
+    double nrm = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double im = rng() * 1.0 / rng.max() - 0.5;
+        const double re = rng() * 1.0 / rng.max() - 0.5;
+        v[i] = make_double2(re, im);
+        nrm += re * re + im * im;
+    }
+    nrm = std::sqrt(nrm);
+    for (int64_t i = 0; i < n; ++i) { v[i].x /= nrm; v[i].y /= nrm; }
+}
+
+// Number of eigenvalues of T below x (Sturm sequence of the LDL^T pivots).
+static int sturm_count(int m, const double* a, const double* b, double x) {
+    int c = 0;
+    double q = 1.0;
+    for (int i = 0; i < m; ++i) {
+        const double bb = i ? b[i - 1] * b[i - 1] : 0.0;
+        q = a[i] - x - (i ? bb / q : 0.0);
+        if (q == 0.0) q = -1e-300;
+        if (q < 0) ++c;
+    }
+    return c;
+}
+
+// Lowest eigenvalue by bisection, eigenvector by inverse iteration.
+double tridiag_lowest(int m, const double* a, const double* b, double* vec) {
+    double lo = a[0], hi = a[0];
+    for (int i = 0; i < m; ++i) {
+        const double r = (i ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(b[i]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        if (sturm_count(m, a, b, mid) >= 1) hi = mid; else lo = mid;
+    }
+    const double e = 0.5 * (lo + hi);
+    if (!vec) return e;
+    // inverse iteration: (T - e I) x = y with partial pivoting (Gaussian elimination on the band)
+    std::vector<double> x(m, 1.0 / std::sqrt((double)m)), y(m);
+    const double shift = e - 1e-14 * std::max(1.0, std::fabs(e));
+    for (int iter = 0; iter < 4; ++iter) {
+        // band LU of (T - shift I) with row pivoting: rows hold (d0, d1, d2) = cols (i, i+1, i+2)
+        std::vector<double> d0(m), d1(m), d2(m), rhs(x);
+        std::vector<double> l(m, 0.0);
+        std::vector<int> piv(m, 0);
+        for (int i = 0; i < m; ++i) {
+            d0[i] = a[i] - shift;
+            d1[i] = i + 1 < m ? b[i] : 0.0;
+            d2[i] = 0.0;
+        }
+        // sub-diagonal of row i+1 is b[i]
+        std::vector<double> sub(m, 0.0);
+        for (int i = 0; i + 1 < m; ++i) sub[i + 1] = b[i];
+        for (int i = 0; i + 1 < m; ++i) {
+            // rows i (pivot candidate) and i+1 (entry sub[i+1] in column i)
+            if (std::fabs(sub[i + 1]) > std::fabs(d0[i])) {
+                // swap rows i and i+1: row i+1 has (sub, d0, d1) in cols (i, i+1, i+2)
+                std::swap(d0[i], sub[i + 1]);
+                std::swap(d1[i], d0[i + 1]);
+                std::swap(d2[i], d1[i + 1]);
+                std::swap(rhs[i], rhs[i + 1]);
+                piv[i] = 1;
+            }
+            if (d0[i] == 0.0) d0[i] = 1e-300;
+            const double f = sub[i + 1] / d0[i];
+            l[i] = f;
+            d0[i + 1] -= f * d1[i];
+            d1[i + 1] -= f * d2[i];
+            rhs[i + 1] -= f * rhs[i];
+        }
+        if (d0[m - 1] == 0.0) d0[m - 1] = 1e-300;
+        for (int i = m - 1; i >= 0; --i) {
+            double s = rhs[i];
+            if (i + 1 < m) s -= d1[i] * y[i + 1];
+            if (i + 2 < m) s -= d2[i] * y[i + 2];
+            y[i] = s / d0[i];
+        }
+        double nr = 0;
+        for (int i = 0; i < m; ++i) nr += y[i] * y[i];
+        nr = std::sqrt(nr);
+        for (int i = 0; i < m; ++i) x[i] = y[i] / nr;
+    }
+    // sign convention: first component non-negative
+    const double sg = x[0] < 0 ? -1.0 : 1.0;
+    for (int i = 0; i < m; ++i) vec[i] = sg * x[i];
+    return e;
+}
+
+}  // namespace blqh
+
+// ---- kernel timing -----------------------------------------------------------------
+namespace blqh {
+Profile& profile() {
+    static Profile p;
+    return p;
+}
+}  // namespace blqh
+
+// ---- public stage functions ------------------------------------------------------
+void blqh_spmv_device_impl(blqh_hamil* H, const void* x, void* y) {
+    const int lpr = autotune_lpr(H);
+    launch_spmv(H, lpr, (const double2*)x, (double2*)y, blqh::stream());
+    BLQH_HIP(hipGetLastError());
+}
+
+void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_result& res, double* psi_out,
+                       double* alpha_out, double* beta_out) {
+    using namespace blqh;
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = stream();
+    const int64_t n = H->n;
+    const int cap = o.lambda;
+    BLQH_REQUIRE(cap >= 1 && cap <= 100000, BLQH_EINVAL, "lambda out of range");
+    const int chk = std::max(1, o.check_every);
+    const int lpr = autotune_lpr(H);
+
+    // Krylov storage: keep all vectors when they fit comfortably
+    size_t free_b = 0, total_b = 0;
+    BLQH_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t need = (size_t)(cap + 1) * n * sizeof(double2);
+    bool store = o.store_basis > 0 || (o.store_basis < 0 && need < free_b / 2);
+    const int nvec = store ? cap + 1 : 3;
+    DevBuf<double2> Wb((size_t)nvec * n), u(n), psi(n);
+    DevBuf<double> sc_buf(3 * (size_t)cap + 8), coef(cap + 1), red(4);
+    const int gmax = std::max(spmv_grid(n, lpr), vec_grid(n));
+    DevBuf<double> partials(gmax);
+    DevBuf<unsigned> ticket(1);
+    BLQH_HIP(hipMemsetAsync(ticket.get(), 0, sizeof(unsigned), st));
+    Scal sc{sc_buf.get(), sc_buf.get() + cap, sc_buf.get() + 2 * cap + 2};
+    const double one = 1.0;
+    BLQH_HIP(hipMemcpyAsync(sc.beta, &one, 8, hipMemcpyHostToDevice, st));
+    BLQH_HIP(hipMemcpyAsync(sc.invb, &one, 8, hipMemcpyHostToDevice, st));
+
+    // start vector
+    std::vector<double2> v0;
+    if (o.start) {
+        v0.assign((const double2*)o.start, (const double2*)o.start + n);
+        double s = 0;
+        for (auto& z : v0) s += z.x * z.x + z.y * z.y;
+        BLQH_REQUIRE(s > 0 && std::isfinite(s), BLQH_EINVAL, "start vector is zero or not finite");
+        s = std::sqrt(s);
+        for (auto& z : v0) { z.x /= s; z.y /= s; }
+    } else {
+        start_vector(n, o.seed, v0);
+    }
+    auto slot = [&](int j) -> double2* { return Wb.get() + (size_t)(store ? j : j % 3) * n; };
+    BLQH_HIP(hipMemcpyAsync(slot(0), v0.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
+
+    const int gs = spmv_grid(n, lpr), gv = vec_grid(n);
+    Profile& prof = profile();
+    int ntimed = 0;
+    auto step = [&](int j, double2* psi_acc, const double* cf) {
+        double2* W = slot(j);
+        double2* Wp = j > 0 ? slot(j - 1) : slot(j);
+        if (prof.on) {
+            while ((int)prof.ev.size() < 2 * (ntimed + 1)) {
+                hipEvent_t e;
+                BLQH_HIP(hipEventCreate(&e));
+                prof.ev.push_back(e);
+            }
+            BLQH_HIP(hipEventRecord(prof.ev[2 * ntimed], st));
+        }
+        BLQH_LPR_SWITCH(lpr, k_lz1, dim3(gs), dim3(kThreads), 0, st, n, H->rowptr.get(), H->col.get(),
+                        H->val.get(), W, Wp, u.get(), sc, j, partials.get(), ticket.get());
+        if (prof.on) BLQH_HIP(hipEventRecord(prof.ev[2 * ntimed++ + 1], st));
+        hipLaunchKernelGGL(k_lz2, dim3(gv), dim3(kThreads), 0, st, n, u.get(), W, slot(j + 1), sc, j, psi_acc, cf,
+                           partials.get(), ticket.get());
+    };
+
+    // pass 1
+    std::vector<double> ha(cap), hb(cap + 1), ritz;
+    int m = 0;  // Krylov vectors in use
+    bool conv = false;
+    double dE = INFINITY;
+    int done = 0;
+    while (done < cap && !conv) {
+        const int upto = std::min(cap, done + chk);
+        for (int j = done; j < upto; ++j) step(j, nullptr, nullptr);
+        BLQH_HIP(hipGetLastError());
+        BLQH_HIP(hipMemcpyAsync(ha.data(), sc.alpha, upto * 8, hipMemcpyDeviceToHost, st));
+        BLQH_HIP(hipMemcpyAsync(hb.data(), sc.beta, (upto + 1) * 8, hipMemcpyDeviceToHost, st));
+        BLQH_HIP(hipStreamSynchronize(st));
+        for (int j = done; j < upto; ++j) {
+            // T_{j+1}: alpha[0..j], off-diagonal beta[1..j]
+            const double e = tridiag_lowest(j + 1, ha.data(), hb.data() + 1, nullptr);
+            ritz.push_back(e);
+            m = j + 1;
+            if (j > 0) dE = std::fabs(ritz[j] - ritz[j - 1]);
+            const double scale = std::max(std::fabs(ha[j]), 1.0);
+            if (!(hb[j + 1] > 1e-13 * scale) || !std::isfinite(hb[j + 1])) { conv = true; break; }  // invariant subspace
+            if (o.tol > 0 && j > 0 && dE < o.tol) { conv = true; break; }
+        }
+        done = upto;
+    }
+    BLQH_REQUIRE(std::isfinite(ritz.back()), BLQH_EINTERNAL, "Lanczos produced non-finite coefficients");
+
+    // pass 2: lowest eigenpair of T_m
+    std::vector<double> c(m);
+    const double E0 = tridiag_lowest(m, ha.data(), hb.data() + 1, c.data());
+    // pass 3
+    std::vector<double> invb(m);
+    {
+        std::vector<double> hib(m);
+        BLQH_HIP(hipMemcpy(hib.data(), sc.invb, m * 8, hipMemcpyDeviceToHost));
+        invb = hib;
+    }
+    if (store) {
+        std::vector<double> w(m);
+        for (int l = 0; l < m; ++l) w[l] = c[l] * invb[l];
+        BLQH_HIP(hipMemcpyAsync(coef.get(), w.data(), m * 8, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_combine, dim3(gv), dim3(kThreads), 0, st, n, m, Wb.get(), n, coef.get(), psi.get());
+        res.passes = 2;
+    } else {
+        // replay with the same kernels: alpha/beta are regenerated bit for bit
+        BLQH_HIP(hipMemcpyAsync(coef.get(), c.data(), m * 8, hipMemcpyHostToDevice, st));
+        BLQH_HIP(hipMemsetAsync(psi.get(), 0, n * sizeof(double2), st));
+        BLQH_HIP(hipMemcpyAsync(slot(0), v0.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
+        for (int j = 0; j + 1 < m; ++j) step(j, psi.get(), coef.get());
+        // last vector's weight needs invb[m-1], which the replay just recomputed
+        std::vector<double> hb2(m);
+        BLQH_HIP(hipMemcpyAsync(hb2.data(), sc.invb, m * 8, hipMemcpyDeviceToHost, st));
+        std::vector<double> ha2(m);
+        BLQH_HIP(hipMemcpyAsync(ha2.data(), sc.alpha, (m - 1) * 8, hipMemcpyDeviceToHost, st));
+        BLQH_HIP(hipStreamSynchronize(st));
+        for (int j = 0; j + 1 < m; ++j)
+            BLQH_REQUIRE(ha2[j] == ha[j], BLQH_EINTERNAL, "pass-3 replay diverged from pass 1");
+        hipLaunchKernelGGL(k_accum, dim3(gv), dim3(kThreads), 0, st, n, slot(m - 1), c[m - 1] * hb2[m - 1], psi.get());
+        res.passes = 3;
+    }
+    // normalise and Rayleigh quotient
+    hipLaunchKernelGGL(k_norm2, dim3(gv), dim3(kThreads), 0, st, n, psi.get(), red.get(), partials.get(), ticket.get());
+    hipLaunchKernelGGL(k_scale, dim3(gv), dim3(kThreads), 0, st, n, psi.get(), red.get());
+    launch_spmv(H, lpr, psi.get(), u.get(), st);
+    hipLaunchKernelGGL(k_redot, dim3(gv), dim3(kThreads), 0, st, n, psi.get(), u.get(), red.get() + 1, partials.get(),
+                       ticket.get());
+    BLQH_HIP(hipGetLastError());
+    double erq = 0;
+    BLQH_HIP(hipMemcpyAsync(&erq, red.get() + 1, 8, hipMemcpyDeviceToHost, st));
+    if (psi_out) BLQH_HIP(hipMemcpyAsync(psi_out, psi.get(), n * sizeof(double2), hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    if (prof.on) {
+        for (int k = 0; k < ntimed; ++k) {
+            float ms = 0;
+            BLQH_HIP(hipEventElapsedTime(&ms, prof.ev[2 * k], prof.ev[2 * k + 1]));
+            prof.ms += ms;
+        }
+        prof.launches += ntimed;
+        prof.bytes += (double)ntimed * (20.0 * H->nnz + 56.0 * n + 8.0);
+    }
+    if (alpha_out) { std::fill(alpha_out, alpha_out + cap, 0.0); std::copy(ha.begin(), ha.begin() + m, alpha_out); }
+    if (beta_out) { std::fill(beta_out, beta_out + cap + 1, 0.0); std::copy(hb.begin(), hb.begin() + m + 1, beta_out); }
+    res.E0 = E0;
+    res.E_rq = erq;
+    res.dE = dE;
+    res.iterations = m;
+    res.converged = conv ? 1 : 0;
+    res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}

@@ -1,0 +1,164 @@
+// rowterms.hpp -- per-element logic of the basis and Hamiltonian kernels,
+// written once for host and device (the CPU test mirror runs the same code).
+#pragma once
+#include "bits.hpp"
+
+namespace blqh {
+
+// One-body (tunnelling) hop between orbitals n and nt; lo < hi is the range
+// whose empty orbitals give the sign (BLQH_without_spin/basis.cpp:187-196).
+struct HopTerm {
+    int n, nt;
+    int lo, hi;
+    double coef;  // prefactor including Delta_SAS
+    double qre, qim;
+    int hasq;
+};
+
+struct RowCtx {
+    Geo g;
+    RankSpec spec;
+    int64_t M;      // sector words (lookup-table length)
+    int K;          // < 0: no translation reduction (D_l = D_r = 1)
+    int spinless;   // operand order of the reference's spinless / spinful expressions
+    int twist_sel;  // 1: spinless interlayer pairs, 2: spinful (0,3)/(1,2) pairs
+    const int16_t* period;
+    const int32_t* lut;
+    const uint32_t* cnt;
+    const double* C;     // [2][nphi][nphi] Coulomb table V[alpha][|dm|][|t|]
+    const double2* FT;   // [nphi][nphi] exp(2 pi i (ql - qr) K / nphi)
+    const double2* tw;   // [2 nphi - 1] twist phase per t
+};
+
+BLQH_HD double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+// Survivor test of the reference's orbit reduction (basis.cpp:82-99): word c
+// at position pos(c) of the map's iteration order survives when no other
+// orbit member in the sector comes later.
+template <class T>
+BLQH_HD bool is_survivor(const Geo& g, const RankSpec& s, const T* cnt, const int32_t* posOf, uint64_t c, int32_t p) {
+    for (int n = 1; n < g.nphi; ++n) {
+        int sg;
+        const uint64_t c2 = inv_translate(g, c, n, sg);
+        if (c2 != c && in_sector(s, c2) && posOf[rank_word(s, cnt, c2)] > p) return false;
+    }
+    return true;
+}
+
+// Lookup-table entry for a sector word y = translate(id[j], q): row j and the
+// shift q with inv_translate(y, q) == id[j] as a word.
+BLQH_HD int32_t lut_pack(int64_t j, int q) { return (int32_t)((j << 5) | q); }
+
+// Representative of a hop target.  L = translate(id[i], ql) carries momentum
+// J + ql*nel, so R0 (same momentum) is first rotated back by ql into the
+// sector; the table then gives (j, q1) and the reference's shift is
+// qr = (ql + q1) mod D_r, the unique qr < D_r with inv_translate(R0, qr) ==
+// id[j] (lanczos_hamiltonian.cpp:147-165).  Its sign is recomputed for qr:
+// rotating by the period itself may carry a fermion sign.
+BLQH_HD bool locate(const RowCtx& c, uint64_t R0, int ql, int& j, int& qr, int& Dr, int& sr) {
+    int s0;
+    const uint64_t Rs = ql ? inv_translate(c.g, R0, ql, s0) : R0;
+    const int64_t rk = rank_word(c.spec, c.cnt, Rs);
+    if (rk < 0 || rk >= c.M) return false;  // defensive: unreachable for sector words
+    const int32_t e = c.lut[rk];
+    if (e < 0) return false;
+    j = e >> 5;
+    if (c.K < 0) {
+        Dr = 1;
+        qr = 0;
+        sr = 1;
+        return true;
+    }
+    Dr = c.period[j];
+    qr = (ql + (e & 31)) % Dr;
+    (void)inv_translate(c.g, R0, qr, sr);
+    return true;
+}
+
+// Two-body term: pair (n < m) occupied in L, shift t.  Returns false when the
+// hop is blocked or its target has no representative.
+// BLQH_without_spin/lanczos_hamiltonian.cpp:109-292, BLQH_with_spin/lanczos_hamiltonian.cpp:109-712.
+BLQH_HD bool pair_term(const RowCtx& c, uint64_t L, int signl, int Dl, int ql, int n, int m, int t, int& j, double& re,
+                       double& im) {
+    const int nphi = c.g.nphi;
+    const int a = n / nphi, b = m / nphi;
+    const int nt = a * nphi + (n - a * nphi + t + nphi) % nphi;
+    const int mt = b * nphi + (m - b * nphi - t + nphi) % nphi;
+    const uint64_t rest = L ^ ((1ULL << n) | (1ULL << m));
+    const uint64_t tgt = (1ULL << nt) | (1ULL << mt);
+    if (nt == mt || (rest & tgt)) return false;
+    int qr, Dr, sr;
+    if (!locate(c, rest | tgt, ql, j, qr, Dr, sr)) return false;
+    const int sign = sign2(L, n, m, nt, mt) * signl * sr;
+    const int half = c.g.S / 2;
+    const int alpha = (a / half) != (b / half) ? 1 : 0;
+    int dm = (mt % nphi) - (n % nphi);
+    dm = dm < 0 ? -dm : dm;
+    const int at = t < 0 ? -t : t;
+    const double Cv = c.C[alpha * nphi * nphi + dm * nphi + at];
+    double2 ft = c.FT[ql * nphi + qr];
+    const bool twisted = c.twist_sel == 1 ? (a != b) : (c.twist_sel == 2 && ((a == 0 && b == 3) || (a == 1 && b == 2)));
+    if (twisted) ft = cmul(ft, c.tw[t + nphi - 1]);
+    const double sq = sqrt((double)(Dl * Dr));
+    if (c.spinless) {
+        const double den = sq * sign;
+        re = (ft.x * Cv) / den;
+        im = (ft.y * Cv) / den;
+    } else {
+        const double cs = Cv * sign;
+        re = (cs * ft.x) / sq;
+        im = (cs * ft.y) / sq;
+    }
+    return true;
+}
+
+// One-body tunnelling term.  BLQH_without_spin/lanczos_hamiltonian.cpp:295-322,
+// BLQH_with_spin/lanczos_hamiltonian.cpp:718-898.
+BLQH_HD bool hop_term(const RowCtx& c, uint64_t L, int signl, int Dl, int ql, const HopTerm& h, int& j, double& re,
+                      double& im) {
+    const uint64_t mask = (1ULL << h.n) | (1ULL << h.nt);
+    const uint64_t kn = L & mask;
+    if (kn == 0 || kn == mask) return false;
+    const uint64_t R0 = L ^ mask;
+    if (c.spec.maskA && popc(R0 & c.spec.maskA) != c.spec.nA) return false;
+    if (c.spec.maskB && popc(R0 & c.spec.maskB) != c.spec.nB) return false;
+    int qr, Dr, sr;
+    if (!locate(c, R0, ql, j, qr, Dr, sr)) return false;
+    const int sign = sign1(L, h.lo, h.hi) * signl * sr;
+    const int nphi = c.g.nphi;
+    const double cs = h.coef * sign;
+    const double2 f = c.FT[ql * nphi + qr];
+    double2 v = make_double2(cs * f.x, cs * f.y);
+    if (h.hasq) v = cmul(v, make_double2(h.qre, h.qim));
+    const double sq = sqrt((double)(Dl * Dr));
+    re = v.x / sq;
+    im = v.y / sq;
+    return true;
+}
+
+// Diagonal: background Ec*nel, charging -d n_U n_L / nphi, bias, Zeeman.
+// BLQH_without_spin/lanczos_hamiltonian.cpp:324-339, BLQH_with_spin/lanczos_hamiltonian.cpp:901-908.
+BLQH_HD double diag_term(const Geo& g, bool spinless, uint64_t c, double Ecn, double d, double dv, double dz) {
+    const int nphi = g.nphi;
+    const uint64_t m = lowbits(nphi);
+    if (spinless) {
+        const int nu = popc(c & m), nl = popc((c >> nphi) & m);
+        return (dv > 0 ? 0.5 * dv * (nu - nl) : 0.0) + Ecn + (-d * nu * (g.nel - nu)) / nphi;
+    }
+    const int n0 = popc(c & m), n1 = popc((c >> nphi) & m), n2 = popc((c >> 2 * nphi) & m),
+              n3 = popc((c >> 3 * nphi) & m);
+    return Ecn + (-d * (n0 + n1) * (n2 + n3)) / nphi + (-0.5 * dv) * (n0 + n1 - n2 - n3) +
+           (-0.5 * dz) * (n0 + n2 - n1 - n3);
+}
+
+// Pair index x (row-major over ia < ib) -> (ia, ib) packed as ia | ib << 8.
+BLQH_HD unsigned short pair_decode(int nel, int x) {
+    int ia = 0, rem = x;
+    while (rem >= nel - 1 - ia) {
+        rem -= nel - 1 - ia;
+        ++ia;
+    }
+    return (unsigned short)(ia | ((ia + 1 + rem) << 8));
+}
+
+}  // namespace blqh

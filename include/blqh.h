@@ -1,0 +1,143 @@
+/* blqh.h -- C ABI of the MI355X exact-diagonalisation engine for bilayer
+ * quantum Hall systems on the torus (drop-in for the basis -> hamiltonian ->
+ * lanczos_hamiltonian -> matrix path of qiongz/BLQH).
+ *
+ * Conventions
+ *  - Every function returns BLQH_OK (0) or a negative error code;
+ *    blqh_last_error() returns the calling thread's last message.
+ *  - Handles are opaque and owned by the library; output buffers are owned by
+ *    the caller.  Complex arrays are interleaved (re, im) doubles.
+ *  - One process drives one GPU through one HIP stream.  A handle must not be
+ *    used from two threads at once.
+ *  - No exceptions cross this boundary.
+ */
+#ifndef BLQH_H
+#define BLQH_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BLQH_OK 0
+#define BLQH_EINVAL (-1)   /* invalid argument or unsupported sector */
+#define BLQH_ENOMEM (-2)   /* device allocation failed */
+#define BLQH_EDEVICE (-3)  /* HIP runtime error or no GPU */
+#define BLQH_ERANGE (-4)   /* caller buffer too small / size out of range */
+#define BLQH_EINTERNAL (-5) /* consistency check failed */
+
+/* A (J,K) sector.  Reference: basis::init(nphi, nel, nel_up, J, K),
+ * BLQH_without_spin/basis.cpp:141-149; spinful
+ * basis::init(nphi, nel, nel_up, nel_su, J, K), BLQH_with_spin/basis.cpp:177-188. */
+typedef struct blqh_sector {
+    int32_t nspecies; /* 2: spinless bilayer; 4: spinful bilayer (upper-up, upper-down, lower-up, lower-down) */
+    int32_t nphi;     /* flux quanta (orbitals) per species */
+    int32_t nel;      /* electrons */
+    int32_t nel_up;   /* electrons in the upper layer; -1 = all layer densities */
+    int32_t nel_su;   /* spin-up electrons (nspecies = 4); -1 = unconstrained */
+    int32_t J;        /* total momentum mod nphi; -1 = no filter */
+    int32_t K;        /* translation momentum; -1 = no translation reduction */
+} blqh_sector;
+
+/* Hamiltonian parameters.  Reference: lhamil::set_hamil(lx, ly, nphi, nLL, d,
+ * Delta_SAS, Delta_V, theta_x, theta_y, nthread),
+ * BLQH_without_spin/lanczos_hamiltonian.cpp:361-385; spinful adds Delta_Z
+ * and theta_B, BLQH_with_spin/lanczos_hamiltonian.cpp:930-954. */
+typedef struct blqh_params {
+    double lx, ly;      /* torus lengths: ly = sqrt(2 pi nphi / gamma), lx = gamma ly */
+    int32_t nLL;        /* Landau level index */
+    double d;           /* layer distance d/l */
+    double delta_sas;   /* tunnelling gap (applied when > 0) */
+    double delta_v;     /* bias (spinless: applied when > 0) */
+    double delta_z;     /* Zeeman energy (spinful) */
+    double theta_b;     /* in-plane field phase (spinful) */
+    double theta_x;     /* twist entering the interlayer table */
+    double theta_y;     /* twist on interlayer pair hops */
+} blqh_params;
+
+typedef struct blqh_basis blqh_basis;
+typedef struct blqh_hamil blqh_hamil;
+
+const char* blqh_last_error(void);
+
+/* Select the GPU (device index) and create the library stream.  Optional:
+ * every other entry point initialises device 0 on first use. */
+int blqh_init(int device);
+
+/* ---- basis: basis::init + id[] / basis_C[]  (BLQH_without_spin/basis.cpp:61-120,
+ *      basis.h:32-34; BLQH_with_spin/basis.cpp:74-141) ---- */
+int blqh_basis_create(const blqh_sector* sector, blqh_basis** out);
+int64_t blqh_basis_size(const blqh_basis* b);          /* nbasis; < 0 on error */
+int blqh_basis_export(const blqh_basis* b, uint64_t* id, int16_t* period);
+void blqh_basis_destroy(blqh_basis* b);
+
+/* ---- Hamiltonian build into CSR: lhamil::set_hamil / peer_set_hamil
+ *      (BLQH_without_spin/lanczos_hamiltonian.cpp:87-385; matrix.h:71-88) ---- */
+int blqh_hamil_build(const blqh_basis* b, const blqh_params* p, blqh_hamil** out);
+int blqh_hamil_info(const blqh_hamil* h, int64_t* nrows, int64_t* nnz);
+/* rowptr[nrows+1], col[nnz], val[2*nnz]; any pointer may be NULL. */
+int blqh_hamil_export(const blqh_hamil* h, int64_t* rowptr, int32_t* col, double* val);
+void blqh_hamil_destroy(blqh_hamil* h);
+
+/* ---- y = H x: Mat::operator*  (BLQH_without_spin/matrix.cpp:244-267) ---- */
+int blqh_spmv(const blqh_hamil* h, const double* x, double* y);            /* host arrays, 2*nrows doubles */
+int blqh_spmv_device(const blqh_hamil* h, const void* x_dev, void* y_dev); /* device arrays, library stream */
+
+/* ---- Lanczos: coeff_explicit_update + diag + eigenstates_reconstruction +
+ *      ground_state_energy (BLQH_without_spin/lanczos_hamiltonian.cpp:453-656) ---- */
+typedef struct blqh_lanczos_opts {
+    int32_t lambda;       /* step cap (-m, lanczos_hamiltonian.h:15) */
+    uint32_t seed;        /* std::mt19937 seed of the start vector */
+    double tol;           /* stop when the lowest Ritz value moves by < tol; <= 0: run lambda steps */
+    int32_t check_every;  /* iterations between convergence checks (>= 1) */
+    int32_t store_basis;  /* 1: keep the Krylov vectors in HBM (two passes); 0: replay (three passes); -1: auto */
+    const double* start;  /* optional start vector (2*nrows doubles), normalised here; NULL = seeded mt19937 */
+} blqh_lanczos_opts;
+
+typedef struct blqh_lanczos_result {
+    double E0;            /* lowest Ritz value (eigenvalues[0]) */
+    double E_rq;          /* Re<psi|H psi> of the reconstructed state (ground_state_energy) */
+    double dE;            /* last change of the lowest Ritz value */
+    int32_t iterations;   /* Lanczos vectors generated (tridiagonal order) */
+    int32_t converged;    /* 1 when |dE| < tol was reached */
+    int32_t passes;       /* 2 (stored basis) or 3 (replay) */
+    double seconds;       /* wall time of the solve */
+} blqh_lanczos_result;
+
+/* psi (2*nrows, may be NULL): reconstructed ground state, psi = sum_l c_l conj(v_l),
+ * normalised, as the reference's psir_0.  alpha/beta (may be NULL): lambda and
+ * lambda+1 doubles (beta[0] = 1 as in the reference). */
+int blqh_lanczos(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_result* r,
+                 double* psi, double* alpha, double* beta);
+
+/* Dense cross-check: hamil::diag + ground_state_energy
+ * (BLQH_without_spin/hamiltonian.cpp:341-353,413-432).  Diagonalises the dense
+ * matrix with rocSOLVER zheevd in the reference's convention (row-major H read
+ * column-major, i.e. H^T); e0 = lowest eigenvalue, e_printed = Re<H v|v> for
+ * the first eigenvector v (the value the reference prints), vec (2*nrows, may
+ * be NULL) = v.  Limited to 16384 states. */
+int blqh_dense_ground_state(const blqh_hamil* h, double* e0, double* e_printed, double* vec);
+
+/* Start vector of the reference: std::mt19937(seed), entry i =
+ * (draw_{2i+1}, draw_{2i}) / max - 0.5, normalised
+ * (lanczos_hamiltonian.cpp:466-483, matrix.cpp:54-75). */
+int blqh_start_vector(int64_t n, uint32_t seed, double* out);
+
+/* Lowest eigenpair of the symmetric tridiagonal matrix (alpha[m], beta[m-1]);
+ * the reference's diag() pass 2 (lanczos_hamiltonian.cpp:581-606). */
+int blqh_tridiag_lowest(int32_t m, const double* alpha, const double* offdiag, double* e0, double* vec);
+
+/* Kernel timing (engine instrumentation; no counterpart in the reference).
+ * When enabled, every fused Lanczos SpMV launch (the solver's dominant kernel)
+ * is bracketed by HIP events on the library stream; blqh_profile_read returns
+ * the launch count, the summed kernel time and the algorithmic bytes summed
+ * over those launches (20 nnz + 56 n + 8 per launch: CSR values, columns,
+ * row pointers, x read once, previous vector read, output written). */
+int blqh_profile_enable(int on);
+int blqh_profile_read(int64_t* launches, double* kernel_ms, double* bytes);
+int blqh_profile_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BLQH_H */

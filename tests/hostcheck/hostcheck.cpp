@@ -1,0 +1,177 @@
+// hostcheck.cpp -- CPU mirror of the blqh device pipeline (test support).
+//
+// Runs the product's own per-element code (bits.hpp / rowterms.hpp: ranking,
+// translations, signs, survivor test, lookup packing, pair / hop / diagonal
+// terms) and its host steps (rank tables, hash iteration order, tables) on the
+// CPU, with plain loops in place of the kernels' scheduling, so the CPU test
+// suite can compare the GPU path's arithmetic with the oracle without a GPU.
+// The GPU-only parts (LDS hash, bitonic sort, reductions) are covered by the
+// GPU tests.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "../../blqh_amd/csrc/engine.hpp"
+
+using namespace blqh;
+
+namespace {
+struct HostBasis {
+    blqh_sector s;
+    Geo g;
+    RankSpec spec;
+    std::vector<uint32_t> cnt;
+    int64_t M = 0;
+    std::vector<uint64_t> id;
+    std::vector<int16_t> period;
+    std::vector<int32_t> lut;
+};
+
+void build(const blqh_sector& s, HostBasis& B) {
+    B.s = s;
+    B.g = Geo{s.nspecies, s.nphi, s.nel};
+    RankSpec smap = make_rank_spec(s, false);
+    int64_t Mmap = 0;
+    std::vector<uint32_t> tmap = make_rank_table(smap, &Mmap);
+    std::vector<int32_t> posOf(Mmap, 0);
+    const bool reduce = s.K >= 0;
+    if (reduce) {
+        std::vector<uint64_t> order = hash_iteration_order(s);
+        for (size_t p = 0; p < order.size(); ++p) posOf[rank_word(smap, tmap.data(), order[p])] = (int32_t)p;
+    }
+    const uint64_t spin_mask = lowbits(s.nphi) | (lowbits(s.nphi) << (2 * s.nphi));
+    B.id.clear();
+    for (int64_t r = 0; r < Mmap; ++r) {
+        const uint64_t c = unrank_word(smap, tmap.data(), r);
+        bool keep = !reduce || is_survivor(B.g, smap, tmap.data(), posOf.data(), c, posOf[r]);
+        if (keep && s.nspecies == 4 && s.nel_su >= 0 && popc(c & spin_mask) != s.nel_su) keep = false;
+        if (keep) B.id.push_back(c);
+    }
+    const int64_t n = (int64_t)B.id.size();
+    B.period.assign(n, (int16_t)s.nphi);
+    for (int64_t j = 0; j < n; ++j)
+        for (int k = 1; k < s.nphi; ++k) {
+            int sg;
+            if (inv_translate(B.g, B.id[j], k, sg) == B.id[j]) { B.period[j] = (int16_t)k; break; }
+        }
+    B.spec = make_rank_spec(s, true);
+    B.cnt = make_rank_table(B.spec, &B.M);
+    B.lut.assign(B.M, -1);
+    for (int64_t j = 0; j < n; ++j) {
+        const int D = reduce ? B.period[j] : 1;
+        for (int q = 0; q < D; ++q) {
+            int sg;
+            const uint64_t y = translate(B.g, B.id[j], q, sg);
+            if (!in_sector(B.spec, y)) continue;
+            B.lut[rank_word(B.spec, B.cnt.data(), y)] = lut_pack(j, q);
+        }
+    }
+}
+}  // namespace
+
+extern "C" {
+
+// Basis through the mirror: returns n (or -n when cap is too small).
+int64_t hc_basis(const blqh_sector* s, uint64_t* id, int16_t* period, int64_t cap) {
+    HostBasis B;
+    build(*s, B);
+    const int64_t n = (int64_t)B.id.size();
+    if (n > cap) return -n;
+    std::memcpy(id, B.id.data(), n * 8);
+    std::memcpy(period, B.period.data(), n * 2);
+    return n;
+}
+
+// Rank table checks: unrank(0..M-1) into words (ascending), M returned.
+int64_t hc_sector_words(const blqh_sector* s, int with_spin, uint64_t* out, int64_t cap) {
+    RankSpec sp = make_rank_spec(*s, with_spin != 0);
+    int64_t M = 0;
+    std::vector<uint32_t> t = make_rank_table(sp, &M);
+    if (M > cap) return -M;
+    for (int64_t r = 0; r < M; ++r) {
+        out[r] = unrank_word(sp, t.data(), r);
+        if (!in_sector(sp, out[r]) || rank_word(sp, t.data(), out[r]) != r) return -1 - r - M;  // inconsistency
+    }
+    return M;
+}
+
+uint64_t hc_translate(int S, int nphi, int nel, uint64_t c, int k, int* sign) {
+    return translate(Geo{S, nphi, nel}, c, k, *sign);
+}
+uint64_t hc_inv_translate(int S, int nphi, int nel, uint64_t c, int k, int* sign) {
+    return inv_translate(Geo{S, nphi, nel}, c, k, *sign);
+}
+int hc_sign2(uint64_t c, int n, int m, int nt, int mt) { return sign2(c, n, m, nt, mt); }
+int hc_sign1(uint64_t c, int n, int nt) { return sign1(c, n, nt); }
+
+// Full Hamiltonian through the mirror.  Returns nnz (or -nnz when cap is too small).
+int64_t hc_hamil(const blqh_sector* s, const blqh_params* p, int64_t* rowptr, int64_t* col, double* val, int64_t cap) {
+    HostBasis B;
+    build(*s, B);
+    HostTables T = host_tables(*s, *p);
+    RowCtx c{};
+    c.g = B.g;
+    c.spec = B.spec;
+    c.M = B.M;
+    c.K = s->K;
+    c.spinless = s->nspecies == 2;
+    c.twist_sel = s->nspecies == 2 ? 1 : 2;
+    c.period = B.period.data();
+    c.lut = B.lut.data();
+    c.cnt = B.cnt.data();
+    c.C = T.C.data();
+    c.FT = T.FT.data();
+    c.tw = T.tw.data();
+    const int nphi = s->nphi, nel = s->nel, Tn = 2 * nphi - 1;
+    const int npairs = nel * (nel - 1) / 2;
+    const int64_t n = (int64_t)B.id.size();
+    std::vector<int64_t> rp(n + 1, 0), cl;
+    std::vector<double> vl;
+    for (int64_t i = 0; i < n; ++i) {
+        std::map<int, std::pair<double, double>> acc;
+        const uint64_t L0 = B.id[i];
+        const int Dl = c.K < 0 ? 1 : B.period[i];
+        for (int ql = 0; ql < Dl; ++ql) {
+            int signl;
+            const uint64_t L = translate(c.g, L0, ql, signl);
+            int occ[64], k = 0;
+            for (int b = 0; b < c.g.S * nphi; ++b)
+                if ((L >> b) & 1ULL) occ[k++] = b;
+            for (int cnd = 0; cnd < npairs * Tn; ++cnd) {
+                const int pr = cnd / Tn, t = cnd - pr * Tn - (nphi - 1);
+                const unsigned short pw = pair_decode(nel, pr);
+                int j;
+                double re, im;
+                if (pair_term(c, L, signl, Dl, ql, occ[pw & 0xff], occ[pw >> 8], t, j, re, im)) {
+                    acc[j].first += re;
+                    acc[j].second += im;
+                }
+            }
+            for (const HopTerm& h : T.hops) {
+                int j;
+                double re, im;
+                if (hop_term(c, L, signl, Dl, ql, h, j, re, im)) {
+                    acc[j].first += re;
+                    acc[j].second += im;
+                }
+            }
+        }
+        acc[(int)i].first += diag_term(c.g, c.spinless != 0, L0, T.Ec * nel, p->d, p->delta_v, p->delta_z);
+        for (auto& kv : acc)
+            if (std::hypot(kv.second.first, kv.second.second) > 1e-10) {
+                cl.push_back(kv.first);
+                vl.push_back(kv.second.first);
+                vl.push_back(kv.second.second);
+            }
+        rp[i + 1] = (int64_t)cl.size();
+    }
+    const int64_t nnz = (int64_t)cl.size();
+    if (nnz > cap) return -nnz;
+    std::memcpy(rowptr, rp.data(), (n + 1) * 8);
+    std::memcpy(col, cl.data(), nnz * 8);
+    std::memcpy(val, vl.data(), nnz * 16);
+    return nnz;
+}
+
+}  // extern "C"

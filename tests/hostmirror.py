@@ -1,0 +1,55 @@
+"""ctypes access to tests/hostcheck/_build/libhostcheck.so, the CPU mirror of
+the device pipeline (same per-element code as the kernels)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+import blqh_amd
+
+PATH = os.path.join(os.path.dirname(__file__), "hostcheck", "_build", "libhostcheck.so")
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        _L = C.CDLL(PATH)
+        _L.hc_basis.restype = C.c_int64
+        _L.hc_basis.argtypes = [C.POINTER(blqh_amd.Sector), C.c_void_p, C.c_void_p, C.c_int64]
+        _L.hc_hamil.restype = C.c_int64
+        _L.hc_hamil.argtypes = [C.POINTER(blqh_amd.Sector), C.POINTER(blqh_amd.Params), C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_int64]
+        _L.hc_sector_words.restype = C.c_int64
+        _L.hc_sector_words.argtypes = [C.POINTER(blqh_amd.Sector), C.c_int, C.c_void_p, C.c_int64]
+    return _L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def basis(sec):
+    cap = 1 << 20
+    ids = np.empty(cap, np.uint64)
+    per = np.empty(cap, np.int16)
+    n = lib().hc_basis(C.byref(sec), _p(ids), _p(per), cap)
+    assert n >= 0
+    return ids[:n].copy(), per[:n].copy()
+
+
+def hamil(sec, par, n, cap):
+    rp = np.empty(n + 1, np.int64)
+    col = np.empty(cap, np.int64)
+    val = np.empty(2 * cap)
+    nnz = lib().hc_hamil(C.byref(sec), C.byref(par), _p(rp), _p(col), _p(val), cap)
+    assert nnz >= 0, "capacity too small"
+    return rp, col[:nnz].copy(), val[:2 * nnz].view(np.complex128).copy()
+
+
+def sector_words(sec, with_spin=True):
+    cap = 1 << 22
+    out = np.empty(cap, np.uint64)
+    m = lib().hc_sector_words(C.byref(sec), 1 if with_spin else 0, _p(out), cap)
+    assert m >= 0, "rank/unrank inconsistency"
+    return out[:m].copy()
