@@ -42,12 +42,13 @@ class Params(C.Structure):
 
 class LanczosOpts(C.Structure):
     _fields_ = [("lam", C.c_int32), ("seed", C.c_uint32), ("tol", C.c_double), ("check_every", C.c_int32),
-                ("store_basis", C.c_int32), ("start", C.c_void_p)]
+                ("store_basis", C.c_int32), ("start", C.c_void_p), ("res_tol", C.c_double)]
 
 
 class LanczosResult(C.Structure):
     _fields_ = [("E0", C.c_double), ("E_rq", C.c_double), ("dE", C.c_double), ("iterations", C.c_int32),
-                ("converged", C.c_int32), ("passes", C.c_int32), ("seconds", C.c_double)]
+                ("converged", C.c_int32), ("passes", C.c_int32), ("seconds", C.c_double),
+                ("residual", C.c_double)]
 
 
 # name -> (restype, argtypes); every symbol here must exist in libblqh.so
@@ -199,8 +200,11 @@ class Hamiltonian:
         _check(lib().blqh_spmv(self.h, _p(x), _p(y)))
         return y
 
-    def lanczos(self, lam=200, seed=0, tol=1e-10, check_every=1, store_basis=-1, start=None, want_psi=True):
-        opts = LanczosOpts(lam, seed, tol, check_every, store_basis, None)
+    def lanczos(self, lam=200, seed=0, tol=1e-10, res_tol=1e-7, check_every=1, store_basis=-1, start=None,
+                want_psi=True):
+        """Ground state.  Stops when the lowest Ritz value moves by < tol and the
+        Ritz residual is < res_tol (tol <= 0: exactly lam steps)."""
+        opts = LanczosOpts(lam, seed, tol, check_every, store_basis, None, res_tol)
         keep = None
         if start is not None:
             keep = np.ascontiguousarray(start, np.complex128)

@@ -195,7 +195,7 @@ int main(int argc, char** argv) {
         std::cout << "E_gs:= skipped (nHilbert > 16384)" << std::endl;
     }
     std::cout << "----------- Lanczos results ---------" << std::endl;
-    blqh_lanczos_opts lo{r.lambda, (uint32_t)r.seed, tol, 1, -1, nullptr};
+    blqh_lanczos_opts lo{r.lambda, (uint32_t)r.seed, tol, 1, -1, nullptr, tol > 0 ? 1e-7 : 0.0};
     blqh_lanczos_result res{};
     std::vector<double> psi(2 * n);
     if ((rc = blqh_lanczos(H, &lo, &res, psi.data(), nullptr, nullptr))) die_api(rc, "lanczos");

@@ -466,7 +466,8 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
     std::vector<double> ha(cap), hb(cap + 1), ritz;
     int m = 0;  // Krylov vectors in use
     bool conv = false;
-    double dE = INFINITY;
+    double dE = INFINITY, resid = INFINITY;
+    std::vector<double> svec(cap);
     int done = 0;
     while (done < cap && !conv) {
         const int upto = std::min(cap, done + chk);
@@ -482,8 +483,18 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
             m = j + 1;
             if (j > 0) dE = std::fabs(ritz[j] - ritz[j - 1]);
             const double scale = std::max(std::fabs(ha[j]), 1.0);
-            if (!(hb[j + 1] > 1e-13 * scale) || !std::isfinite(hb[j + 1])) { conv = true; break; }  // invariant subspace
-            if (o.tol > 0 && j > 0 && dE < o.tol) { conv = true; break; }
+            if (!(hb[j + 1] > 1e-13 * scale) || !std::isfinite(hb[j + 1])) {  // invariant subspace
+                resid = 0;
+                conv = true;
+                break;
+            }
+            if (o.tol > 0 && j > 0 && dE < o.tol) {
+                if (o.res_tol <= 0) { conv = true; break; }
+                // residual of the Ritz pair: |H y - theta y| = beta_{j+1} |s_j|
+                (void)tridiag_lowest(j + 1, ha.data(), hb.data() + 1, svec.data());
+                resid = hb[j + 1] * std::fabs(svec[j]);
+                if (resid < o.res_tol) { conv = true; break; }
+            }
         }
         done = upto;
     }
@@ -547,6 +558,7 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
     res.E0 = E0;
     res.E_rq = erq;
     res.dE = dE;
+    res.residual = std::isfinite(resid) ? resid : hb[m] * std::fabs(c[m - 1]);
     res.iterations = m;
     res.converged = conv ? 1 : 0;
     res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

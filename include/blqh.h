@@ -92,6 +92,8 @@ typedef struct blqh_lanczos_opts {
     int32_t check_every;  /* iterations between convergence checks (>= 1) */
     int32_t store_basis;  /* 1: keep the Krylov vectors in HBM (two passes); 0: replay (three passes); -1: auto */
     const double* start;  /* optional start vector (2*nrows doubles), normalised here; NULL = seeded mt19937 */
+    double res_tol;       /* also require the Ritz residual beta_{m} |s_m| < res_tol (bounds the state's
+                             error; <= 0: energy criterion only) */
 } blqh_lanczos_opts;
 
 typedef struct blqh_lanczos_result {
@@ -102,6 +104,7 @@ typedef struct blqh_lanczos_result {
     int32_t converged;    /* 1 when |dE| < tol was reached */
     int32_t passes;       /* 2 (stored basis) or 3 (replay) */
     double seconds;       /* wall time of the solve */
+    double residual;      /* Ritz residual estimate beta_{m} |s_m| at the stop */
 } blqh_lanczos_result;
 
 /* psi (2*nrows, may be NULL): reconstructed ground state, psi = sum_l c_l conj(v_l),

@@ -77,13 +77,19 @@ def test_lanczos_matches_lapack(oracle_mod, name, case, kw):
     r = H.lanczos(lam=200, seed=1, tol=1e-10)
     assert r["converged"] == 1
     assert r["E0"] == pytest.approx(w[0], rel=1e-10)
-    assert r["E_rq"] == pytest.approx(w[0], rel=1e-9) or (w[1] - w[0]) < 1e-6
+    if np.abs(val.imag).max() < 1e-14 and w[1] - w[0] > 1e-6 * abs(w[0]):
+        # real sector: the reference's printed Rayleigh quotient is E0
+        assert r["E_rq"] == pytest.approx(w[0], rel=1e-10)
     if w[1] - w[0] > 1e-6 * abs(w[0]):
         # psi = sum c_l conj(v_l): the reference's conjugated convention
         ov = abs(np.vdot(np.conj(V[:, 0]), r["psi"]))
         assert ov >= 1 - 1e-10
         printed = np.real(np.conj(V[:, 0]) @ Hd.T @ V[:, 0])
-        assert r["E_rq"] == pytest.approx(printed, rel=1e-10)
+        # For a complex sector conj(u0) is not an eigenvector, so this printed
+        # quantity is first-order (not second-order) in the state's error:
+        # with |dE| < 1e-10 the state error is ~1e-5, hence 1e-6 there.
+        complex_sector = np.abs(val.imag).max() > 1e-14
+        assert r["E_rq"] == pytest.approx(printed, rel=1e-6 if complex_sector else 1e-10)
 
 
 def test_readme_known_answers():
@@ -93,7 +99,8 @@ def test_readme_known_answers():
         print(f"README d={d}", flush=True)
         B, H = _build((2, 8, 8, 4, -1, 4, 4), dict(d=d))
         assert H.n == 79
-        r = H.lanczos(lam=200, seed=7, tol=1e-10)
+        # amplitudes are printed to 10 digits: converge the state that far
+        r = H.lanczos(lam=200, seed=7, tol=1e-10, res_tol=1e-12)
         assert float("%.10g" % (r["E_rq"] / 8)) == e_lz
         got = np.sort(np.abs(r["psi"]))[::-1][:len(amps)]
         np.testing.assert_allclose(got, amps, atol=6e-11)
