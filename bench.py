@@ -93,7 +93,14 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--lam", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run the plain copy kernel over this many MiB (hardware counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration_mib:
+        import blqh_amd
+        blqh_amd.copy_calibration(args.copy_calibration_mib << 20)
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration_mib << 20, "reps": 5}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE={world}); "
@@ -160,7 +167,7 @@ def main():
             "E0_relative_spread_over_seeds": spread,
             "basis_s": round(t_basis, 2),
             "hamiltonian_build_s": round(t_build, 3),
-            "spmv_lanes_per_row": None,
+            "spmv_variant": dict(zip(("kind", "lanes", "threads", "wgs_per_cu", "rows"), H.get_spmv())),
         },
         "roofline": {
             "bound": "hbm",

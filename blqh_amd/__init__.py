@@ -63,6 +63,8 @@ _SIGS = {
     "blqh_hamil_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "blqh_hamil_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "blqh_hamil_destroy": (None, [C.c_void_p]),
+    "blqh_hamil_get_spmv": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int32)] * 5),
+    "blqh_hamil_set_spmv": (C.c_int, [C.c_void_p] + [C.c_int32] * 5),
     "blqh_spmv": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "blqh_spmv_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "blqh_lanczos": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.POINTER(LanczosResult), C.c_void_p,
@@ -70,10 +72,29 @@ _SIGS = {
     "blqh_dense_ground_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]),
     "blqh_start_vector": (C.c_int, [C.c_int64, C.c_uint32, C.c_void_p]),
     "blqh_tridiag_lowest": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
+    "blqh_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "blqh_device_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64]),
+    "blqh_device_free": (C.c_int, [C.c_void_p]),
+    "blqh_synchronize": (C.c_int, []),
     "blqh_profile_enable": (C.c_int, [C.c_int]),
     "blqh_profile_read": (C.c_int, [C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "blqh_profile_reset": (C.c_int, []),
 }
+
+
+def copy_calibration(nbytes, reps=5):
+    """Run the plain copy kernel `reps` times over nbytes (counter calibration)."""
+    a = C.c_void_p()
+    b = C.c_void_p()
+    _check(lib().blqh_device_alloc(C.byref(a), nbytes))
+    _check(lib().blqh_device_alloc(C.byref(b), nbytes))
+    try:
+        for _ in range(reps):
+            _check(lib().blqh_device_copy(a, b, nbytes))
+        _check(lib().blqh_synchronize())
+    finally:
+        lib().blqh_device_free(a)
+        lib().blqh_device_free(b)
 
 
 def profile_enable(on=True):
@@ -193,6 +214,15 @@ class Hamiltonian:
         val = np.empty(self.nnz, np.complex128)
         _check(lib().blqh_hamil_export(self.h, _p(rowptr), _p(col), _p(val)))
         return rowptr, col, val
+
+    def get_spmv(self):
+        """(kind, lanes, threads, wgs_per_cu, rows) of the SpMV variant in use (kind -1: not tuned yet)."""
+        v = [C.c_int32() for _ in range(5)]
+        _check(lib().blqh_hamil_get_spmv(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def set_spmv(self, kind, lanes=16, threads=1024, wgs_per_cu=1, rows=2):
+        _check(lib().blqh_hamil_set_spmv(self.h, kind, lanes, threads, wgs_per_cu, rows))
 
     def spmv(self, x):
         x = np.ascontiguousarray(x, np.complex128)

@@ -15,6 +15,7 @@ void blqh_spmv_device_impl(blqh_hamil* H, const void* x, void* y);
 void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_result& r, double* psi, double* alpha,
                        double* beta);
 void blqh_dense_ground_state_impl(const blqh_hamil* H, double* e0, double* e_printed, double* vec);
+void blqh_device_copy_impl(const void* src, void* dst, int64_t bytes);
 namespace blqh {
 void start_vector(int64_t n, uint32_t seed, std::vector<double2>& v);
 double tridiag_lowest(int m, const double* a, const double* b, double* vec);
@@ -147,6 +148,37 @@ int blqh_hamil_export(const blqh_hamil* h, int64_t* rowptr, int32_t* col, double
 
 void blqh_hamil_destroy(blqh_hamil* h) { delete h; }
 
+int blqh_hamil_get_spmv(const blqh_hamil* h, int32_t* kind, int32_t* lanes, int32_t* threads, int32_t* wpc,
+                        int32_t* rows) {
+    if (!h) return fail(BLQH_EINVAL, "null hamiltonian");
+    if (kind) *kind = h->spmv_kind;
+    if (lanes) *lanes = h->spmv_lpr;
+    if (threads) *threads = h->spmv_threads;
+    if (wpc) *wpc = h->spmv_wgs_per_cu;
+    if (rows) *rows = h->spmv_rows;
+    return BLQH_OK;
+}
+
+int blqh_hamil_set_spmv(blqh_hamil* h, int32_t kind, int32_t lanes, int32_t threads, int32_t wpc, int32_t rows) {
+    if (!h) return fail(BLQH_EINVAL, "null hamiltonian");
+    bool ok = false;
+    if (kind == -1) ok = true;
+    if (kind == 0) ok = lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64;
+    if (kind == 1)
+        ok = h->pan.built && (lanes == 4 || lanes == 8 || lanes == 16) && (threads == 512 || threads == 1024) &&
+             (wpc == 1 || wpc == 2) && (rows == 2 || rows == 4);
+    if (kind == 2)
+        ok = h->pan.built && lanes == 64 && (threads == 512 || threads == 1024) && wpc >= 1 && wpc <= 4 &&
+             (rows == 1 || rows == 2 || rows == 4 || rows == 8);
+    if (!ok) return fail(BLQH_EINVAL, "unsupported SpMV variant");
+    h->spmv_kind = kind;
+    h->spmv_lpr = kind < 0 ? 0 : lanes;
+    h->spmv_threads = kind >= 1 ? threads : 256;
+    h->spmv_wgs_per_cu = kind >= 1 ? wpc : 0;
+    h->spmv_rows = kind >= 1 ? rows : 0;
+    return BLQH_OK;
+}
+
 int blqh_spmv(const blqh_hamil* h, const double* x, double* y) {
     if (!h || !x || !y) return fail(BLQH_EINVAL, "null argument");
     return guarded([&] {
@@ -176,6 +208,27 @@ int blqh_lanczos(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_r
 int blqh_dense_ground_state(const blqh_hamil* h, double* e0, double* e_printed, double* vec) {
     if (!h) return fail(BLQH_EINVAL, "null hamiltonian");
     return guarded([&] { blqh_dense_ground_state_impl(h, e0, e_printed, vec); });
+}
+
+int blqh_device_copy(const void* src, void* dst, int64_t bytes) {
+    if (!src || !dst) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] { blqh_device_copy_impl(src, dst, bytes); });
+}
+
+int blqh_device_alloc(void** ptr, int64_t bytes) {
+    if (!ptr || bytes <= 0) return fail(BLQH_EINVAL, "bad argument");
+    return guarded([&] {
+        blqh::runtime();
+        BLQH_HIP(hipMalloc(ptr, (size_t)bytes));
+    });
+}
+
+int blqh_device_free(void* ptr) {
+    return guarded([&] { if (ptr) BLQH_HIP(hipFree(ptr)); });
+}
+
+int blqh_synchronize(void) {
+    return guarded([&] { BLQH_HIP(hipStreamSynchronize(blqh::stream())); });
 }
 
 int blqh_profile_enable(int on) {

@@ -7,6 +7,7 @@
 
 #include "bits.hpp"
 #include "common.hpp"
+#include "panel.hpp"
 #include "rowterms.hpp"
 
 struct blqh_basis {
@@ -31,7 +32,12 @@ struct blqh_hamil {
     blqh::DevBuf<int64_t> rowptr;
     blqh::DevBuf<int32_t> col;
     blqh::DevBuf<double2> val;
-    int spmv_lpr = 0;                // lanes per row chosen by the autotuner (0 = not tuned)
+    blqh::Panels pan;                // panel-major copy for the SpMV hot loop
+    int spmv_kind = -1;              // autotuned SpMV: 0 = row-major CSR, 1 = panel-major (-1 = not tuned)
+    int spmv_lpr = 0;                // lanes per row of the chosen kernel
+    int spmv_threads = 0;            // workgroup size of the panel kernel
+    int spmv_wgs_per_cu = 0;         // persistent workgroups per CU (panel kernel)
+    int spmv_rows = 0;               // rows per lane group in flight (panel kernel)
     double seconds_build = 0;
 };
 
@@ -60,6 +66,7 @@ struct HostTables {
     double Ec = 0;
 };
 HostTables host_tables(const blqh_sector& s, const blqh_params& p);
+void build_panels(blqh_hamil* H);
 
 // Row context with the basis' device pointers (tables left null).
 inline RowCtx make_row_ctx(const blqh_basis* B) {

@@ -388,5 +388,6 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil
     BLQH_HIP(hipStreamSynchronize(st));
     // the small-hash fill pass reports exactly the overflow rows as errors
     BLQH_REQUIRE(e1[2] == novf && e2[2] == 0, BLQH_EINTERNAL, "count and fill passes disagree");
-    H->seconds_build = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    build_panels(H);
+    H->seconds_build =std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }

@@ -19,6 +19,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
+#include <cstdlib>
 #include <random>
 
 #include "engine.hpp"
@@ -42,7 +44,7 @@ __device__ __forceinline__ double2 ntload(const double2* p) {
 // order and hands the total to `fin`.
 template <class Fin>
 __device__ __forceinline__ void grid_reduce(double v, double* partials, unsigned* ticket, Fin fin) {
-    __shared__ double sh[kThreads / 64];
+    __shared__ double sh[16];  // up to 1024 threads
     __shared__ bool last;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
@@ -223,6 +225,382 @@ __global__ void __launch_bounds__(kThreads) k_redot(int64_t n, const double2* __
     grid_reduce(acc, partials, ticket, [&](double s) { *out = s; });
 }
 
+__global__ void __launch_bounds__(kThreads) k_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                    int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+        dst[i] = src[i];
+}
+
+// ---- panel-major SpMV ---------------------------------------------------------------
+// Persistent kernel: workgroup b owns rows [b*rows_per_wg, ...), walks the panels
+// in order with its rows' partial sums in LDS (chunk rows at a time), then runs
+// the epilogue.  LZ: the fused Lanczos K1 epilogue, x = W.
+__device__ __forceinline__ double ntload1(const double* p) { return __builtin_nontemporal_load(p); }
+
+template <bool REAL>
+__device__ __forceinline__ void seg_fma(const PanelView& P, const double2* __restrict__ xp, int64_t k, double& re,
+                                        double& im) {
+    const int c = __builtin_nontemporal_load(&P.pcol[k]);
+    if (REAL) {
+        const double v = ntload1(&P.pvr[k]);
+        const double2 xv = xp[c];
+        re += v * xv.x;
+        im += v * xv.y;
+    } else {
+        const double2 v = ntload(&P.pvc[k]);
+        const double2 xv = xp[c];
+        re += v.x * xv.x - v.y * xv.y;
+        im += v.x * xv.y + v.y * xv.x;
+    }
+}
+
+// U rows per lane group at a time (independent loads in flight), the next
+// group's row offsets prefetched while the current one computes.
+template <int LPR, int PT, int U, bool REAL, bool LZ>
+__global__ void __launch_bounds__(PT) k_pspmv(PanelView P, const double2* __restrict__ x, double2* __restrict__ out,
+                                              int64_t rows_per_wg, int chunk, const double2* __restrict__ Wp, Scal sc,
+                                              int j, double* partials, unsigned* ticket) {
+    extern __shared__ double2 yl[];
+    constexpr int NSUB = PT / LPR;
+    const int tid = threadIdx.x, sub = tid / LPR, lane = tid % LPR;
+    const int64_t n = P.n;
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_wg;
+    const int64_t r1 = r0 + rows_per_wg < n ? r0 + rows_per_wg : n;
+    double ib = 0, cp = 0;
+    if (LZ) {
+        ib = sc.invb[j];
+        cp = j > 0 ? sc.beta[j] * sc.invb[j - 1] : 0.0;
+    }
+    double dacc = 0.0;
+    for (int64_t c0 = r0; c0 < r1; c0 += chunk) {
+        const int nr = (int)(r1 - c0 < chunk ? r1 - c0 : chunk);
+        for (int i = tid; i < nr; i += PT) yl[i] = make_double2(0.0, 0.0);
+        __syncthreads();
+        for (int p = 0; p < P.npan; ++p) {
+            const uint32_t* rp = P.prow + (int64_t)p * (n + 1) + c0;
+            const int64_t base = P.pbase[p];
+            const double2* xp = x + (int64_t)p * kPanelWidth;
+            int rb = sub * U;
+            uint32_t s[U], e[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int r = rb + u;
+                s[u] = r < nr ? rp[r] : 0u;
+                e[u] = r < nr ? rp[r + 1] : 0u;
+            }
+            while (rb < nr) {
+                const int nb = rb + NSUB * U;
+                uint32_t sn[U], en[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int r = nb + u;
+                    sn[u] = r < nr ? rp[r] : 0u;
+                    en[u] = r < nr ? rp[r + 1] : 0u;
+                }
+                double re[U], im[U];
+                uint32_t mx = 0;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    re[u] = 0.0;
+                    im[u] = 0.0;
+                    mx = e[u] - s[u] > mx ? e[u] - s[u] : mx;
+                }
+                for (uint32_t off = lane; off < mx; off += LPR) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (s[u] + off < e[u]) seg_fma<REAL>(P, xp, base + s[u] + off, re[u], im[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+#pragma unroll
+                    for (int o = LPR / 2; o > 0; o >>= 1) {
+                        re[u] += __shfl_xor(re[u], o, LPR);
+                        im[u] += __shfl_xor(im[u], o, LPR);
+                    }
+                }
+                if (lane == 0) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (rb + u < nr) {
+                            double2 t = yl[rb + u];
+                            t.x += re[u];
+                            t.y += im[u];
+                            yl[rb + u] = t;
+                        }
+                }
+                rb = nb;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    s[u] = sn[u];
+                    e[u] = en[u];
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < nr; i += PT) {
+            const int64_t row = c0 + i;
+            const double2 yv = yl[i];
+            if (LZ) {
+                double ur = ib * yv.x, ui = ib * yv.y;
+                if (j > 0) {
+                    const double2 q = Wp[row];
+                    ur -= cp * q.x;
+                    ui -= cp * q.y;
+                }
+                out[row] = make_double2(ur, ui);
+                const double2 w = x[row];
+                dacc += w.x * ur + w.y * ui;
+            } else {
+                out[row] = yv;
+            }
+        }
+        __syncthreads();
+    }
+    if (LZ) grid_reduce(dacc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
+}
+
+// ---- slice-streaming panel SpMV ------------------------------------------------------
+// A wavefront owns a slice of 64 consecutive rows (lane l <-> row l) and keeps
+// their sums in registers across all panels.  Within a panel the slice's
+// entries are one contiguous run; it is streamed in windows of 64 entries (one
+// per lane, fully coalesced), the products are prefix-summed across the
+// wavefront, and each row lane adds P[last] - P[first-1] of its own segment's
+// part of the window.  Workgroups of WPB wavefronts sweep the panels together
+// (one barrier per panel) and the persistent grid walks slice groups in the
+// same order, so the active x panel stays resident in L2.
+__device__ __forceinline__ double wave_incl_scan(double v, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
+// Wavefront `wid` (of W in the grid) owns the slices wid, wid + W, wid + 2W,
+// ... (up to MAXS per round), spread uniformly over the matrix: nearby rows put
+// their entries in similar panels, so interleaving gives every workgroup about
+// the average work per panel and the grid sweeps the panels in step.
+template <bool REAL, bool LZ, int WPB, int MAXS>
+__global__ void __launch_bounds__(WPB * 64) k_sps(PanelView P, const double2* __restrict__ x,
+                                                  double2* __restrict__ out, const double2* __restrict__ Wp, Scal sc,
+                                                  int j, double* partials, unsigned* ticket) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t n = P.n;
+    const int64_t nslices = (n + 63) >> 6;
+    const int64_t W = (int64_t)gridDim.x * WPB;
+    const int64_t wid = (int64_t)blockIdx.x * WPB + wv;
+    double ib = 0, cp = 0;
+    if (LZ) {
+        ib = sc.invb[j];
+        cp = j > 0 ? sc.beta[j] * sc.invb[j - 1] : 0.0;
+    }
+    __shared__ double2 yl[WPB * MAXS * 64];  // row sums of this wavefront's slices
+    double2* my = yl + wv * MAXS * 64;
+    double dacc = 0.0;
+    for (int64_t r0 = 0; r0 < nslices; r0 += W * MAXS) {
+        for (int k = 0; k < MAXS; ++k) my[k * 64 + lane] = make_double2(0.0, 0.0);
+        for (int p = 0; p < P.npan; ++p) {
+            __syncthreads();  // keep the workgroup on one panel
+            const uint32_t* rp = P.prow + (int64_t)p * (n + 1);
+            const int64_t base = P.pbase[p];
+            const double2* xp = x + (int64_t)p * kPanelWidth;
+#pragma unroll 1
+            for (int k = 0; k < MAXS; ++k) {
+                const int64_t slice = r0 + wid + (int64_t)k * W;
+                if (slice >= nslices) continue;  // wave-uniform
+                const int64_t row0 = slice * 64, myrow = row0 + lane;
+                const int64_t rend = row0 + 64 < n ? row0 + 64 : n;
+                const uint32_t s = myrow < n ? rp[myrow] : rp[rend];
+                const uint32_t e = myrow < n ? rp[myrow + 1] : rp[rend];
+                const uint32_t E0 = __shfl(s, 0, 64);
+                const uint32_t E1 = __shfl(e, 63, 64);
+                double yr = 0.0, yi = 0.0;
+                for (uint32_t K = E0; K < E1; K += 64) {
+                    const uint32_t kk = K + lane;
+                    double pr = 0.0, pi = 0.0;
+                    if (kk < E1) {
+                        const int c = __builtin_nontemporal_load(&P.pcol[base + kk]);
+                        const double2 xv = xp[c];
+                        if (REAL) {
+                            const double v = ntload1(&P.pvr[base + kk]);
+                            pr = v * xv.x;
+                            pi = v * xv.y;
+                        } else {
+                            const double2 v = ntload(&P.pvc[base + kk]);
+                            pr = v.x * xv.x - v.y * xv.y;
+                            pi = v.x * xv.y + v.y * xv.x;
+                        }
+                    }
+                    const double Pr = wave_incl_scan(pr, lane), Pi = wave_incl_scan(pi, lane);
+                    // this row's lanes in the window: [a, b]
+                    const int a = (int)(s > K ? s - K : 0);
+                    const int b = (int)((e < K + 64 ? e : K + 64) - K) - 1;
+                    const bool mine = s < e && s < K + 64 && e > K;
+                    const int bl = mine ? b : 0, al = mine && a > 0 ? a - 1 : 0;
+                    const double hr = __shfl(Pr, bl, 64), hi = __shfl(Pi, bl, 64);
+                    const double lr = __shfl(Pr, al, 64), li = __shfl(Pi, al, 64);
+                    if (mine) {
+                        yr += a > 0 ? hr - lr : hr;
+                        yi += a > 0 ? hi - li : hi;
+                    }
+                }
+                double2 t = my[k * 64 + lane];
+                t.x += yr;
+                t.y += yi;
+                my[k * 64 + lane] = t;
+            }
+        }
+        for (int k = 0; k < MAXS; ++k) {
+            const int64_t slice = r0 + wid + (int64_t)k * W;
+            const int64_t myrow = slice * 64 + lane;
+            if (slice >= nslices || myrow >= n) continue;
+            const double2 yv = my[k * 64 + lane];
+            if (LZ) {
+                double ur = ib * yv.x, ui = ib * yv.y;
+                if (j > 0) {
+                    const double2 q = Wp[myrow];
+                    ur -= cp * q.x;
+                    ui -= cp * q.y;
+                }
+                out[myrow] = make_double2(ur, ui);
+                const double2 w = x[myrow];
+                dacc += w.x * ur + w.y * ui;
+            } else {
+                out[myrow] = yv;
+            }
+        }
+    }
+    if (LZ) grid_reduce(dacc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
+}
+
+// ---- row-per-lane panel SpMV with LDS staging ---------------------------------------
+// Same slice ownership as k_sps.  For each (slice, panel) run the wavefront
+// copies the run's columns and values into its LDS staging area with
+// coalesced loads (CH entries at a time); then every lane walks its own row's
+// segment out of LDS, gathering x and accumulating in registers, four
+// entries per step.  No cross-lane arithmetic, each row summed in column order.
+template <bool REAL, bool LZ, int WPB, int MAXS>
+__global__ void __launch_bounds__(WPB * 64) k_rpl(PanelView P, const double2* __restrict__ x,
+                                                  double2* __restrict__ out, const double2* __restrict__ Wp, Scal sc,
+                                                  int j, double* partials, unsigned* ticket) {
+    constexpr int CH = 256;
+    constexpr int VW = REAL ? 1 : 2;  // doubles per value
+    __shared__ double2 yl[WPB * MAXS * 64];
+    __shared__ uint16_t scol[WPB * CH];
+    __shared__ double sval[WPB * CH * VW];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint16_t* mc = scol + wv * CH;
+    double* mv = sval + wv * CH * VW;
+    double2* my = yl + wv * MAXS * 64;
+    const int64_t n = P.n;
+    const int64_t nslices = (n + 63) >> 6;
+    const int64_t W = (int64_t)gridDim.x * WPB;
+    const int64_t wid = (int64_t)blockIdx.x * WPB + wv;
+    double ib = 0, cp = 0;
+    if (LZ) {
+        ib = sc.invb[j];
+        cp = j > 0 ? sc.beta[j] * sc.invb[j - 1] : 0.0;
+    }
+    double dacc = 0.0;
+    for (int64_t r0 = 0; r0 < nslices; r0 += W * MAXS) {
+        for (int k = 0; k < MAXS; ++k) my[k * 64 + lane] = make_double2(0.0, 0.0);
+        for (int p = 0; p < P.npan; ++p) {
+            __syncthreads();  // keep the workgroup on one panel
+            const uint32_t* rp = P.prow + (int64_t)p * (n + 1);
+            const int64_t base = P.pbase[p];
+            const double2* xp = x + (int64_t)p * kPanelWidth;
+#pragma unroll 1
+            for (int k = 0; k < MAXS; ++k) {
+                const int64_t slice = r0 + wid + (int64_t)k * W;
+                if (slice >= nslices) continue;  // wave-uniform
+                const int64_t row0 = slice * 64, myrow = row0 + lane;
+                const int64_t rend = row0 + 64 < n ? row0 + 64 : n;
+                const uint32_t s = myrow < n ? rp[myrow] : rp[rend];
+                const uint32_t e = myrow < n ? rp[myrow + 1] : rp[rend];
+                const uint32_t E0 = __shfl(s, 0, 64);
+                const uint32_t E1 = __shfl(e, 63, 64);
+                double ar = 0.0, ai = 0.0;
+                for (uint32_t C0 = E0; C0 < E1; C0 += CH) {
+                    const uint32_t C1 = C0 + CH < E1 ? C0 + CH : E1;
+                    for (uint32_t t = C0 + lane; t < C1; t += 64) {
+                        mc[t - C0] = __builtin_nontemporal_load(&P.pcol[base + t]);
+                        if (REAL) {
+                            mv[t - C0] = ntload1(&P.pvr[base + t]);
+                        } else {
+                            const double2 v = ntload(&P.pvc[base + t]);
+                            mv[2 * (t - C0)] = v.x;
+                            mv[2 * (t - C0) + 1] = v.y;
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const int lo = (int)((s > C0 ? s : C0) - C0);
+                    const int hi = (int)((e < C1 ? e : C1) - C0);
+                    int t = lo;
+                    for (; t + 4 <= hi; t += 4) {
+                        double2 xv[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) xv[u] = xp[mc[t + u]];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            if (REAL) {
+                                const double v = mv[t + u];
+                                ar += v * xv[u].x;
+                                ai += v * xv[u].y;
+                            } else {
+                                const double vr = mv[2 * (t + u)], vi = mv[2 * (t + u) + 1];
+                                ar += vr * xv[u].x - vi * xv[u].y;
+                                ai += vr * xv[u].y + vi * xv[u].x;
+                            }
+                        }
+                    }
+                    for (; t < hi; ++t) {
+                        const double2 xv = xp[mc[t]];
+                        if (REAL) {
+                            const double v = mv[t];
+                            ar += v * xv.x;
+                            ai += v * xv.y;
+                        } else {
+                            const double vr = mv[2 * t], vi = mv[2 * t + 1];
+                            ar += vr * xv.x - vi * xv.y;
+                            ai += vr * xv.y + vi * xv.x;
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+                double2 tacc = my[k * 64 + lane];
+                tacc.x += ar;
+                tacc.y += ai;
+                my[k * 64 + lane] = tacc;
+            }
+        }
+        for (int k = 0; k < MAXS; ++k) {
+            const int64_t slice = r0 + wid + (int64_t)k * W;
+            const int64_t myrow = slice * 64 + lane;
+            if (slice >= nslices || myrow >= n) continue;
+            const double2 yv = my[k * 64 + lane];
+            if (LZ) {
+                double ur = ib * yv.x, ui = ib * yv.y;
+                if (j > 0) {
+                    const double2 q = Wp[myrow];
+                    ur -= cp * q.x;
+                    ui -= cp * q.y;
+                }
+                out[myrow] = make_double2(ur, ui);
+                const double2 w = x[myrow];
+                dacc += w.x * ur + w.y * ui;
+            } else {
+                out[myrow] = yv;
+            }
+        }
+    }
+    if (LZ) grid_reduce(dacc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
+}
+
 // ---- launch helpers -----------------------------------------------------------
 int vec_grid(int64_t n) {
     const int ncu = runtime().num_cus;
@@ -244,39 +622,184 @@ int spmv_grid(int64_t n, int lpr) {
         default: hipLaunchKernelGGL(KERNEL<64>, __VA_ARGS__); break; \
     }
 
-void launch_spmv(const blqh_hamil* H, int lpr, const double2* x, double2* y, hipStream_t st) {
-    BLQH_LPR_SWITCH(lpr, k_spmv, dim3(spmv_grid(H->n, lpr)), dim3(kThreads), 0, st, H->n, H->rowptr.get(),
-                    H->col.get(), H->val.get(), x, y);
+// SpMV variant: row-major CSR with `lpr` lanes per row, or the panel kernel
+// with `lpr` lanes per row segment, `threads` per workgroup and `wpc`
+// persistent workgroups per CU.
+struct Plan {
+    int kind, lpr, threads, wpc, rows;  // rows: rows per lane group in flight (panel kernel)
+};
+
+struct PanelLaunch {
+    int grid;
+    int64_t rows_per_wg;
+    int chunk;
+    size_t smem;
+};
+
+PanelLaunch panel_launch(const blqh_hamil* H, const Plan& pl) {
+    const int ncu = runtime().num_cus;
+    PanelLaunch L;
+    L.grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ncu * pl.wpc, ceil_div(H->n, 64)));
+    L.rows_per_wg = ceil_div(H->n, L.grid);
+    const int64_t lds_rows = (pl.wpc == 1 ? 144 * 1024 : 72 * 1024) / (int64_t)sizeof(double2);
+    L.chunk = (int)std::max<int64_t>(1, std::min<int64_t>(L.rows_per_wg, lds_rows));
+    L.smem = (size_t)L.chunk * sizeof(double2);
+    return L;
 }
 
-// Pick lanes-per-row by timing each variant on this matrix.
-int autotune_lpr(blqh_hamil* H) {
-    if (H->spmv_lpr) return H->spmv_lpr;
+template <int LPR, int PT, int U, bool REAL, bool LZ>
+void panel_go(const blqh_hamil* H, const PanelLaunch& L, const double2* x, double2* out, const double2* Wp,
+              const Scal& sc, int j, double* partials, unsigned* ticket, hipStream_t st) {
+    auto kern = k_pspmv<LPR, PT, U, REAL, LZ>;
+    static bool attr = false;
+    if (!attr) {
+        BLQH_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+        attr = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(L.grid), dim3(PT), L.smem, st, view(H->pan, H->n), x, out, L.rows_per_wg, L.chunk,
+                       Wp, sc, j, partials, ticket);
+}
+
+template <bool LZ>
+void panel_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2* out, const double2* Wp,
+                    const Scal& sc, int j, double* partials, unsigned* ticket, hipStream_t st) {
+    const PanelLaunch L = panel_launch(H, pl);
+    const bool real = H->pan.real;
+#define BLQH_PANEL_CASE(LPR, PT, U)                                                                  \
+    if (pl.lpr == LPR && pl.threads == PT && pl.rows == U) {                                          \
+        if (real) panel_go<LPR, PT, U, true, LZ>(H, L, x, out, Wp, sc, j, partials, ticket, st);     \
+        else panel_go<LPR, PT, U, false, LZ>(H, L, x, out, Wp, sc, j, partials, ticket, st);         \
+        return;                                                                                       \
+    }
+    BLQH_PANEL_CASE(4, 512, 2) BLQH_PANEL_CASE(8, 512, 2) BLQH_PANEL_CASE(16, 512, 2)
+    BLQH_PANEL_CASE(4, 1024, 2) BLQH_PANEL_CASE(8, 1024, 2) BLQH_PANEL_CASE(16, 1024, 2)
+    BLQH_PANEL_CASE(4, 512, 4) BLQH_PANEL_CASE(8, 512, 4) BLQH_PANEL_CASE(16, 512, 4)
+    BLQH_PANEL_CASE(4, 1024, 4) BLQH_PANEL_CASE(8, 1024, 4) BLQH_PANEL_CASE(16, 1024, 4)
+#undef BLQH_PANEL_CASE
+    throw Error(BLQH_EINTERNAL, "unsupported panel SpMV plan");
+}
+
+Plan plan_of(const blqh_hamil* H) {
+    return Plan{H->spmv_kind, H->spmv_lpr, H->spmv_threads, H->spmv_wgs_per_cu, H->spmv_rows};
+}
+
+int sps_grid(const blqh_hamil* H, const Plan& pl) {
+    const int64_t groups = ceil_div(ceil_div(H->n, 64), (int64_t)(pl.threads / 64) * pl.rows);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(groups, (int64_t)runtime().num_cus * pl.wpc));
+}
+
+template <bool LZ>
+void sps_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2* out, const double2* Wp,
+                  const Scal& sc, int j, double* partials, unsigned* ticket, hipStream_t st) {
+    const PanelView V = view(H->pan, H->n);
+    const dim3 grid(sps_grid(H, pl)), block(pl.threads);
+#define BLQH_SPS_CASE(WPB, MAXS)                                                                            \
+    if (pl.threads == WPB * 64 && pl.rows == MAXS) {                                                        \
+        if (H->pan.real)                                                                                    \
+            hipLaunchKernelGGL((k_sps<true, LZ, WPB, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_sps<false, LZ, WPB, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+        return;                                                                                             \
+    }
+    BLQH_SPS_CASE(8, 1) BLQH_SPS_CASE(8, 2) BLQH_SPS_CASE(8, 4) BLQH_SPS_CASE(8, 8)
+    BLQH_SPS_CASE(16, 1) BLQH_SPS_CASE(16, 2) BLQH_SPS_CASE(16, 4) BLQH_SPS_CASE(16, 8)
+#undef BLQH_SPS_CASE
+    throw Error(BLQH_EINTERNAL, "unsupported slice-streaming SpMV plan");
+}
+
+void launch_spmv_plan(const blqh_hamil* H, const Plan& pl, const double2* x, double2* y, hipStream_t st) {
+    if (pl.kind == 2) {
+        sps_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
+    } else if (pl.kind == 1) {
+        panel_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
+    } else {
+        BLQH_LPR_SWITCH(pl.lpr, k_spmv, dim3(spmv_grid(H->n, pl.lpr)), dim3(kThreads), 0, st, H->n,
+                        H->rowptr.get(), H->col.get(), H->val.get(), x, y);
+    }
+}
+
+// grid size (for partial-sum buffers) of the fused K1 under a plan
+int lz1_grid(const blqh_hamil* H, const Plan& pl) {
+    return pl.kind == 2 ? sps_grid(H, pl) : pl.kind == 1 ? panel_launch(H, pl).grid : spmv_grid(H->n, pl.lpr);
+}
+
+void launch_lz1(const blqh_hamil* H, const Plan& pl, const double2* W, const double2* Wp, double2* u, const Scal& sc,
+                int j, double* partials, unsigned* ticket, hipStream_t st) {
+    if (pl.kind == 2) {
+        sps_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
+    } else if (pl.kind == 1) {
+        panel_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
+    } else {
+        BLQH_LPR_SWITCH(pl.lpr, k_lz1, dim3(spmv_grid(H->n, pl.lpr)), dim3(kThreads), 0, st, H->n,
+                        H->rowptr.get(), H->col.get(), H->val.get(), W, Wp, u, sc, j, partials, ticket);
+    }
+}
+
+// Autotuner: time every variant on this matrix (random x) and keep the fastest.
+void autotune(blqh_hamil* H) {
+    if (H->spmv_kind >= 0) return;
+    // BLQH_SPMV_PLAN="kind,lanes,threads,wgs_per_cu,rows" pins a variant (profiling, reproducibility)
+    if (const char* env = std::getenv("BLQH_SPMV_PLAN")) {
+        Plan p{};
+        if (std::sscanf(env, "%d,%d,%d,%d,%d", &p.kind, &p.lpr, &p.threads, &p.wpc, &p.rows) == 5 &&
+            (p.kind == 0 || H->pan.built)) {
+            H->spmv_kind = p.kind;
+            H->spmv_lpr = p.lpr;
+            H->spmv_threads = p.threads;
+            H->spmv_wgs_per_cu = p.wpc;
+            H->spmv_rows = p.rows;
+            return;
+        }
+    }
     hipStream_t st = stream();
     const double mean_row = H->n ? (double)H->nnz / H->n : 0;
+    std::vector<Plan> cands;
+    for (int lpr : {4, 8, 16, 32, 64})
+        if (!(lpr > 4 * mean_row + 8 && lpr > 8)) cands.push_back(Plan{0, lpr, kThreads, 0, 0});
+    if (H->pan.built)
+        for (int pt : {512, 1024})
+            for (int lpr : {4, 8, 16})
+                for (int wpc : {1, 2})
+                    for (int u : {2, 4}) cands.push_back(Plan{1, lpr, pt, wpc, u});
+    if (H->pan.built)
+        for (int pt : {512, 1024})
+            for (int wpc : {1, 2, 4}) {
+                // slices per wavefront: the smallest supported count that covers the matrix in one round
+                const int64_t waves = (int64_t)runtime().num_cus * wpc * (pt / 64);
+                const int64_t need = ceil_div(ceil_div(H->n, 64), waves);
+                int ms = 1;
+                while (ms < need && ms < 8) ms <<= 1;
+                cands.push_back(Plan{2, 64, pt, wpc, ms});
+            }
+    std::vector<double2> hx(H->n);
+    std::mt19937 rng(12345);
+    for (auto& z : hx) z = make_double2(rng() * 1.0 / rng.max() - 0.5, rng() * 1.0 / rng.max() - 0.5);
     DevBuf<double2> x(H->n), y(H->n);
-    BLQH_HIP(hipMemsetAsync(x.get(), 0, H->n * sizeof(double2), st));
+    BLQH_HIP(hipMemcpyAsync(x.get(), hx.data(), H->n * sizeof(double2), hipMemcpyHostToDevice, st));
     hipEvent_t e0, e1;
     BLQH_HIP(hipEventCreate(&e0));
     BLQH_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    int best_lpr = 32;
-    const int reps = H->nnz > 50000000 ? 2 : 5;
-    for (int lpr : {4, 8, 16, 32, 64}) {
-        if (lpr > 4 * mean_row + 8 && lpr > 8) continue;  // too wide for these rows
-        launch_spmv(H, lpr, x.get(), y.get(), st);        // warm
+    Plan bp = cands.front();
+    const int reps = H->nnz > 50000000 ? 3 : 10;
+    for (const Plan& pl : cands) {
+        launch_spmv_plan(H, pl, x.get(), y.get(), st);  // warm
         BLQH_HIP(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; ++r) launch_spmv(H, lpr, x.get(), y.get(), st);
+        for (int r = 0; r < reps; ++r) launch_spmv_plan(H, pl, x.get(), y.get(), st);
         BLQH_HIP(hipEventRecord(e1, st));
         BLQH_HIP(hipEventSynchronize(e1));
         float ms = 0;
         BLQH_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) { best = ms; best_lpr = lpr; }
+        if (ms < best) { best = ms; bp = pl; }
     }
+    BLQH_HIP(hipGetLastError());
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    H->spmv_lpr = best_lpr;
-    return best_lpr;
+    H->spmv_kind = bp.kind;
+    H->spmv_lpr = bp.lpr;
+    H->spmv_threads = bp.threads;
+    H->spmv_wgs_per_cu = bp.wpc;
+    H->spmv_rows = bp.rows;
 }
 
 }  // namespace
@@ -392,9 +915,17 @@ Profile& profile() {
 }  // namespace blqh
 
 // ---- public stage functions ------------------------------------------------------
+void blqh_device_copy_impl(const void* src, void* dst, int64_t bytes) {
+    BLQH_REQUIRE(bytes >= 0 && bytes % 16 == 0, BLQH_EINVAL, "copy size must be a multiple of 16 bytes");
+    const int64_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy, dim3(vec_grid(n)), dim3(kThreads), 0, blqh::stream(), (const double2*)src,
+                       (double2*)dst, n);
+    BLQH_HIP(hipGetLastError());
+}
+
 void blqh_spmv_device_impl(blqh_hamil* H, const void* x, void* y) {
-    const int lpr = autotune_lpr(H);
-    launch_spmv(H, lpr, (const double2*)x, (double2*)y, blqh::stream());
+    autotune(H);
+    launch_spmv_plan(H, plan_of(H), (const double2*)x, (double2*)y, blqh::stream());
     BLQH_HIP(hipGetLastError());
 }
 
@@ -407,7 +938,8 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
     const int cap = o.lambda;
     BLQH_REQUIRE(cap >= 1 && cap <= 100000, BLQH_EINVAL, "lambda out of range");
     const int chk = std::max(1, o.check_every);
-    const int lpr = autotune_lpr(H);
+    autotune(H);
+    const Plan plan = plan_of(H);
 
     // Krylov storage: keep all vectors when they fit comfortably
     size_t free_b = 0, total_b = 0;
@@ -417,7 +949,7 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
     const int nvec = store ? cap + 1 : 3;
     DevBuf<double2> Wb((size_t)nvec * n), u(n), psi(n);
     DevBuf<double> sc_buf(3 * (size_t)cap + 8), coef(cap + 1), red(4);
-    const int gmax = std::max(spmv_grid(n, lpr), vec_grid(n));
+    const int gmax = std::max(std::max(lz1_grid(H, plan), spmv_grid(n, 4)), vec_grid(n));
     DevBuf<double> partials(gmax);
     DevBuf<unsigned> ticket(1);
     BLQH_HIP(hipMemsetAsync(ticket.get(), 0, sizeof(unsigned), st));
@@ -441,7 +973,7 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
     auto slot = [&](int j) -> double2* { return Wb.get() + (size_t)(store ? j : j % 3) * n; };
     BLQH_HIP(hipMemcpyAsync(slot(0), v0.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
 
-    const int gs = spmv_grid(n, lpr), gv = vec_grid(n);
+    const int gv = vec_grid(n);
     Profile& prof = profile();
     int ntimed = 0;
     auto step = [&](int j, double2* psi_acc, const double* cf) {
@@ -455,8 +987,7 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
             }
             BLQH_HIP(hipEventRecord(prof.ev[2 * ntimed], st));
         }
-        BLQH_LPR_SWITCH(lpr, k_lz1, dim3(gs), dim3(kThreads), 0, st, n, H->rowptr.get(), H->col.get(),
-                        H->val.get(), W, Wp, u.get(), sc, j, partials.get(), ticket.get());
+        launch_lz1(H, plan, W, Wp, u.get(), sc, j, partials.get(), ticket.get(), st);
         if (prof.on) BLQH_HIP(hipEventRecord(prof.ev[2 * ntimed++ + 1], st));
         hipLaunchKernelGGL(k_lz2, dim3(gv), dim3(kThreads), 0, st, n, u.get(), W, slot(j + 1), sc, j, psi_acc, cf,
                            partials.get(), ticket.get());
@@ -536,7 +1067,7 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
     // normalise and Rayleigh quotient
     hipLaunchKernelGGL(k_norm2, dim3(gv), dim3(kThreads), 0, st, n, psi.get(), red.get(), partials.get(), ticket.get());
     hipLaunchKernelGGL(k_scale, dim3(gv), dim3(kThreads), 0, st, n, psi.get(), red.get());
-    launch_spmv(H, lpr, psi.get(), u.get(), st);
+    launch_spmv_plan(H, plan, psi.get(), u.get(), st);
     hipLaunchKernelGGL(k_redot, dim3(gv), dim3(kThreads), 0, st, n, psi.get(), u.get(), red.get() + 1, partials.get(),
                        ticket.get());
     BLQH_HIP(hipGetLastError());

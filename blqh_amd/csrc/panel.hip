@@ -1,0 +1,140 @@
+// panel.hip -- panel-major copy of the Hamiltonian for the SpMV hot loop.
+//
+// The CSR rows of a Hamiltonian touch ~350 columns spread over the whole
+// vector, so a row-major SpMV gathers x from everywhere: at 2.3M states the
+// 37 MB vector falls out of L2 and every 16-byte gather pulls a full line.
+// Here the columns are cut into panels of 65536 (x slice of 1 MB for complex
+// vectors, which stays resident in each XCD's 4 MB L2).  Panel p stores, for
+// every row, that row's entries with p*65536 <= col < (p+1)*65536, rows in
+// order, so each panel is one sequential stream:
+//   pbase[p]        first entry of panel p
+//   prow[p][r]      offset of row r's segment inside panel p (uint32, n+1 per panel)
+//   pcol[k]         col - p*65536 (uint16)
+//   pvr[k] / pvc[k] value (real double when the matrix is real, else double2)
+// The SpMV kernel is persistent: workgroup b owns a contiguous row range and
+// keeps those rows' partial sums in LDS while it walks the panels in order, so
+// all workgroups sweep the same panel at about the same time and y is written
+// once.  The matrix is stored real when every imaginary part is below 1e-14
+// of the largest entry (the K = 0 sectors are exactly real; K = nphi/2 sectors
+// carry sin(pi n) rounding residue of 1e-16).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "engine.hpp"
+#include "panel.hpp"
+
+namespace {
+
+using namespace blqh;
+
+constexpr int kPanelShift = 16;
+constexpr int kBlk = 256;
+
+// per row: entries per panel -> cnt[p * n + row]
+__global__ void k_panel_count(int64_t n, int npan, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                              int32_t* __restrict__ cnt) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        int p = 0, c = 0;
+        for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+            const int q = col[k] >> kPanelShift;
+            while (p < q) { cnt[(int64_t)p * n + r] = c; c = 0; ++p; }
+            ++c;
+        }
+        while (p < npan) { cnt[(int64_t)p * n + r] = c; c = 0; ++p; }
+    }
+}
+
+// off = exclusive scan of cnt over (panel, row): global destination of each segment
+__global__ void k_panel_scatter(int64_t n, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                const double2* __restrict__ val, const int64_t* __restrict__ off, int real,
+                                uint16_t* __restrict__ pcol, double* __restrict__ pvr, double2* __restrict__ pvc) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        int p = -1;
+        int64_t dst = 0;
+        for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+            const int q = col[k] >> kPanelShift;
+            if (q != p) { p = q; dst = off[(int64_t)q * n + r]; }
+            pcol[dst] = (uint16_t)(col[k] & 0xFFFF);
+            if (real) pvr[dst] = val[k].x; else pvc[dst] = val[k];
+            ++dst;
+        }
+    }
+}
+
+// prow[p][r] = off[p*n + r] - pbase[p] for r <= n
+__global__ void k_panel_rows(int64_t n, int npan, const int64_t* __restrict__ off, int64_t total,
+                             uint32_t* __restrict__ prow, int64_t* __restrict__ pbase) {
+    const int64_t m = (int64_t)npan * (n + 1);
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < m; t += (int64_t)gridDim.x * blockDim.x) {
+        const int p = (int)(t / (n + 1));
+        const int64_t r = t - (int64_t)p * (n + 1);
+        const int64_t base = off[(int64_t)p * n];
+        const int64_t g = (int64_t)p * n + r;  // r == n reads the next panel's first offset (or the total)
+        const int64_t v = g < (int64_t)npan * n ? off[g] : total;
+        prow[t] = (uint32_t)(v - base);
+        if (r == 0) pbase[p] = base;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) pbase[npan] = total;
+}
+
+// max |re| and max |im| of the values (two doubles via atomicMax on the bit pattern of non-negative doubles)
+__global__ void k_absmax(int64_t nnz, const double2* __restrict__ val, unsigned long long* __restrict__ out) {
+    double mr = 0, mi = 0;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
+        mr = fmax(mr, fabs(val[k].x));
+        mi = fmax(mi, fabs(val[k].y));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mr = fmax(mr, __shfl_xor(mr, o, 64));
+        mi = fmax(mi, __shfl_xor(mi, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out[0], (unsigned long long)__double_as_longlong(mr));
+        atomicMax(&out[1], (unsigned long long)__double_as_longlong(mi));
+    }
+}
+
+}  // namespace
+
+namespace blqh {
+
+void build_panels(blqh_hamil* H) {
+    hipStream_t st = stream();
+    const int64_t n = H->n;
+    const int npan = (int)std::max<int64_t>(1, ceil_div(n, (int64_t)1 << kPanelShift));
+    Panels& P = H->pan;
+    P.npan = npan;
+    DevBuf<unsigned long long> mx(2);
+    BLQH_HIP(hipMemsetAsync(mx.get(), 0, 16, st));
+    const int g = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(H->nnz, kBlk), 1), 65536);
+    hipLaunchKernelGGL(k_absmax, dim3(g), dim3(kBlk), 0, st, H->nnz, H->val.get(), mx.get());
+    unsigned long long hm[2];
+    BLQH_HIP(hipMemcpyAsync(hm, mx.get(), 16, hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    double mre, mim;
+    std::memcpy(&mre, &hm[0], 8);
+    std::memcpy(&mim, &hm[1], 8);
+    P.real = mim <= 1e-14 * std::max(mre, mim);
+    DevBuf<int32_t> cnt((size_t)npan * n);
+    DevBuf<int64_t> off((size_t)npan * n + 1);
+    const int gr = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, kBlk), 1), 65536);
+    hipLaunchKernelGGL(k_panel_count, dim3(gr), dim3(kBlk), 0, st, n, npan, H->rowptr.get(), H->col.get(), cnt.get());
+    BLQH_HIP(hipGetLastError());
+    exclusive_scan(cnt.get(), off.get(), (int64_t)npan * n, st);
+    P.prow.alloc((size_t)npan * (n + 1));
+    P.pbase.alloc(npan + 1);
+    const int gp = (int)std::min<int64_t>(ceil_div((int64_t)npan * (n + 1), kBlk), 65536);
+    hipLaunchKernelGGL(k_panel_rows, dim3(gp), dim3(kBlk), 0, st, n, npan, off.get(), H->nnz, P.prow.get(),
+                       P.pbase.get());
+    P.pcol.alloc(std::max<int64_t>(H->nnz, 1));
+    if (P.real) P.pvr.alloc(std::max<int64_t>(H->nnz, 1));
+    else P.pvc.alloc(std::max<int64_t>(H->nnz, 1));
+    hipLaunchKernelGGL(k_panel_scatter, dim3(gr), dim3(kBlk), 0, st, n, H->rowptr.get(), H->col.get(), H->val.get(),
+                       off.get(), P.real ? 1 : 0, P.pcol.get(), P.pvr.get(), P.pvc.get());
+    BLQH_HIP(hipGetLastError());
+    BLQH_HIP(hipStreamSynchronize(st));  // cnt / off are freed on return
+    P.built = true;
+}
+
+}  // namespace blqh

@@ -83,6 +83,23 @@ void blqh_hamil_destroy(blqh_hamil* h);
 int blqh_spmv(const blqh_hamil* h, const double* x, double* y);            /* host arrays, 2*nrows doubles */
 int blqh_spmv_device(const blqh_hamil* h, const void* x_dev, void* y_dev); /* device arrays, library stream */
 
+/* SpMV variant used for this matrix (engine tuning; no counterpart in the
+ * reference).  kind 0: row-major CSR with `lanes` lanes per row; kind 1:
+ * panel-major (65536-column panels, x slice resident in L2) with `lanes`
+ * lanes per row segment, `threads` per workgroup (512 or 1024) and
+ * `wgs_per_cu` persistent workgroups per CU (1 or 2) and `rows` rows per lane
+ * group in flight (2 or 4); kind 2: panel-major slice streaming (a wavefront
+ * owns 64-row slices spread over the matrix, lanes = 64) with `threads` 512
+ * or 1024, `wgs_per_cu` 1-4 and `rows` = slices per wavefront and round (1, 2,
+ * 4 or 8).
+ * get: kind = -1 until
+ * the autotuner has run (first SpMV or solve).  set: pins a variant (returns
+ * BLQH_EINVAL for an unsupported one); kind = -1 re-arms the autotuner. */
+int blqh_hamil_get_spmv(const blqh_hamil* h, int32_t* kind, int32_t* lanes, int32_t* threads, int32_t* wgs_per_cu,
+                        int32_t* rows);
+int blqh_hamil_set_spmv(blqh_hamil* h, int32_t kind, int32_t lanes, int32_t threads, int32_t wgs_per_cu,
+                        int32_t rows);
+
 /* ---- Lanczos: coeff_explicit_update + diag + eigenstates_reconstruction +
  *      ground_state_energy (BLQH_without_spin/lanczos_hamiltonian.cpp:453-656) ---- */
 typedef struct blqh_lanczos_opts {
@@ -136,6 +153,13 @@ int blqh_tridiag_lowest(int32_t m, const double* alpha, const double* offdiag, d
  * the launch count, the summed kernel time and the algorithmic bytes summed
  * over those launches (20 nnz + 56 n + 8 per launch: CSR values, columns,
  * row pointers, x read once, previous vector read, output written). */
+/* Plain device-to-device copy kernel of `bytes` bytes (multiple of 16) on the
+ * library stream: the reference point for scaling hardware byte counters. */
+int blqh_device_copy(const void* src_dev, void* dst_dev, int64_t bytes);
+/* Device allocation helpers for callers without a HIP runtime of their own. */
+int blqh_device_alloc(void** ptr, int64_t bytes);
+int blqh_device_free(void* ptr);
+int blqh_synchronize(void);
 int blqh_profile_enable(int on);
 int blqh_profile_read(int64_t* launches, double* kernel_ms, double* bytes);
 int blqh_profile_reset(void);

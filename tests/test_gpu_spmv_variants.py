@@ -1,0 +1,59 @@
+"""Every SpMV variant the autotuner can pick, on multi-panel sectors (more
+than 65536 states), against scipy's CSR product of the exported matrix (the
+CSR itself is pinned to the oracle elsewhere)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import blqh_amd
+
+pytestmark = pytest.mark.gpu
+
+PLANS = [(0, l, 256, 0, 0) for l in (4, 8, 16, 32, 64)] + \
+        [(1, l, t, w, u) for l in (4, 8, 16) for t in (512, 1024) for w in (1, 2) for u in (2, 4)] + \
+        [(2, 64, t, w, u) for t in (512, 1024) for w in (1, 2, 4) for u in (1, 2, 4, 8)]
+
+SECTORS = [
+    ("real_4panels", (2, 14, 14, -1, -1, 0, 0), dict(d=1.0, delta_sas=0.05)),
+    ("complex_4panels", (2, 14, 14, -1, -1, 0, 3), dict(d=0.5, delta_sas=0.05)),
+    ("halfK_stored_real", (2, 14, 14, 7, -1, 7, 7), dict(d=1.5)),
+]
+
+
+@pytest.fixture(scope="module", params=SECTORS, ids=[s[0] for s in SECTORS])
+def built(request):
+    name, (S, nphi, nel, nu, nsu, J, K), kw = request.param
+    print(f"sector {name}", flush=True)
+    B = blqh_amd.Basis(nphi, nel, nu, J, K, nspecies=S, nel_su=nsu)
+    H = blqh_amd.Hamiltonian(B, blqh_amd.params(nphi, **kw))
+    rp, col, val = H.export()
+    A = sp.csr_matrix((val, col, rp), shape=(H.n, H.n))
+    return name, B, H, A
+
+
+@pytest.mark.parametrize("plan", PLANS, ids=[f"k{p[0]}_l{p[1]}_t{p[2]}_w{p[3]}_u{p[4]}" for p in PLANS])
+def test_variant_matches_csr_product(built, plan):
+    name, B, H, A = built
+    print(f"{name} plan {plan}", flush=True)
+    H.set_spmv(*plan)
+    assert H.get_spmv()[:2] == plan[:2]
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=H.n) + 1j * rng.normal(size=H.n)
+    y = H.spmv(x)
+    ref = A @ x
+    scale = np.abs(A).max() * np.abs(x).max() * np.diff(A.indptr).max()
+    np.testing.assert_allclose(y, ref, rtol=0, atol=2e-15 * scale)
+
+
+def test_solver_independent_of_variant(built):
+    name, B, H, A = built
+    print(f"{name} solver variants", flush=True)
+    H.set_spmv(0, 16)
+    e_csr = H.lanczos(lam=300, seed=4, want_psi=False)["E0"]
+    H.set_spmv(1, 8, 1024, 1, 4)
+    e_pan = H.lanczos(lam=300, seed=4, want_psi=False)["E0"]
+    H.set_spmv(-1)
+    e_auto = H.lanczos(lam=300, seed=4, want_psi=False)["E0"]
+    assert e_pan == pytest.approx(e_csr, rel=1e-11)
+    assert e_auto == pytest.approx(e_csr, rel=1e-11)
+    assert H.get_spmv()[0] in (0, 1)
