@@ -27,6 +27,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "K-sector ground states/sec at |dE|<1e-10 (batched+warm vs plain); SpMV HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+# fused Lanczos K1 kernel of each SpMV variant (lanczos.hip)
+KERNEL_NAMES = {0: "k_lz1", 1: "k_pspmv<LZ=true>", 2: "k_sps<LZ=true>", 3: "k_jag<LZ=true>"}
 
 WORKLOADS = {
     # name: (nspecies, nphi, nel, nel_up, J, K, d, delta_sas, label)
@@ -176,7 +178,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_lz1 (fused complex-FP64 CSR SpMV + Lanczos epilogue)",
+            "kernel": KERNEL_NAMES.get(H.get_spmv()[0], "?") + " (fused complex-FP64 SpMV + Lanczos K1 epilogue)",
             "avg_launch_ms": avg_ms,
             "launches": launches,
             "algorithmic_bytes_per_launch": kbytes / max(launches, 1),

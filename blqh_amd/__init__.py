@@ -158,8 +158,9 @@ class Basis:
 
     def __init__(self, nphi, nel, nel_up=-1, J=-1, K=-1, nspecies=2, nel_su=-1):
         self.sector = Sector(nspecies, nphi, nel, nel_up, nel_su, J, K)
+        self._lib = lib()
         h = C.c_void_p()
-        _check(lib().blqh_basis_create(C.byref(self.sector), C.byref(h)))
+        _check(self._lib.blqh_basis_create(C.byref(self.sector), C.byref(h)))
         self.h = h
         self.n = lib().blqh_basis_size(self.h)
         self._id = None
@@ -187,11 +188,14 @@ class Basis:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().blqh_basis_destroy(self.h)
+            self._lib.blqh_basis_destroy(self.h)
             self.h = None
 
     def __del__(self):
-        self.close()
+        # the library handle is kept on the object: module globals may already
+        # be torn down when this runs at interpreter exit
+        if getattr(self, "_lib", None) is not None:
+            self.close()
 
 
 class Hamiltonian:
@@ -200,8 +204,9 @@ class Hamiltonian:
     def __init__(self, basis, par):
         self.basis = basis
         self.par = par
+        self._lib = lib()
         h = C.c_void_p()
-        _check(lib().blqh_hamil_build(basis.h, C.byref(par), C.byref(h)))
+        _check(self._lib.blqh_hamil_build(basis.h, C.byref(par), C.byref(h)))
         self.h = h
         n = C.c_int64()
         nnz = C.c_int64()
@@ -257,11 +262,14 @@ class Hamiltonian:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().blqh_hamil_destroy(self.h)
+            self._lib.blqh_hamil_destroy(self.h)
             self.h = None
 
     def __del__(self):
-        self.close()
+        # the library handle is kept on the object: module globals may already
+        # be torn down when this runs at interpreter exit
+        if getattr(self, "_lib", None) is not None:
+            self.close()
 
 
 def start_vector(n, seed):

@@ -67,6 +67,8 @@ struct HostTables {
 };
 HostTables host_tables(const blqh_sector& s, const blqh_params& p);
 void build_panels(blqh_hamil* H);
+// Converts the panel runs to (jagged = true) or from step-major order; no-op when already there.
+void set_panel_layout(blqh_hamil* H, bool jagged);
 
 // Row context with the basis' device pointers (tables left null).
 inline RowCtx make_row_ctx(const blqh_basis* B) {

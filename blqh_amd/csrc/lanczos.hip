@@ -474,25 +474,20 @@ __global__ void __launch_bounds__(WPB * 64) k_sps(PanelView P, const double2* __
     if (LZ) grid_reduce(dacc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
 }
 
-// ---- row-per-lane panel SpMV with LDS staging ---------------------------------------
-// Same slice ownership as k_sps.  For each (slice, panel) run the wavefront
-// copies the run's columns and values into its LDS staging area with
-// coalesced loads (CH entries at a time); then every lane walks its own row's
-// segment out of LDS, gathering x and accumulating in registers, four
-// entries per step.  No cross-lane arithmetic, each row summed in column order.
+// ---- jagged panel SpMV --------------------------------------------------------------
+// Panels in step-major order (Panels::jagged): inside each (64-row slice,
+// panel) run, step t holds entry t of every row that has more than t entries
+// there, rows in order.  Lane l owns row l and walks its own entries; at step t
+// the active lanes read consecutive positions off_t + (active lanes below l),
+// so every matrix load is coalesced, each row is summed in column order in
+// registers, and no value crosses lanes.  Slice ownership and the panel sweep
+// are those of k_sps.  Four steps are issued together so that four loads per
+// stream and four x gathers are in flight per lane.
 template <bool REAL, bool LZ, int WPB, int MAXS>
-__global__ void __launch_bounds__(WPB * 64) k_rpl(PanelView P, const double2* __restrict__ x,
+__global__ void __launch_bounds__(WPB * 64) k_jag(PanelView P, const double2* __restrict__ x,
                                                   double2* __restrict__ out, const double2* __restrict__ Wp, Scal sc,
                                                   int j, double* partials, unsigned* ticket) {
-    constexpr int CH = 256;
-    constexpr int VW = REAL ? 1 : 2;  // doubles per value
-    __shared__ double2 yl[WPB * MAXS * 64];
-    __shared__ uint16_t scol[WPB * CH];
-    __shared__ double sval[WPB * CH * VW];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint16_t* mc = scol + wv * CH;
-    double* mv = sval + wv * CH * VW;
-    double2* my = yl + wv * MAXS * 64;
     const int64_t n = P.n;
     const int64_t nslices = (n + 63) >> 6;
     const int64_t W = (int64_t)gridDim.x * WPB;
@@ -502,6 +497,8 @@ __global__ void __launch_bounds__(WPB * 64) k_rpl(PanelView P, const double2* __
         ib = sc.invb[j];
         cp = j > 0 ? sc.beta[j] * sc.invb[j - 1] : 0.0;
     }
+    __shared__ double2 yl[WPB * MAXS * 64];  // row sums of this wavefront's slices
+    double2* my = yl + wv * MAXS * 64;
     double dacc = 0.0;
     for (int64_t r0 = 0; r0 < nslices; r0 += W * MAXS) {
         for (int k = 0; k < MAXS; ++k) my[k * 64 + lane] = make_double2(0.0, 0.0);
@@ -518,59 +515,49 @@ __global__ void __launch_bounds__(WPB * 64) k_rpl(PanelView P, const double2* __
                 const int64_t rend = row0 + 64 < n ? row0 + 64 : n;
                 const uint32_t s = myrow < n ? rp[myrow] : rp[rend];
                 const uint32_t e = myrow < n ? rp[myrow + 1] : rp[rend];
-                const uint32_t E0 = __shfl(s, 0, 64);
-                const uint32_t E1 = __shfl(e, 63, 64);
+                const uint32_t len = e - s;
+                const int64_t run = base + (uint32_t)__builtin_amdgcn_readfirstlane((int)s);  // lane 0: rp[row0]
+                const uint16_t* cr = P.pcol + run;
+                const double* vr = P.pvr + run;
+                const double2* vc = P.pvc + run;
                 double ar = 0.0, ai = 0.0;
-                for (uint32_t C0 = E0; C0 < E1; C0 += CH) {
-                    const uint32_t C1 = C0 + CH < E1 ? C0 + CH : E1;
-                    for (uint32_t t = C0 + lane; t < C1; t += 64) {
-                        mc[t - C0] = __builtin_nontemporal_load(&P.pcol[base + t]);
-                        if (REAL) {
-                            mv[t - C0] = ntload1(&P.pvr[base + t]);
-                        } else {
-                            const double2 v = ntload(&P.pvc[base + t]);
-                            mv[2 * (t - C0)] = v.x;
-                            mv[2 * (t - C0) + 1] = v.y;
-                        }
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    const int lo = (int)((s > C0 ? s : C0) - C0);
-                    const int hi = (int)((e < C1 ? e : C1) - C0);
-                    int t = lo;
-                    for (; t + 4 <= hi; t += 4) {
-                        double2 xv[4];
+                uint32_t off = 0;
+                for (uint32_t t = 0;; t += 4) {
+                    uint64_t m[4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) xv[u] = xp[mc[t + u]];
+                    for (int u = 0; u < 4; ++u) m[u] = __ballot(t + u < len);
+                    if (m[0] == 0) break;
+                    uint32_t q[4];
+                    q[0] = off + lanes_below(m[0]);
+                    uint32_t o = off + (uint32_t)__popcll(m[0]);
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            if (REAL) {
-                                const double v = mv[t + u];
-                                ar += v * xv[u].x;
-                                ai += v * xv[u].y;
-                            } else {
-                                const double vr = mv[2 * (t + u)], vi = mv[2 * (t + u) + 1];
-                                ar += vr * xv[u].x - vi * xv[u].y;
-                                ai += vr * xv[u].y + vi * xv[u].x;
-                            }
-                        }
+                    for (int u = 1; u < 4; ++u) {
+                        q[u] = o + lanes_below(m[u]);
+                        o += (uint32_t)__popcll(m[u]);
                     }
-                    for (; t < hi; ++t) {
-                        const double2 xv = xp[mc[t]];
+                    off = o;
+                    int c[4];
+                    double2 v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const bool act = t + u < len;
+                        c[u] = act ? (int)__builtin_nontemporal_load(&cr[q[u]]) : 0;
+                        if (REAL) v[u] = make_double2(act ? ntload1(&vr[q[u]]) : 0.0, 0.0);
+                        else v[u] = act ? ntload(&vc[q[u]]) : make_double2(0.0, 0.0);
+                    }
+                    double2 xv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) xv[u] = xp[c[u]];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
                         if (REAL) {
-                            const double v = mv[t];
-                            ar += v * xv.x;
-                            ai += v * xv.y;
+                            ar += v[u].x * xv[u].x;
+                            ai += v[u].x * xv[u].y;
                         } else {
-                            const double vr = mv[2 * t], vi = mv[2 * t + 1];
-                            ar += vr * xv.x - vi * xv.y;
-                            ai += vr * xv.y + vi * xv.x;
+                            ar += v[u].x * xv[u].x - v[u].y * xv[u].y;
+                            ai += v[u].x * xv[u].y + v[u].y * xv[u].x;
                         }
                     }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
                 double2 tacc = my[k * 64 + lane];
                 tacc.x += ar;
@@ -663,6 +650,7 @@ void panel_go(const blqh_hamil* H, const PanelLaunch& L, const double2* x, doubl
 template <bool LZ>
 void panel_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2* out, const double2* Wp,
                     const Scal& sc, int j, double* partials, unsigned* ticket, hipStream_t st) {
+    BLQH_REQUIRE(!H->pan.jagged, BLQH_EINTERNAL, "row-segment SpMV on a jagged panel layout");
     const PanelLaunch L = panel_launch(H, pl);
     const bool real = H->pan.real;
 #define BLQH_PANEL_CASE(LPR, PT, U)                                                                  \
@@ -691,6 +679,7 @@ int sps_grid(const blqh_hamil* H, const Plan& pl) {
 template <bool LZ>
 void sps_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2* out, const double2* Wp,
                   const Scal& sc, int j, double* partials, unsigned* ticket, hipStream_t st) {
+    BLQH_REQUIRE(!H->pan.jagged, BLQH_EINTERNAL, "slice-streaming SpMV on a jagged panel layout");
     const PanelView V = view(H->pan, H->n);
     const dim3 grid(sps_grid(H, pl)), block(pl.threads);
 #define BLQH_SPS_CASE(WPB, MAXS)                                                                            \
@@ -707,8 +696,31 @@ void sps_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2
     throw Error(BLQH_EINTERNAL, "unsupported slice-streaming SpMV plan");
 }
 
+template <bool LZ>
+void jag_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2* out, const double2* Wp,
+                  const Scal& sc, int j, double* partials, unsigned* ticket, hipStream_t st) {
+    BLQH_REQUIRE(H->pan.jagged, BLQH_EINTERNAL, "jagged SpMV on a row-major panel layout");
+    const PanelView V = view(H->pan, H->n);
+    const dim3 grid(sps_grid(H, pl)), block(pl.threads);
+#define BLQH_JAG_CASE(WPB, MAXS)                                                                            \
+    if (pl.threads == WPB * 64 && pl.rows == MAXS) {                                                        \
+        if (H->pan.real)                                                                                    \
+            hipLaunchKernelGGL((k_jag<true, LZ, WPB, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_jag<false, LZ, WPB, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+        return;                                                                                             \
+    }
+    BLQH_JAG_CASE(4, 1) BLQH_JAG_CASE(4, 2) BLQH_JAG_CASE(4, 4) BLQH_JAG_CASE(4, 8)
+    BLQH_JAG_CASE(8, 1) BLQH_JAG_CASE(8, 2) BLQH_JAG_CASE(8, 4) BLQH_JAG_CASE(8, 8)
+    BLQH_JAG_CASE(16, 1) BLQH_JAG_CASE(16, 2) BLQH_JAG_CASE(16, 4) BLQH_JAG_CASE(16, 8)
+#undef BLQH_JAG_CASE
+    throw Error(BLQH_EINTERNAL, "unsupported jagged SpMV plan");
+}
+
 void launch_spmv_plan(const blqh_hamil* H, const Plan& pl, const double2* x, double2* y, hipStream_t st) {
-    if (pl.kind == 2) {
+    if (pl.kind == 3) {
+        jag_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
+    } else if (pl.kind == 2) {
         sps_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
     } else if (pl.kind == 1) {
         panel_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
@@ -720,12 +732,14 @@ void launch_spmv_plan(const blqh_hamil* H, const Plan& pl, const double2* x, dou
 
 // grid size (for partial-sum buffers) of the fused K1 under a plan
 int lz1_grid(const blqh_hamil* H, const Plan& pl) {
-    return pl.kind == 2 ? sps_grid(H, pl) : pl.kind == 1 ? panel_launch(H, pl).grid : spmv_grid(H->n, pl.lpr);
+    return pl.kind >= 2 ? sps_grid(H, pl) : pl.kind == 1 ? panel_launch(H, pl).grid : spmv_grid(H->n, pl.lpr);
 }
 
 void launch_lz1(const blqh_hamil* H, const Plan& pl, const double2* W, const double2* Wp, double2* u, const Scal& sc,
                 int j, double* partials, unsigned* ticket, hipStream_t st) {
-    if (pl.kind == 2) {
+    if (pl.kind == 3) {
+        jag_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
+    } else if (pl.kind == 2) {
         sps_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
     } else if (pl.kind == 1) {
         panel_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
@@ -736,8 +750,7 @@ void launch_lz1(const blqh_hamil* H, const Plan& pl, const double2* W, const dou
 }
 
 // Autotuner: time every variant on this matrix (random x) and keep the fastest.
-void autotune(blqh_hamil* H) {
-    if (H->spmv_kind >= 0) return;
+void tune(blqh_hamil* H) {
     // BLQH_SPMV_PLAN="kind,lanes,threads,wgs_per_cu,rows" pins a variant (profiling, reproducibility)
     if (const char* env = std::getenv("BLQH_SPMV_PLAN")) {
         Plan p{};
@@ -771,6 +784,15 @@ void autotune(blqh_hamil* H) {
                 while (ms < need && ms < 8) ms <<= 1;
                 cands.push_back(Plan{2, 64, pt, wpc, ms});
             }
+    if (H->pan.built)  // jagged kernel last: one layout conversion
+        for (int pt : {256, 512, 1024})
+            for (int wpc : {1, 2, 4}) {
+                const int64_t waves = (int64_t)runtime().num_cus * wpc * (pt / 64);
+                const int64_t need = ceil_div(ceil_div(H->n, 64), waves);
+                int ms = 1;
+                while (ms < need && ms < 8) ms <<= 1;
+                cands.push_back(Plan{3, 64, pt, wpc, ms});
+            }
     std::vector<double2> hx(H->n);
     std::mt19937 rng(12345);
     for (auto& z : hx) z = make_double2(rng() * 1.0 / rng.max() - 0.5, rng() * 1.0 / rng.max() - 0.5);
@@ -782,7 +804,9 @@ void autotune(blqh_hamil* H) {
     float best = 1e30f;
     Plan bp = cands.front();
     const int reps = H->nnz > 50000000 ? 3 : 10;
+    const bool log = std::getenv("BLQH_TUNE_LOG") != nullptr;  // print the timing of every variant
     for (const Plan& pl : cands) {
+        if (pl.kind >= 1) set_panel_layout(H, pl.kind == 3);
         launch_spmv_plan(H, pl, x.get(), y.get(), st);  // warm
         BLQH_HIP(hipEventRecord(e0, st));
         for (int r = 0; r < reps; ++r) launch_spmv_plan(H, pl, x.get(), y.get(), st);
@@ -790,6 +814,8 @@ void autotune(blqh_hamil* H) {
         BLQH_HIP(hipEventSynchronize(e1));
         float ms = 0;
         BLQH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (log) std::fprintf(stderr, "blqh spmv tune: kind %d lanes %d threads %d wgs/cu %d rows %d: %.3f ms\n",
+                              pl.kind, pl.lpr, pl.threads, pl.wpc, pl.rows, ms / reps);
         if (ms < best) { best = ms; bp = pl; }
     }
     BLQH_HIP(hipGetLastError());
@@ -800,6 +826,13 @@ void autotune(blqh_hamil* H) {
     H->spmv_threads = bp.threads;
     H->spmv_wgs_per_cu = bp.wpc;
     H->spmv_rows = bp.rows;
+}
+
+// Tunes on first use (unless a variant is pinned) and puts the panels in the
+// entry order the chosen kernel reads.
+void autotune(blqh_hamil* H) {
+    if (H->spmv_kind < 0) tune(H);
+    if (H->spmv_kind >= 1) set_panel_layout(H, H->spmv_kind == 3);
 }
 
 }  // namespace

@@ -55,7 +55,7 @@ __global__ void k_panel_scatter(int64_t n, const int64_t* __restrict__ rowptr, c
         for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
             const int q = col[k] >> kPanelShift;
             if (q != p) { p = q; dst = off[(int64_t)q * n + r]; }
-            pcol[dst] = (uint16_t)(col[k] & 0xFFFF);
+            pcol[dst] = (uint16_t)(col[k] & (kPanelWidth - 1));
             if (real) pvr[dst] = val[k].x; else pvc[dst] = val[k];
             ++dst;
         }
@@ -95,9 +95,80 @@ __global__ void k_absmax(int64_t nnz, const double2* __restrict__ val, unsigned 
     }
 }
 
+// Reorders the entries of every (64-row slice, panel) run between row-major
+// order and step-major ("jagged") order: step t holds entry t of each row of
+// the slice that has more than t entries in the panel, rows in order.  One
+// wavefront per run, lane l <-> row l: at step t the rows still active take
+// consecutive positions, off_t + (active lanes below l).
+template <bool REAL>
+__global__ void k_panel_relayout(int64_t n, int npan, const int64_t* __restrict__ pbase,
+                                 const uint32_t* __restrict__ prow, const uint16_t* __restrict__ scol,
+                                 const double* __restrict__ svr, const double2* __restrict__ svc,
+                                 uint16_t* __restrict__ dcol, double* __restrict__ dvr, double2* __restrict__ dvc,
+                                 int to_jagged) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nslices = (n + 63) >> 6;
+    const int64_t total = nslices * npan;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < total; w += nw) {
+        const int p = (int)(w / nslices);
+        const int64_t slice = w - (int64_t)p * nslices;
+        const uint32_t* rp = prow + (int64_t)p * (n + 1);
+        const int64_t row0 = slice * 64, myrow = row0 + lane;
+        const int64_t rend = row0 + 64 < n ? row0 + 64 : n;
+        const uint32_t s = myrow < n ? rp[myrow] : rp[rend];
+        const uint32_t e = myrow < n ? rp[myrow + 1] : rp[rend];
+        const uint32_t len = e - s;
+        const int64_t pb = pbase[p];
+        const int64_t run = pb + rp[row0];
+        int64_t off = 0;
+        for (uint32_t t = 0;; ++t) {
+            const uint64_t m = __ballot(t < len);
+            if (m == 0) break;
+            if (t < len) {
+                const int64_t jg = run + off + lanes_below(m);
+                const int64_t rm = pb + s + t;
+                const int64_t from = to_jagged ? rm : jg, to = to_jagged ? jg : rm;
+                dcol[to] = scol[from];
+                if (REAL) dvr[to] = svr[from];
+                else dvc[to] = svc[from];
+            }
+            off += __popcll(m);
+        }
+    }
+}
+
 }  // namespace
 
 namespace blqh {
+
+void set_panel_layout(blqh_hamil* H, bool jagged) {
+    Panels& P = H->pan;
+    if (!P.built || P.jagged == jagged) return;
+    hipStream_t st = stream();
+    const int64_t m = std::max<int64_t>(H->nnz, 1);
+    DevBuf<uint16_t> col(m);
+    DevBuf<double> vr;
+    DevBuf<double2> vc;
+    if (P.real) vr.alloc(m);
+    else vc.alloc(m);
+    const int64_t runs = ceil_div(H->n, 64) * P.npan;
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(runs, kBlk / 64), 65536));
+    if (P.real)
+        hipLaunchKernelGGL(k_panel_relayout<true>, dim3(g), dim3(kBlk), 0, st, H->n, P.npan, P.pbase.get(),
+                           P.prow.get(), P.pcol.get(), P.pvr.get(), P.pvc.get(), col.get(), vr.get(), vc.get(),
+                           jagged ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_panel_relayout<false>, dim3(g), dim3(kBlk), 0, st, H->n, P.npan, P.pbase.get(),
+                           P.prow.get(), P.pcol.get(), P.pvr.get(), P.pvc.get(), col.get(), vr.get(), vc.get(),
+                           jagged ? 1 : 0);
+    BLQH_HIP(hipGetLastError());
+    BLQH_HIP(hipStreamSynchronize(st));
+    P.pcol = std::move(col);
+    if (P.real) P.pvr = std::move(vr);
+    else P.pvc = std::move(vc);
+    P.jagged = jagged;
+}
 
 void build_panels(blqh_hamil* H) {
     hipStream_t st = stream();

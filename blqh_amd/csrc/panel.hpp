@@ -13,6 +13,7 @@ constexpr int kPanelWidth = 1 << 16;
 struct Panels {
     bool built = false;
     bool real = false;
+    bool jagged = false;     // entry order inside each (64-row slice, panel) run: row-major or step-major
     int npan = 0;
     DevBuf<int64_t> pbase;   // [npan + 1]
     DevBuf<uint32_t> prow;   // [npan][n + 1]
@@ -30,6 +31,11 @@ struct PanelView {
     const double* pvr;
     const double2* pvc;
 };
+
+// number of set bits of m that belong to lanes below the calling lane
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
 
 inline PanelView view(const Panels& P, int64_t n) {
     return PanelView{n, P.npan, P.pbase.get(), P.prow.get(), P.pcol.get(), P.pvr.get(), P.pvc.get()};

@@ -91,7 +91,10 @@ int blqh_spmv_device(const blqh_hamil* h, const void* x_dev, void* y_dev); /* de
  * group in flight (2 or 4); kind 2: panel-major slice streaming (a wavefront
  * owns 64-row slices spread over the matrix, lanes = 64) with `threads` 512
  * or 1024, `wgs_per_cu` 1-4 and `rows` = slices per wavefront and round (1, 2,
- * 4 or 8).
+ * 4 or 8); kind 3: the same slices over panels stored step-major (one lane per
+ * row, coalesced, lanes = 64) with `threads` 256, 512 or 1024, `wgs_per_cu`
+ * 1-4 and `rows` 1, 2, 4 or 8.  Switching between kind 3 and kinds 1-2
+ * reorders the panel entries on the device (temporary copy of the entries).
  * get: kind = -1 until
  * the autotuner has run (first SpMV or solve).  set: pins a variant (returns
  * BLQH_EINVAL for an unsupported one); kind = -1 re-arms the autotuner. */
