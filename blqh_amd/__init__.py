@@ -69,6 +69,12 @@ _SIGS = {
     "blqh_spmv_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "blqh_lanczos": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.POINTER(LanczosResult), C.c_void_p,
                                C.c_void_p, C.c_void_p]),
+    "blqh_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]),
+    "blqh_batch_lanczos": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.c_void_p, C.c_void_p,
+                                     C.POINTER(LanczosResult), C.c_void_p]),
+    "blqh_batch_get_lanes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "blqh_batch_set_lanes": (C.c_int, [C.c_void_p, C.c_int32]),
+    "blqh_batch_destroy": (None, [C.c_void_p]),
     "blqh_dense_ground_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]),
     "blqh_start_vector": (C.c_int, [C.c_int64, C.c_uint32, C.c_void_p]),
     "blqh_tridiag_lowest": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
@@ -268,6 +274,66 @@ class Hamiltonian:
     def __del__(self):
         # the library handle is kept on the object: module globals may already
         # be torn down when this runs at interpreter exit
+        if getattr(self, "_lib", None) is not None:
+            self.close()
+
+
+class Batch:
+    """Resident Hamiltonians solved together (blqh_batch_*).  The Hamiltonian
+    objects are kept alive by the batch."""
+
+    def __init__(self, hams):
+        self.hams = list(hams)
+        self._lib = lib()
+        arr = (C.c_void_p * len(self.hams))(*[H.h.value for H in self.hams])
+        h = C.c_void_p()
+        _check(self._lib.blqh_batch_create(arr, len(self.hams), C.byref(h)))
+        self.h = h
+
+    @property
+    def lanes(self):
+        v = C.c_int32()
+        _check(self._lib.blqh_batch_get_lanes(self.h, C.byref(v)))
+        return v.value
+
+    @lanes.setter
+    def lanes(self, v):
+        _check(self._lib.blqh_batch_set_lanes(self.h, v))
+
+    def lanczos(self, lam=200, seeds=0, tol=1e-10, res_tol=1e-7, check_every=5, starts=None, want_psi=True):
+        """Ground states of every problem; `seeds` is one seed or one per problem,
+        `starts` None or a list of start vectors (None entries: seeded)."""
+        P = len(self.hams)
+        opts = LanczosOpts(lam, 0, tol, check_every, 1, None, res_tol)
+        sd = np.broadcast_to(np.asarray(seeds, np.uint32), (P,)).copy()
+        keep = []
+        st = None
+        if starts is not None:
+            st = (C.c_void_p * P)()
+            for i, v in enumerate(starts):
+                if v is not None:
+                    a = np.ascontiguousarray(v, np.complex128)
+                    keep.append(a)
+                    st[i] = a.ctypes.data
+        res = (LanczosResult * P)()
+        psis = [np.empty(H.n, np.complex128) for H in self.hams] if want_psi else None
+        po = None
+        if want_psi:
+            po = (C.c_void_p * P)(*[x.ctypes.data for x in psis])
+        _check(self._lib.blqh_batch_lanczos(self.h, C.byref(opts), sd.ctypes.data, st, res, po))
+        out = []
+        for i in range(P):
+            d = {k: getattr(res[i], k) for k, _ in LanczosResult._fields_}
+            d["psi"] = psis[i] if want_psi else None
+            out.append(d)
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.blqh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
         if getattr(self, "_lib", None) is not None:
             self.close()
 

@@ -14,6 +14,12 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil
 void blqh_spmv_device_impl(blqh_hamil* H, const void* x, void* y);
 void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_result& r, double* psi, double* alpha,
                        double* beta);
+blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count);
+void blqh_batch_destroy_impl(blqh_batch* b);
+int blqh_batch_lanes_impl(const blqh_batch* b);
+void blqh_batch_set_lanes_impl(blqh_batch* b, int lanes);
+void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const uint32_t* seeds,
+                             const double* const* starts, blqh_lanczos_result* res, double* const* psi_out);
 void blqh_dense_ground_state_impl(const blqh_hamil* H, double* e0, double* e_printed, double* vec);
 void blqh_device_copy_impl(const void* src, void* dst, int64_t bytes);
 namespace blqh {
@@ -206,6 +212,33 @@ int blqh_lanczos(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_r
         std::memset(r, 0, sizeof(*r));
         blqh_lanczos_impl(const_cast<blqh_hamil*>(h), *o, *r, psi, alpha, beta);
     });
+}
+
+int blqh_batch_create(blqh_hamil* const* hs, int32_t count, blqh_batch** out) {
+    if (!hs || !out || count < 1) return fail(BLQH_EINVAL, "bad argument");
+    *out = nullptr;
+    return guarded([&] { *out = blqh_batch_create_impl(hs, count); });
+}
+
+int blqh_batch_lanczos(blqh_batch* b, const blqh_lanczos_opts* o, const uint32_t* seeds, const double* const* starts,
+                       blqh_lanczos_result* res, double* const* psi) {
+    if (!b || !o || !res) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] { blqh_batch_lanczos_impl(b, *o, seeds, starts, res, psi); });
+}
+
+int blqh_batch_get_lanes(const blqh_batch* b, int32_t* lanes) {
+    if (!b || !lanes) return fail(BLQH_EINVAL, "null argument");
+    *lanes = blqh_batch_lanes_impl(b);
+    return BLQH_OK;
+}
+
+int blqh_batch_set_lanes(blqh_batch* b, int32_t lanes) {
+    if (!b) return fail(BLQH_EINVAL, "null batch");
+    return guarded([&] { blqh_batch_set_lanes_impl(b, lanes); });
+}
+
+void blqh_batch_destroy(blqh_batch* b) {
+    if (b) blqh_batch_destroy_impl(b);
 }
 
 int blqh_dense_ground_state(const blqh_hamil* h, double* e0, double* e_printed, double* vec) {

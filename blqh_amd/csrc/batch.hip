@@ -1,0 +1,702 @@
+// batch.hip -- many Lanczos ground-state solves advanced together.
+//
+// Ancestor: the reference's serial sweeps over sectors and parameter points
+// (BLQH_without_spin/main.cpp:76-165, BLQH_with_spin/main.cpp:158-302), each
+// of which runs lhamil's three passes (lanczos_hamiltonian.cpp:453-656) on one
+// Hamiltonian at a time.  Here every problem of a batch (one resident
+// Hamiltonian + one start vector) takes Lanczos step j in the same two
+// launches:
+//   K1  u = (H W_j)/beta_j - (beta_j/beta_{j-1}) W_{j-1},  alpha partials
+//   K2  alpha_j = sum of the problem's K1 partials (fixed order),
+//       W_{j+1} = u - alpha_j W_j / beta_j,               |W_{j+1}|^2 partials
+//   K3  beta_{j+1} = sqrt(sum of the K2 partials)          (one wave per problem)
+// The rows of every problem are cut into work items of kItemRows rows; a
+// persistent grid walks the item list, so problems of any size share one
+// launch and each item's partial sum has a fixed owner (bit-reproducible).
+// Kernels read the step from a device counter and skip problems whose active
+// flag is clear, so `check_every` steps are captured once in a hipGraph and
+// replayed; between replays the host reads alpha/beta of every problem,
+// runs the convergence test of the single-problem solver and clears the flags
+// of the converged ones (data-dependent termination without a host round trip
+// per step).  The Krylov vectors of every problem stay in HBM, so the ground
+// state is one weighted sum per problem afterwards (no replay pass).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "engine.hpp"
+
+namespace blqh {
+void start_vector(int64_t n, uint32_t seed, std::vector<double2>& v);
+double tridiag_lowest(int m, const double* a, const double* b, double* vec);
+}  // namespace blqh
+
+namespace {
+
+using namespace blqh;
+
+constexpr int kItemRows = 256;  // rows per work item
+constexpr int kBT = 256;        // threads per workgroup
+
+// device descriptor of one problem
+struct BProb {
+    int64_t n;
+    int32_t npan, real;
+    const int64_t* pbase;
+    const uint32_t* prow;
+    const uint16_t* pcol;
+    const double* pvr;
+    const double2* pvc;
+    int64_t item0;  // first work item
+    int64_t voff;   // row offset in the pooled vectors
+    double2* W;     // Krylov block: W_0 .. W_cap, n each
+};
+
+// per-step device state
+struct BState {
+    const BProb* probs;
+    const int32_t* item_prob;  // work item -> problem
+    int64_t nitems;
+    const int32_t* active;     // [P]
+    int32_t* jbase;            // step of the first kernel in a replay (advanced by k_bnext)
+    int cap;                   // lambda
+    double* alpha;             // [P][cap]
+    double* beta;              // [P][cap + 1], beta[p][0] = 1
+    double* invb;              // [P][cap + 1]
+    double2* u;                // pooled rows
+    double* part1;             // [nitems]
+    double* part2;             // [nitems]
+};
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0;
+    for (int w = 0; w < kBT / 64; ++w) t += sh[w];
+    return t;
+}
+
+// sum of part[i0 .. i0+cnt) by one wavefront in a fixed order; every lane
+// returns the same bits (the butterfly adds commutative pairs)
+__device__ __forceinline__ double wave_sum(const double* part, int64_t i0, int64_t cnt, int lane) {
+    double s = 0;
+    for (int64_t i = lane; i < cnt; i += 64) s += part[i0 + i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    return s;
+}
+
+// y_r = sum_k H_rk x_k for one row, LPR lanes (panel storage, any panel count)
+template <int LPR>
+__device__ __forceinline__ double2 prow_dot(const BProb& P, const double2* __restrict__ x, int64_t r, int lane) {
+    double re = 0.0, im = 0.0;
+    for (int pan = 0; pan < P.npan; ++pan) {
+        const uint32_t* rp = P.prow + (int64_t)pan * (P.n + 1);
+        const uint32_t s = rp[r], e = rp[r + 1];
+        const int64_t base = P.pbase[pan];
+        const double2* xp = x + (int64_t)pan * kPanelWidth;
+        if (P.real) {
+            for (uint32_t k = s + lane; k < e; k += LPR) {
+                const int c = __builtin_nontemporal_load(&P.pcol[base + k]);
+                const double v = __builtin_nontemporal_load(&P.pvr[base + k]);
+                const double2 xv = xp[c];
+                re += v * xv.x;
+                im += v * xv.y;
+            }
+        } else {
+            for (uint32_t k = s + lane; k < e; k += LPR) {
+                const int c = __builtin_nontemporal_load(&P.pcol[base + k]);
+                const double2 v = P.pvc[base + k];
+                const double2 xv = xp[c];
+                re += v.x * xv.x - v.y * xv.y;
+                im += v.x * xv.y + v.y * xv.x;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) {
+        re += __shfl_xor(re, o, LPR);
+        im += __shfl_xor(im, o, LPR);
+    }
+    return make_double2(re, im);
+}
+
+// K1 (LZ) or a plain product y = H x with partial Re<x|y> (!LZ: x and y are
+// the pooled vectors xs / ys).
+template <int LPR, bool LZ>
+__global__ void __launch_bounds__(kBT) k_b1(BState S, int jrel, const double2* __restrict__ xs,
+                                            double2* __restrict__ ys, double* __restrict__ part) {
+    __shared__ double sh[kBT / 64];
+    const int j = LZ ? *S.jbase + jrel : 0;
+    if (LZ && j >= S.cap) return;
+    constexpr int RPP = kBT / LPR;
+    const int sub = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    for (int64_t it = blockIdx.x; it < S.nitems; it += gridDim.x) {
+        const int p = S.item_prob[it];
+        if (!S.active[p]) continue;  // uniform over the workgroup
+        const BProb P = S.probs[p];
+        const int64_t r0 = (it - P.item0) * kItemRows;
+        const int64_t r1 = r0 + kItemRows < P.n ? r0 + kItemRows : P.n;
+        const double2* x;
+        double2* y;
+        const double2* Wp = nullptr;
+        double ib = 1.0, cp = 0.0;
+        if (LZ) {
+            const int64_t ld = S.cap + 1;
+            ib = S.invb[p * ld + j];
+            cp = j > 0 ? S.beta[p * ld + j] * S.invb[p * ld + j - 1] : 0.0;
+            x = P.W + (int64_t)j * P.n;
+            Wp = j > 0 ? x - P.n : x;
+            y = S.u + P.voff;
+        } else {
+            x = xs + P.voff;
+            y = ys + P.voff;
+        }
+        double acc = 0.0;
+        for (int64_t r = r0 + sub; r < r1; r += RPP) {
+            const double2 h = prow_dot<LPR>(P, x, r, lane);
+            if (lane == 0) {
+                double ur = h.x, ui = h.y;
+                if (LZ) {
+                    ur = ib * h.x;
+                    ui = ib * h.y;
+                    if (j > 0) {
+                        const double2 q = Wp[r];
+                        ur -= cp * q.x;
+                        ui -= cp * q.y;
+                    }
+                }
+                y[r] = make_double2(ur, ui);
+                const double2 w = x[r];
+                acc += w.x * ur + w.y * ui;
+            }
+        }
+        const double t = block_sum(acc, sh);
+        if (threadIdx.x == 0) part[it] = t;
+    }
+}
+
+// K2: alpha_j, W_{j+1} = u - alpha_j W_j / beta_j, |W_{j+1}|^2 partials
+__global__ void __launch_bounds__(kBT) k_b2(BState S, int jrel) {
+    __shared__ double sh[kBT / 64];
+    __shared__ double sa;
+    const int j = *S.jbase + jrel;
+    if (j >= S.cap) return;
+    const int64_t ld = S.cap + 1;
+    for (int64_t it = blockIdx.x; it < S.nitems; it += gridDim.x) {
+        const int p = S.item_prob[it];
+        if (!S.active[p]) continue;
+        const BProb P = S.probs[p];
+        const int64_t nit = (P.n + kItemRows - 1) / kItemRows;
+        const double ib = S.invb[p * ld + j];
+        if (threadIdx.x < 64) {
+            const double s = wave_sum(S.part1, P.item0, nit, threadIdx.x);
+            if (threadIdx.x == 0) {
+                sa = s * ib;
+                if (it == P.item0) S.alpha[p * S.cap + j] = s * ib;
+            }
+        }
+        __syncthreads();
+        const double a = sa * ib;
+        const int64_t r = (it - P.item0) * kItemRows + threadIdx.x;
+        double acc = 0.0;
+        if (r < P.n) {
+            const double2 w = P.W[(int64_t)j * P.n + r], x = S.u[P.voff + r];
+            const double2 v = make_double2(x.x - a * w.x, x.y - a * w.y);
+            P.W[(int64_t)(j + 1) * P.n + r] = v;
+            acc = v.x * v.x + v.y * v.y;
+        }
+        const double t = block_sum(acc, sh);
+        if (threadIdx.x == 0) S.part2[it] = t;
+        __syncthreads();  // sa is reused by the next item
+    }
+}
+
+// K3: beta_{j+1}, 1/beta_{j+1}; one wavefront per problem
+__global__ void __launch_bounds__(64) k_b3(BState S, int jrel, int nprob) {
+    const int j = *S.jbase + jrel;
+    if (j >= S.cap) return;
+    const int p = blockIdx.x;
+    if (p >= nprob || !S.active[p]) return;
+    const BProb P = S.probs[p];
+    const int64_t nit = (P.n + kItemRows - 1) / kItemRows;
+    const double s = wave_sum(S.part2, P.item0, nit, threadIdx.x);
+    if (threadIdx.x == 0) {
+        const int64_t ld = S.cap + 1;
+        const double b = sqrt(s);
+        S.beta[p * ld + j + 1] = b;
+        S.invb[p * ld + j + 1] = b > 0 ? 1.0 / b : 0.0;
+    }
+}
+
+// end of a replay: the next replay starts `count` steps later
+__global__ void k_bnext(int32_t* jbase, int count) {
+    if (threadIdx.x == 0) *jbase += count;
+}
+
+// psi_r = sum_l coef_l conj(W_l[r]) over the problem's m vectors; |psi|^2 partials
+__global__ void __launch_bounds__(kBT) k_bcombine(BState S, const double* __restrict__ coef,
+                                                  const int32_t* __restrict__ mvec, double2* __restrict__ psi) {
+    __shared__ double sh[kBT / 64];
+    for (int64_t it = blockIdx.x; it < S.nitems; it += gridDim.x) {
+        const int p = S.item_prob[it];
+        const BProb P = S.probs[p];
+        const int m = mvec[p];
+        const double* cf = coef + (int64_t)p * S.cap;
+        const int64_t r = (it - P.item0) * kItemRows + threadIdx.x;
+        double acc = 0.0;
+        if (r < P.n) {
+            double re = 0.0, im = 0.0;
+            for (int l = 0; l < m; ++l) {
+                const double2 w = P.W[(int64_t)l * P.n + r];
+                re += cf[l] * w.x;
+                im -= cf[l] * w.y;
+            }
+            psi[P.voff + r] = make_double2(re, im);
+            acc = re * re + im * im;
+        }
+        const double t = block_sum(acc, sh);
+        if (threadIdx.x == 0) S.part1[it] = t;
+    }
+}
+
+__global__ void __launch_bounds__(kBT) k_bscale(BState S, const double* __restrict__ scale, double2* __restrict__ psi) {
+    for (int64_t it = blockIdx.x; it < S.nitems; it += gridDim.x) {
+        const int p = S.item_prob[it];
+        const BProb P = S.probs[p];
+        const int64_t r = (it - P.item0) * kItemRows + threadIdx.x;
+        if (r < P.n) {
+            double2 z = psi[P.voff + r];
+            z.x *= scale[p];
+            z.y *= scale[p];
+            psi[P.voff + r] = z;
+        }
+    }
+}
+
+}  // namespace
+
+struct blqh_batch {
+    std::vector<blqh_hamil*> hs;
+    std::vector<BProb> h_probs;
+    std::vector<int64_t> item0;  // [P + 1]
+    int64_t rows = 0, nitems = 0, nnz = 0;
+    DevBuf<int32_t> item_prob;
+    int lpr = 0;                 // lanes per row of k_b1 (autotuned)
+    int grid = 0;
+};
+
+namespace {
+
+// Row-major panel layout and current panel buffers of every member.
+void refresh_probs(blqh_batch* b) {
+    for (size_t p = 0; p < b->hs.size(); ++p) {
+        blqh_hamil* H = b->hs[p];
+        set_panel_layout(H, false);  // the batched kernels read row-major panel runs
+        BProb& d = b->h_probs[p];
+        d.pbase = H->pan.pbase.get();
+        d.prow = H->pan.prow.get();
+        d.pcol = H->pan.pcol.get();
+        d.pvr = H->pan.pvr.get();
+        d.pvc = H->pan.pvc.get();
+    }
+}
+
+template <bool LZ>
+void launch_b1(const blqh_batch* b, const BState& S, int jrel, const double2* xs, double2* ys, double* part,
+               hipStream_t st) {
+    const dim3 g(b->grid), t(kBT);
+    switch (b->lpr) {
+        case 4: hipLaunchKernelGGL((k_b1<4, LZ>), g, t, 0, st, S, jrel, xs, ys, part); break;
+        case 8: hipLaunchKernelGGL((k_b1<8, LZ>), g, t, 0, st, S, jrel, xs, ys, part); break;
+        case 16: hipLaunchKernelGGL((k_b1<16, LZ>), g, t, 0, st, S, jrel, xs, ys, part); break;
+        case 32: hipLaunchKernelGGL((k_b1<32, LZ>), g, t, 0, st, S, jrel, xs, ys, part); break;
+        default: throw Error(BLQH_EINTERNAL, "unsupported batched SpMV lanes");
+    }
+}
+
+// Pooled per-problem scalars and vectors of one batched solve.
+struct Pools {
+    DevBuf<BProb> probs;
+    DevBuf<int32_t> active, jbase, mvec;
+    DevBuf<double> alpha, beta, invb, part1, part2, coef, scale;
+    DevBuf<double2> u, psi, W;
+};
+
+BState make_state(const blqh_batch* b, Pools& pl, int cap) {
+    BState S{};
+    S.probs = pl.probs.get();
+    S.item_prob = b->item_prob.get();
+    S.nitems = b->nitems;
+    S.active = pl.active.get();
+    S.jbase = pl.jbase.get();
+    S.cap = cap;
+    S.alpha = pl.alpha.get();
+    S.beta = pl.beta.get();
+    S.invb = pl.invb.get();
+    S.u = pl.u.get();
+    S.part1 = pl.part1.get();
+    S.part2 = pl.part2.get();
+    return S;
+}
+
+// time the plain batched product for each lane count and keep the fastest
+void tune_batch(blqh_batch* b) {
+    hipStream_t st = stream();
+    Pools pl;
+    const int P = (int)b->hs.size();
+    pl.probs.alloc(P);
+    pl.active.alloc(P);
+    BLQH_HIP(hipMemcpyAsync(pl.probs.get(), b->h_probs.data(), P * sizeof(BProb), hipMemcpyHostToDevice, st));
+    std::vector<int32_t> on(P, 1);
+    BLQH_HIP(hipMemcpyAsync(pl.active.get(), on.data(), P * 4, hipMemcpyHostToDevice, st));
+    DevBuf<double2> x(b->rows), y(b->rows);
+    DevBuf<double> part(b->nitems);
+    BLQH_HIP(hipMemsetAsync(x.get(), 0, b->rows * sizeof(double2), st));
+    BState S = make_state(b, pl, 1);
+    hipEvent_t e0, e1;
+    BLQH_HIP(hipEventCreate(&e0));
+    BLQH_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bl = 8;
+    for (int l : {4, 8, 16, 32}) {
+        b->lpr = l;
+        launch_b1<false>(b, S, 0, x.get(), y.get(), part.get(), st);
+        BLQH_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < 3; ++r) launch_b1<false>(b, S, 0, x.get(), y.get(), part.get(), st);
+        BLQH_HIP(hipEventRecord(e1, st));
+        BLQH_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        BLQH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; bl = l; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->lpr = bl;
+}
+
+}  // namespace
+
+blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count) {
+    BLQH_REQUIRE(count >= 1, BLQH_EINVAL, "empty batch");
+    auto* b = new blqh_batch;
+    try {
+        b->hs.assign(hs, hs + count);
+        b->item0.assign(count + 1, 0);
+        std::vector<int32_t> ip;
+        for (int p = 0; p < count; ++p) {
+            blqh_hamil* H = hs[p];
+            BLQH_REQUIRE(H != nullptr, BLQH_EINVAL, "null hamiltonian in batch");
+            BLQH_REQUIRE(H->pan.built && H->n > 0, BLQH_EINVAL, "batched solve needs a non-empty panel layout");
+            BProb d{};
+            d.n = H->n;
+            d.npan = H->pan.npan;
+            d.real = H->pan.real ? 1 : 0;
+            d.item0 = b->nitems;
+            d.voff = b->rows;
+            const int64_t nit = ceil_div(H->n, kItemRows);
+            for (int64_t i = 0; i < nit; ++i) ip.push_back(p);
+            b->nitems += nit;
+            b->rows += H->n;
+            b->nnz += H->nnz;
+            b->item0[p + 1] = b->nitems;
+            b->h_probs.push_back(d);
+        }
+        BLQH_REQUIRE(b->nitems < (int64_t)1 << 31, BLQH_ERANGE, "batch too large");
+        b->item_prob.alloc(ip.size());
+        BLQH_HIP(hipMemcpy(b->item_prob.get(), ip.data(), ip.size() * 4, hipMemcpyHostToDevice));
+        b->grid = (int)std::max<int64_t>(1, std::min<int64_t>(b->nitems, (int64_t)runtime().num_cus * 8));
+        refresh_probs(b);
+        tune_batch(b);
+    } catch (...) {
+        delete b;
+        throw;
+    }
+    return b;
+}
+
+void blqh_batch_destroy_impl(blqh_batch* b) { delete b; }
+
+int blqh_batch_lanes_impl(const blqh_batch* b) { return b->lpr; }
+
+void blqh_batch_set_lanes_impl(blqh_batch* b, int lanes) {
+    BLQH_REQUIRE(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32, BLQH_EINVAL, "unsupported lanes per row");
+    b->lpr = lanes;
+}
+
+void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const uint32_t* seeds,
+                             const double* const* starts, blqh_lanczos_result* res, double* const* psi_out) {
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = stream();
+    const int P = (int)b->hs.size();
+    const int cap = o.lambda;
+    BLQH_REQUIRE(cap >= 2 && cap <= 100000, BLQH_EINVAL, "lambda out of range");
+    const int chk = std::max(1, o.check_every);
+    const int64_t ld = cap + 1;
+
+    // Panel pointers are re-read here: a single-problem solve or SpMV of a
+    // member Hamiltonian may have reordered its panels (new buffers) since
+    // the batch was created.
+    refresh_probs(b);
+    // Krylov blocks of all problems
+    size_t vecs = 0;
+    for (int p = 0; p < P; ++p) vecs += (size_t)ld * b->h_probs[p].n;
+    Pools pl;
+    pl.W.alloc(vecs);
+    std::vector<BProb> hp = b->h_probs;
+    {
+        size_t off = 0;
+        for (int p = 0; p < P; ++p) {
+            hp[p].W = pl.W.get() + off;
+            off += (size_t)ld * hp[p].n;
+        }
+    }
+    pl.probs.alloc(P);
+    pl.active.alloc(P);
+    pl.jbase.alloc(1);
+    pl.mvec.alloc(P);
+    pl.alpha.alloc((size_t)P * cap);
+    pl.beta.alloc((size_t)P * ld);
+    pl.invb.alloc((size_t)P * ld);
+    pl.part1.alloc(b->nitems);
+    pl.part2.alloc(b->nitems);
+    pl.coef.alloc((size_t)P * cap);
+    pl.scale.alloc(P);
+    pl.u.alloc(b->rows);
+    pl.psi.alloc(b->rows);
+    BLQH_HIP(hipMemcpyAsync(pl.probs.get(), hp.data(), P * sizeof(BProb), hipMemcpyHostToDevice, st));
+
+    // start vectors (host mt19937 or caller's), beta_0 = 1
+    {
+        std::vector<std::vector<double2>> v0(P);
+        auto make = [&](int p0, int p1) {
+            for (int p = p0; p < p1; ++p) {
+                const int64_t n = hp[p].n;
+                if (starts && starts[p]) {
+                    v0[p].assign((const double2*)starts[p], (const double2*)starts[p] + n);
+                    double s = 0;
+                    for (auto& z : v0[p]) s += z.x * z.x + z.y * z.y;
+                    if (!(s > 0) || !std::isfinite(s)) { v0[p].clear(); continue; }
+                    s = std::sqrt(s);
+                    for (auto& z : v0[p]) { z.x /= s; z.y /= s; }
+                } else {
+                    start_vector(n, seeds ? seeds[p] : o.seed, v0[p]);
+                }
+            }
+        };
+        const int nt = std::max(1, std::min<int>(P, (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; ++t) th.emplace_back(make, (int)((int64_t)P * t / nt), (int)((int64_t)P * (t + 1) / nt));
+        for (auto& t : th) t.join();
+        for (int p = 0; p < P; ++p)
+            BLQH_REQUIRE(!v0[p].empty(), BLQH_EINVAL, "start vector is zero or not finite");
+        for (int p = 0; p < P; ++p)
+            BLQH_HIP(hipMemcpyAsync(hp[p].W, v0[p].data(), hp[p].n * sizeof(double2), hipMemcpyHostToDevice, st));
+        std::vector<double> ones((size_t)P * ld, 1.0);
+        BLQH_HIP(hipMemcpyAsync(pl.beta.get(), ones.data(), ones.size() * 8, hipMemcpyHostToDevice, st));
+        BLQH_HIP(hipMemcpyAsync(pl.invb.get(), ones.data(), ones.size() * 8, hipMemcpyHostToDevice, st));
+        BLQH_HIP(hipStreamSynchronize(st));  // v0 and ones are freed at scope exit
+    }
+    std::vector<int32_t> act(P, 1);
+    BLQH_HIP(hipMemcpyAsync(pl.active.get(), act.data(), P * 4, hipMemcpyHostToDevice, st));
+    BState S = make_state(b, pl, cap);
+
+    // `chk` steps as one graph; kernels read the first step from pl.jbase and
+    // the graph's last kernel advances it
+    Profile& prof = profile();
+    const bool timed = prof.on;  // per-kernel events need plain launches
+    std::vector<hipEvent_t> ev;
+    int ntimed = 0;
+    auto chunk = [&](bool record, int first) {
+        for (int k = 0; k < chk; ++k) {
+            const bool t = record && first + k < cap;
+            if (t) {
+                while ((int)ev.size() < 2 * (ntimed + 1)) {
+                    hipEvent_t e;
+                    BLQH_HIP(hipEventCreate(&e));
+                    ev.push_back(e);
+                }
+                BLQH_HIP(hipEventRecord(ev[2 * ntimed], st));
+            }
+            launch_b1<true>(b, S, k, nullptr, nullptr, pl.part1.get(), st);
+            if (t) BLQH_HIP(hipEventRecord(ev[2 * ntimed++ + 1], st));
+            hipLaunchKernelGGL(k_b2, dim3(b->grid), dim3(kBT), 0, st, S, k);
+            hipLaunchKernelGGL(k_b3, dim3(P), dim3(64), 0, st, S, k, P);
+        }
+        hipLaunchKernelGGL(k_bnext, dim3(1), dim3(64), 0, st, pl.jbase.get(), chk);
+    };
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    struct GraphGuard {
+        hipGraphExec_t& e;
+        hipGraph_t& g;
+        std::vector<hipEvent_t>& ev;
+        ~GraphGuard() {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+            for (auto x : ev) (void)hipEventDestroy(x);
+        }
+    } guard{gexec, graph, ev};
+    if (!timed) {
+        BLQH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        chunk(false, 0);
+        BLQH_HIP(hipStreamEndCapture(st, &graph));
+        BLQH_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    }
+    const int zero = 0;
+    BLQH_HIP(hipMemcpy(pl.jbase.get(), &zero, 4, hipMemcpyHostToDevice));
+
+    // pinned staging: alpha/beta read back per replay, active flags (two slots,
+    // so a slot is rewritten only after its copy has run)
+    const size_t pin_bytes = ((size_t)P * cap + (size_t)P * ld) * 8 + 2 * (size_t)P * 4;
+    char* pin = nullptr;
+    BLQH_HIP(hipHostMalloc((void**)&pin, pin_bytes, hipHostMallocDefault));
+    struct PinGuard {
+        char* p;
+        ~PinGuard() { if (p) (void)hipHostFree(p); }
+    } pguard{pin};
+    double* ha = (double*)pin;
+    double* hb = ha + (size_t)P * cap;
+    int32_t* actpin = (int32_t*)(hb + (size_t)P * ld);
+    hipEvent_t evc;
+    BLQH_HIP(hipEventCreateWithFlags(&evc, hipEventDisableTiming));
+    struct EvGuard {
+        hipEvent_t e;
+        ~EvGuard() { (void)hipEventDestroy(e); }
+    } eguard{evc};
+
+    std::vector<std::vector<double>> ritz(P);
+    std::vector<int> m(P, 0), steps_run(P, 0);
+    std::vector<char> conv(P, 0), live(P, 1);
+    std::vector<double> dE(P, INFINITY), resid(P, INFINITY), svec(cap);
+    // Replay r+1 is queued before the host tests replay r, so the host test
+    // overlaps the GPU; a problem that converged in replay r runs one more
+    // replay of steps that are never used.
+    auto enqueue = [&](int first) {
+        if (timed) chunk(true, first);
+        else BLQH_HIP(hipGraphLaunch(gexec, st));
+        BLQH_HIP(hipGetLastError());
+    };
+    auto readback = [&]() {
+        BLQH_HIP(hipMemcpyAsync(ha, pl.alpha.get(), (size_t)P * cap * 8, hipMemcpyDeviceToHost, st));
+        BLQH_HIP(hipMemcpyAsync(hb, pl.beta.get(), (size_t)P * ld * 8, hipMemcpyDeviceToHost, st));
+        BLQH_HIP(hipEventRecord(evc, st));
+    };
+    int done = 0, nlive = P, slot = 0;
+    enqueue(0);
+    readback();
+    while (true) {
+        const int upto = std::min(cap, done + chk);
+        const bool more = upto < cap;
+        if (more) enqueue(upto);  // speculative next replay
+        BLQH_HIP(hipEventSynchronize(evc));
+        // the single-problem solver's convergence test, per live problem
+        bool changed = false;
+        for (int p = 0; p < P; ++p) {
+            if (!live[p]) continue;
+            steps_run[p] = std::min(cap, upto + (more ? chk : 0));
+            const double* a = ha + (size_t)p * cap;
+            const double* bb = hb + (size_t)p * ld;
+            for (int j = done; j < upto; ++j) {
+                const double e = tridiag_lowest(j + 1, a, bb + 1, nullptr);
+                ritz[p].push_back(e);
+                m[p] = j + 1;
+                if (j > 0) dE[p] = std::fabs(ritz[p][j] - ritz[p][j - 1]);
+                const double scale = std::max(std::fabs(a[j]), 1.0);
+                if (!(bb[j + 1] > 1e-13 * scale) || !std::isfinite(bb[j + 1])) {
+                    resid[p] = 0;
+                    conv[p] = 1;
+                    break;
+                }
+                if (o.tol > 0 && j > 0 && dE[p] < o.tol) {
+                    if (o.res_tol <= 0) { conv[p] = 1; break; }
+                    (void)tridiag_lowest(j + 1, a, bb + 1, svec.data());
+                    resid[p] = bb[j + 1] * std::fabs(svec[j]);
+                    if (resid[p] < o.res_tol) { conv[p] = 1; break; }
+                }
+            }
+            if (conv[p]) {
+                live[p] = 0;
+                --nlive;
+                changed = true;
+            }
+        }
+        done = upto;
+        if (!more || nlive == 0) break;
+        if (changed) {
+            int32_t* dst = actpin + (size_t)slot * P;
+            for (int p = 0; p < P; ++p) dst[p] = live[p];
+            BLQH_HIP(hipMemcpyAsync(pl.active.get(), dst, P * 4, hipMemcpyHostToDevice, st));
+            slot ^= 1;
+        }
+        readback();
+    }
+    BLQH_HIP(hipStreamSynchronize(st));  // a speculative replay may still run
+
+    // pass 2 per problem, then psi = sum_l c_l conj(v_l), normalised, and Re<psi|H psi>
+    std::vector<double> hinvb((size_t)P * ld);
+    BLQH_HIP(hipMemcpyAsync(hinvb.data(), pl.invb.get(), hinvb.size() * 8, hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    std::vector<double> w((size_t)P * cap, 0.0), E0(P);
+    std::vector<int32_t> hm(P);
+    for (int p = 0; p < P; ++p) {
+        BLQH_REQUIRE(m[p] >= 1 && std::isfinite(ritz[p].back()), BLQH_EINTERNAL,
+                     "Lanczos produced non-finite coefficients");
+        std::vector<double> c(m[p]);
+        E0[p] = tridiag_lowest(m[p], ha + (size_t)p * cap, hb + (size_t)p * ld + 1, c.data());
+        for (int l = 0; l < m[p]; ++l) w[(size_t)p * cap + l] = c[l] * hinvb[(size_t)p * ld + l];
+        hm[p] = m[p];
+        if (!std::isfinite(resid[p])) resid[p] = hb[(size_t)p * ld + m[p]] * std::fabs(c[m[p] - 1]);
+    }
+    BLQH_HIP(hipMemcpyAsync(pl.coef.get(), w.data(), w.size() * 8, hipMemcpyHostToDevice, st));
+    BLQH_HIP(hipMemcpyAsync(pl.mvec.get(), hm.data(), P * 4, hipMemcpyHostToDevice, st));
+    std::fill(act.begin(), act.end(), 1);
+    BLQH_HIP(hipMemcpyAsync(pl.active.get(), act.data(), P * 4, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_bcombine, dim3(b->grid), dim3(kBT), 0, st, S, pl.coef.get(), pl.mvec.get(), pl.psi.get());
+    std::vector<double> hp1(b->nitems), hp2(b->nitems), sc(P);
+    BLQH_HIP(hipMemcpyAsync(hp1.data(), pl.part1.get(), b->nitems * 8, hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    for (int p = 0; p < P; ++p) {
+        double s = 0;
+        for (int64_t i = b->item0[p]; i < b->item0[p + 1]; ++i) s += hp1[i];
+        sc[p] = s > 0 ? 1.0 / std::sqrt(s) : 0.0;
+    }
+    BLQH_HIP(hipMemcpyAsync(pl.scale.get(), sc.data(), P * 8, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_bscale, dim3(b->grid), dim3(kBT), 0, st, S, pl.scale.get(), pl.psi.get());
+    launch_b1<false>(b, S, 0, pl.psi.get(), pl.u.get(), pl.part2.get(), st);
+    BLQH_HIP(hipGetLastError());
+    BLQH_HIP(hipMemcpyAsync(hp2.data(), pl.part2.get(), b->nitems * 8, hipMemcpyDeviceToHost, st));
+    for (int p = 0; p < P; ++p)
+        if (psi_out && psi_out[p])
+            BLQH_HIP(hipMemcpyAsync(psi_out[p], pl.psi.get() + hp[p].voff, hp[p].n * sizeof(double2),
+                                    hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    if (timed) {
+        for (int k = 0; k < ntimed; ++k) {
+            float ms = 0;
+            BLQH_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+            prof.ms += ms;
+        }
+        prof.launches += ntimed;
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int p = 0; p < P; ++p) {
+        double erq = 0;
+        for (int64_t i = b->item0[p]; i < b->item0[p + 1]; ++i) erq += hp2[i];
+        // bytes of every K1 launch this problem took part in (speculative steps included)
+        if (timed) prof.bytes += (double)steps_run[p] * (20.0 * b->hs[p]->nnz + 56.0 * hp[p].n + 8.0);
+        blqh_lanczos_result& r = res[p];
+        std::memset(&r, 0, sizeof(r));
+        r.E0 = E0[p];
+        r.E_rq = erq;
+        r.dE = dE[p];
+        r.iterations = m[p];
+        r.converged = conv[p];
+        r.passes = 2;
+        r.seconds = secs;
+        r.residual = resid[p];
+    }
+}

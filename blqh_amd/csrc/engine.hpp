@@ -19,7 +19,7 @@ struct blqh_basis {
     int64_t n = 0;                   // basis states (orbit representatives)
     blqh::DevBuf<uint64_t> id;       // representatives, ascending
     blqh::DevBuf<int16_t> period;    // orbit period (basis_C)
-    blqh::DevBuf<int32_t> lut;       // sector word rank -> (row << 6 | q << 1 | sign bit), -1 = none
+    blqh::DevBuf<int32_t> lut;       // sector word rank -> (row << 5) | shift q of its orbit member, -1 = none
     std::vector<uint64_t> h_id;
     std::vector<int16_t> h_period;
     double seconds_host = 0, seconds_device = 0;

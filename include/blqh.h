@@ -133,6 +133,27 @@ typedef struct blqh_lanczos_result {
 int blqh_lanczos(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_result* r,
                  double* psi, double* alpha, double* beta);
 
+/* ---- Batched solves: the sweeps over sectors and parameter points of
+ *      BLQH_without_spin/main.cpp:76-165 and BLQH_with_spin/main.cpp:158-302,
+ *      each of which runs the three passes above on one Hamiltonian at a time.
+ * A batch groups resident Hamiltonians of any sizes (they stay owned by the
+ * caller and must outlive the batch).  blqh_batch_lanczos advances every
+ * problem through the same launches (steps captured in a hipGraph, replayed
+ * every `check_every` steps), stops each problem by the rule of blqh_lanczos,
+ * keeps all Krylov vectors in HBM ((lambda+1) * nrows * 16 bytes per problem;
+ * BLQH_ENOMEM when they do not fit) and reconstructs each ground state.
+ * seeds: per-problem mt19937 seeds (NULL: o->seed for all); starts: per-problem
+ * start vectors (NULL array or NULL entry: seeded); psi: per-problem outputs
+ * (2*nrows doubles; NULL array or entry: not copied); res[count]. ---- */
+typedef struct blqh_batch blqh_batch;
+int blqh_batch_create(blqh_hamil* const* hs, int32_t count, blqh_batch** out);
+int blqh_batch_lanczos(blqh_batch* b, const blqh_lanczos_opts* o, const uint32_t* seeds,
+                       const double* const* starts, blqh_lanczos_result* res, double* const* psi);
+/* Lanes per row of the batched SpMV (4, 8, 16 or 32; autotuned at create). */
+int blqh_batch_get_lanes(const blqh_batch* b, int32_t* lanes);
+int blqh_batch_set_lanes(blqh_batch* b, int32_t lanes);
+void blqh_batch_destroy(blqh_batch* b);
+
 /* Dense cross-check: hamil::diag + ground_state_energy
  * (BLQH_without_spin/hamiltonian.cpp:341-353,413-432).  Diagonalises the dense
  * matrix with rocSOLVER zheevd in the reference's convention (row-major H read
