@@ -44,6 +44,21 @@ WORKLOADS = {
 }
 
 
+# Sweeps of all K-sectors x parameter points, solved by the batched engine.
+SWEEPS = {
+    "c3": dict(S=2, nphi=12, nel=12, nu=-1, nsu=-1, dv=0.0, dz=0.0,
+               sectors=[(J, K) for J in range(12) for K in range(12)],
+               points=[(round(0.2 * i, 1), ds) for i in range(16) for ds in (0.0, 0.05, 0.1)],
+               label="spinless bilayer Nphi=12 Ne=12 all densities, all 144 (J,K) sectors, "
+                     "d in {0,0.2,..,3.0} x DSAS in {0,0.05,0.1}"),
+    "c5": dict(S=4, nphi=8, nel=8, nu=4, nsu=4, dv=0.02, dz=0.01,
+               sectors=[(J, K) for J in range(8) for K in range(8)],
+               points=[(d, 0.05) for d in (0.5, 1.0, 1.5, 2.0)],
+               label="spinful bilayer Nphi=8 Ne=8 nel_up=4 nel_su=4, all 64 (J,K) sectors, "
+                     "DZ=0.01 DSAS=0.05 DV=0.02, d in {0.5,1.0,1.5,2.0}"),
+}
+
+
 def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
@@ -93,12 +108,173 @@ def cpu_baseline(basis_ids, periods, wl, n, nnz, iters_ref=200, budget_s=20.0):
     }
 
 
+def sweep_cpu_baseline(sw, iters_ref=200):
+    """Oracle (CPU port of the reference: dense-row build, fixed 200-step
+    three-pass Lanczos) on one sector of the sweep, 16 threads."""
+    import oracle as O
+    J, K = sw["sectors"][0]
+    d, ds = sw["points"][0]
+    sec = O.sector(S=sw["S"], nphi=sw["nphi"], nel=sw["nel"], nel_up=sw["nu"], nel_su=sw["nsu"], J=J, K=K)
+    par = O.params(sw["nphi"], d=d, dsas=ds, dv=sw["dv"], dz=sw["dz"])
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.time()
+    OB = O.Basis(sec)
+    t_basis = time.time() - t0
+    t0 = time.time()
+    OH = O.Hamiltonian(OB, par, nthreads=threads)
+    t_build = time.time() - t0
+    t0 = time.time()
+    OH.lanczos(lam=iters_ref, seed=1)
+    t_solve = time.time() - t0
+    return {
+        "value": 1.0 / t_solve,
+        "unit": "K-sector ground states/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle on sector (J,K)=({J},{K}) at d={d}, DSAS={ds} (N={len(OB)}, nnz={OH.nnz}): "
+                   f"fixed {iters_ref}-step three-pass Lanczos {t_solve:.2f} s (solve only, H resident); "
+                   f"row build {t_build:.2f} s, basis {t_basis:.2f} s"),
+    }
+
+
+def run_sweep(args):
+    import blqh_amd
+    sw = SWEEPS[args.workload]
+    S, nphi, nel = sw["S"], sw["nphi"], sw["nel"]
+    log(f"workload {args.workload}: {sw['label']}")
+    t0 = time.time()
+    bases = [blqh_amd.Basis(nphi, nel, sw["nu"], J, K, nspecies=S, nel_su=sw["nsu"]) for J, K in sw["sectors"]]
+    t_basis = time.time() - t0
+    log(f"{len(bases)} bases, N {min(b.n for b in bases)}..{max(b.n for b in bases)}, {t_basis:.1f} s")
+    pts = sw["points"]
+    nwarm = max(1, args.warmup)
+    order = pts[:nwarm] + pts[nwarm:nwarm + args.steps]
+    seeds = list(range(1, len(bases) + 1))
+
+    def build(d, ds):
+        t = time.time()
+        hs = [blqh_amd.Hamiltonian(B, blqh_amd.params(nphi, d=d, delta_sas=ds, delta_v=sw["dv"],
+                                                       delta_z=sw["dz"])) for B in bases]
+        return hs, time.time() - t
+
+    plain = None
+    batch = None
+    solved = solves = 0
+    t_solve = t_build = 0.0
+    iters, e_by_point = [], {}
+    for k, (d, ds) in enumerate(order):
+        hs, tb = build(d, ds)
+        warm = k < nwarm
+        prev = batch.hams if batch is not None else None  # freed after the timed solve
+        t = time.perf_counter()
+        if batch is None:
+            batch = blqh_amd.Batch(hs)  # autotunes the batched SpMV once for the sweep
+        else:
+            batch.rebind(hs)  # same sectors, next parameter point: pools and replay kept
+        res = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=args.check_every, want_psi=False)
+        dt = time.perf_counter() - t
+        t = time.time()
+        del prev  # the previous point's Hamiltonians (device frees)
+        tb += time.time() - t
+        ok = sum(int(r["converged"] == 1 and r["dE"] < 1e-10) for r in res)
+        log(f"{'warmup' if warm else 'point'} d={d} DSAS={ds}: build {tb:.2f} s, batched solve {dt:.3f} s, "
+            f"{ok}/{len(res)} converged, iterations {min(r['iterations'] for r in res)}.."
+            f"{max(r['iterations'] for r in res)}")
+        if warm and plain is None:
+            # plain single-problem solves of the same Hamiltonians (each autotuned by a first solve)
+            for i, H in enumerate(hs):
+                H.lanczos(lam=args.lam, seed=seeds[i], tol=1e-10, check_every=args.check_every, want_psi=False)
+            tp = time.perf_counter()
+            pres = [H.lanczos(lam=args.lam, seed=seeds[i], tol=1e-10, check_every=args.check_every, want_psi=False)
+                    for i, H in enumerate(hs)]
+            tp = time.perf_counter() - tp
+            pok = sum(int(r["converged"] == 1 and r["dE"] < 1e-10) for r in pres)
+            dmax = max(abs(a["E0"] - b["E0"]) / abs(b["E0"]) for a, b in zip(res, pres))
+            plain = {"solves": len(pres), "seconds": tp, "rate": pok / tp, "max_rel_dE0_vs_batched": dmax}
+            log(f"plain: {len(pres)} single-problem solves in {tp:.3f} s ({pok / tp:.1f}/s); "
+                f"max |E0 batched - plain|/|E0| = {dmax:.1e}")
+        if not warm:
+            solved += ok
+            solves += len(res)
+            t_solve += dt
+            t_build += tb
+            iters += [r["iterations"] for r in res]
+        if k == len(order) - 1:
+            # roofline pass: one more solve with HIP events around every batched K1 launch
+            blqh_amd.profile_reset()
+            blqh_amd.profile_enable(True)
+            batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=args.check_every, want_psi=False)
+            blqh_amd.profile_enable(False)
+            launches, kms, kbytes = blqh_amd.profile_read()
+            nnz_total = sum(H.nnz for H in hs)
+        del hs
+    avg_ms = kms / max(launches, 1)
+    achieved = (kbytes / max(launches, 1)) / (avg_ms * 1e-3) / 1e9 if launches else 0.0
+    value = solved / t_solve if t_solve > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "K-sector ground states/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": nwarm,
+        "ms_per_step": t_solve / max(1, args.steps) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": sw["label"],
+            "step": "all sectors at one parameter point, batched solve with H resident",
+            "sectors": len(bases),
+            "points_timed": args.steps,
+            "solves_timed": solves,
+            "converged": solved,
+            "lanczos_cap": args.lam,
+            "tolerance": 1e-10,
+            "check_every": args.check_every,
+            "iterations_mean": float(np.mean(iters)) if iters else None,
+            "batched_rate": value,
+            "plain_rate": plain["rate"] if plain else None,
+            "batched_over_plain": value / plain["rate"] if plain and plain["rate"] else None,
+            "plain": plain,
+            "end_to_end_rate_incl_build": solved / (t_solve + t_build) if t_solve + t_build > 0 else None,
+            "build_s_per_point": t_build / max(1, args.steps),  # build + teardown of the point's Hamiltonians
+            "basis_s_once": round(t_basis, 2),
+            "batched_spmv_lanes": batch.lanes,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_b1<LZ=true> (batched fused SpMV + Lanczos K1)",
+            "avg_launch_ms": avg_ms,
+            "launches": launches,
+            "algorithmic_bytes_per_launch": kbytes / max(launches, 1),
+            "batch_nnz": nnz_total,
+            "measured": "separate profiled solve of the last point (plain launches with HIP events)",
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle on one sector)")
+        try:
+            out["cpu_baseline"] = sweep_cpu_baseline(sw, iters_ref=args.lam)
+        except Exception as e:  # report, never hide, a failed baseline
+            out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + sorted(SWEEPS))
+    ap.add_argument("--check-every", type=int, default=5)
     ap.add_argument("--lam", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -115,6 +291,10 @@ def main():
               "sectors and parameter points shard across GPUs without communication, see DESIGN.md",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload in SWEEPS:
+        run_sweep(args)
+        return
 
     import blqh_amd
 

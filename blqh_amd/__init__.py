@@ -69,9 +69,10 @@ _SIGS = {
     "blqh_spmv_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "blqh_lanczos": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.POINTER(LanczosResult), C.c_void_p,
                                C.c_void_p, C.c_void_p]),
-    "blqh_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]),
+    "blqh_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "blqh_batch_lanczos": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.c_void_p, C.c_void_p,
                                      C.POINTER(LanczosResult), C.c_void_p]),
+    "blqh_batch_rebind": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32]),
     "blqh_batch_get_lanes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "blqh_batch_set_lanes": (C.c_int, [C.c_void_p, C.c_int32]),
     "blqh_batch_destroy": (None, [C.c_void_p]),
@@ -282,13 +283,21 @@ class Batch:
     """Resident Hamiltonians solved together (blqh_batch_*).  The Hamiltonian
     objects are kept alive by the batch."""
 
-    def __init__(self, hams):
+    def __init__(self, hams, lanes=0):
+        """lanes=0 autotunes the batched SpMV; otherwise 4, 8, 16, 32 or 64."""
         self.hams = list(hams)
         self._lib = lib()
         arr = (C.c_void_p * len(self.hams))(*[H.h.value for H in self.hams])
         h = C.c_void_p()
-        _check(self._lib.blqh_batch_create(arr, len(self.hams), C.byref(h)))
+        _check(self._lib.blqh_batch_create(arr, len(self.hams), lanes, C.byref(h)))
         self.h = h
+
+    def rebind(self, hams):
+        """Members of the same sizes, same order (next point of a sweep)."""
+        hams = list(hams)
+        arr = (C.c_void_p * len(hams))(*[H.h.value for H in hams])
+        _check(self._lib.blqh_batch_rebind(self.h, arr, len(hams)))
+        self.hams = hams
 
     @property
     def lanes(self):

@@ -14,8 +14,9 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil
 void blqh_spmv_device_impl(blqh_hamil* H, const void* x, void* y);
 void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_result& r, double* psi, double* alpha,
                        double* beta);
-blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count);
+blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count, int lanes);
 void blqh_batch_destroy_impl(blqh_batch* b);
+void blqh_batch_rebind_impl(blqh_batch* b, blqh_hamil* const* hs, int count);
 int blqh_batch_lanes_impl(const blqh_batch* b);
 void blqh_batch_set_lanes_impl(blqh_batch* b, int lanes);
 void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const uint32_t* seeds,
@@ -214,10 +215,15 @@ int blqh_lanczos(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_r
     });
 }
 
-int blqh_batch_create(blqh_hamil* const* hs, int32_t count, blqh_batch** out) {
+int blqh_batch_create(blqh_hamil* const* hs, int32_t count, int32_t lanes, blqh_batch** out) {
     if (!hs || !out || count < 1) return fail(BLQH_EINVAL, "bad argument");
     *out = nullptr;
-    return guarded([&] { *out = blqh_batch_create_impl(hs, count); });
+    return guarded([&] { *out = blqh_batch_create_impl(hs, count, lanes); });
+}
+
+int blqh_batch_rebind(blqh_batch* b, blqh_hamil* const* hs, int32_t count) {
+    if (!b || !hs) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] { blqh_batch_rebind_impl(b, hs, count); });
 }
 
 int blqh_batch_lanczos(blqh_batch* b, const blqh_lanczos_opts* o, const uint32_t* seeds, const double* const* starts,

@@ -23,6 +23,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 
@@ -179,6 +181,73 @@ __global__ void __launch_bounds__(kBT) k_b1(BState S, int jrel, const double2* _
     }
 }
 
+// Same contract as k_b1 over panels in step-major order (jag_row_sum): wave w
+// of the workgroup owns the item's w-th 64-row slice, one lane per row.
+// REAL: launched once for the problems with real panel values and once for
+// the complex ones (each skips the other kind), so each instantiation holds
+// only its own registers.
+template <bool LZ, bool REAL>
+__global__ void __launch_bounds__(kBT) k_b1j(BState S, int jrel, const double2* __restrict__ xs,
+                                             double2* __restrict__ ys, double* __restrict__ part) {
+    __shared__ double sh[kBT / 64];
+    const int j = LZ ? *S.jbase + jrel : 0;
+    if (LZ && j >= S.cap) return;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int64_t it = blockIdx.x; it < S.nitems; it += gridDim.x) {
+        const int p = S.item_prob[it];
+        if (!S.active[p]) continue;  // uniform over the workgroup
+        const BProb P = S.probs[p];
+        if (P.real != (REAL ? 1 : 0)) continue;
+        const double2* x;
+        double2* y;
+        const double2* Wp = nullptr;
+        double ib = 1.0, cp = 0.0;
+        if (LZ) {
+            const int64_t ld = S.cap + 1;
+            ib = S.invb[p * ld + j];
+            cp = j > 0 ? S.beta[p * ld + j] * S.invb[p * ld + j - 1] : 0.0;
+            x = P.W + (int64_t)j * P.n;
+            Wp = j > 0 ? x - P.n : x;
+            y = S.u + P.voff;
+        } else {
+            x = xs + P.voff;
+            y = ys + P.voff;
+        }
+        const int64_t row0 = (it - P.item0) * kItemRows + wv * 64;
+        double acc = 0.0;
+        if (row0 < P.n) {  // uniform over the wavefront
+            const int64_t n = P.n, myrow = row0 + lane;
+            const int64_t rend = row0 + 64 < n ? row0 + 64 : n;
+            double ar = 0.0, ai = 0.0;
+            for (int pan = 0; pan < P.npan; ++pan) {
+                const uint32_t* rp = P.prow + (int64_t)pan * (n + 1);
+                const uint32_t s = myrow < n ? rp[myrow] : rp[rend];
+                const uint32_t e = myrow < n ? rp[myrow + 1] : rp[rend];
+                const int64_t run = P.pbase[pan] + (uint32_t)__builtin_amdgcn_readfirstlane((int)s);  // rp[row0]
+                const double2* xp = x + (int64_t)pan * kPanelWidth;
+                jag_row_sum<REAL>(P.pcol + run, P.pvr + run, P.pvc + run, xp, e - s, ar, ai);
+            }
+            if (myrow < n) {
+                double ur = ar, ui = ai;
+                if (LZ) {
+                    ur = ib * ar;
+                    ui = ib * ai;
+                    if (j > 0) {
+                        const double2 q = Wp[myrow];
+                        ur -= cp * q.x;
+                        ui -= cp * q.y;
+                    }
+                }
+                y[myrow] = make_double2(ur, ui);
+                const double2 w = x[myrow];
+                acc = w.x * ur + w.y * ui;
+            }
+        }
+        const double t = block_sum(acc, sh);
+        if (threadIdx.x == 0) part[it] = t;
+    }
+}
+
 // K2: alpha_j, W_{j+1} = u - alpha_j W_j / beta_j, |W_{j+1}|^2 partials
 __global__ void __launch_bounds__(kBT) k_b2(BState S, int jrel) {
     __shared__ double sh[kBT / 64];
@@ -279,6 +348,16 @@ __global__ void __launch_bounds__(kBT) k_bscale(BState S, const double* __restri
 
 }  // namespace
 
+namespace {
+// Pooled per-problem scalars and vectors of a batched solve.
+struct Pools {
+    DevBuf<BProb> probs;
+    DevBuf<int32_t> active, jbase, mvec;
+    DevBuf<double> alpha, beta, invb, part1, part2, coef, scale;
+    DevBuf<double2> u, psi, W;
+};
+}  // namespace
+
 struct blqh_batch {
     std::vector<blqh_hamil*> hs;
     std::vector<BProb> h_probs;
@@ -286,16 +365,34 @@ struct blqh_batch {
     int64_t rows = 0, nitems = 0, nnz = 0;
     DevBuf<int32_t> item_prob;
     int lpr = 0;                 // lanes per row of k_b1 (autotuned)
+    bool has_real = false, has_complex = false;  // value kinds among the members
     int grid = 0;
+    // kept between solves of the same lambda: device pools, the captured
+    // replay (for one check interval and lane count) and pinned staging
+    Pools pl;
+    int pool_cap = -1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_chk = -1, g_lpr = -1, g_cap = -1, g_kinds = -1;
+    char* pin = nullptr;
+    DevBuf<char> relayout;       // scratch for reordering members' panels
+    ~blqh_batch() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (pin) (void)hipHostFree(pin);
+    }
 };
 
 namespace {
 
 // Row-major panel layout and current panel buffers of every member.
 void refresh_probs(blqh_batch* b) {
+    b->has_real = b->has_complex = false;
+    for (blqh_hamil* H : b->hs) (H->pan.real ? b->has_real : b->has_complex) = true;
+    // k_b1j reads step-major runs, k_b1 row-major ones: one relayout launch for all members
+    set_panel_layouts(b->hs.data(), (int)b->hs.size(), b->lpr == 64, b->relayout.get(), b->relayout.bytes());
     for (size_t p = 0; p < b->hs.size(); ++p) {
         blqh_hamil* H = b->hs[p];
-        set_panel_layout(H, false);  // the batched kernels read row-major panel runs
         BProb& d = b->h_probs[p];
         d.pbase = H->pan.pbase.get();
         d.prow = H->pan.prow.get();
@@ -310,6 +407,10 @@ void launch_b1(const blqh_batch* b, const BState& S, int jrel, const double2* xs
                hipStream_t st) {
     const dim3 g(b->grid), t(kBT);
     switch (b->lpr) {
+        case 64:
+            if (b->has_real) hipLaunchKernelGGL((k_b1j<LZ, true>), g, t, 0, st, S, jrel, xs, ys, part);
+            if (b->has_complex) hipLaunchKernelGGL((k_b1j<LZ, false>), g, t, 0, st, S, jrel, xs, ys, part);
+            break;
         case 4: hipLaunchKernelGGL((k_b1<4, LZ>), g, t, 0, st, S, jrel, xs, ys, part); break;
         case 8: hipLaunchKernelGGL((k_b1<8, LZ>), g, t, 0, st, S, jrel, xs, ys, part); break;
         case 16: hipLaunchKernelGGL((k_b1<16, LZ>), g, t, 0, st, S, jrel, xs, ys, part); break;
@@ -317,14 +418,6 @@ void launch_b1(const blqh_batch* b, const BState& S, int jrel, const double2* xs
         default: throw Error(BLQH_EINTERNAL, "unsupported batched SpMV lanes");
     }
 }
-
-// Pooled per-problem scalars and vectors of one batched solve.
-struct Pools {
-    DevBuf<BProb> probs;
-    DevBuf<int32_t> active, jbase, mvec;
-    DevBuf<double> alpha, beta, invb, part1, part2, coef, scale;
-    DevBuf<double2> u, psi, W;
-};
 
 BState make_state(const blqh_batch* b, Pools& pl, int cap) {
     BState S{};
@@ -362,8 +455,10 @@ void tune_batch(blqh_batch* b) {
     BLQH_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     int bl = 8;
-    for (int l : {4, 8, 16, 32}) {
+    for (int l : {4, 8, 16, 32, 64}) {  // 64: one lane per row, step-major panels (last: one relayout)
         b->lpr = l;
+        refresh_probs(b);
+        BLQH_HIP(hipMemcpyAsync(pl.probs.get(), b->h_probs.data(), P * sizeof(BProb), hipMemcpyHostToDevice, st));
         launch_b1<false>(b, S, 0, x.get(), y.get(), part.get(), st);
         BLQH_HIP(hipEventRecord(e0, st));
         for (int r = 0; r < 3; ++r) launch_b1<false>(b, S, 0, x.get(), y.get(), part.get(), st);
@@ -376,11 +471,14 @@ void tune_batch(blqh_batch* b) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->lpr = bl;
+    refresh_probs(b);
 }
 
 }  // namespace
 
-blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count) {
+blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count, int lanes) {
+    BLQH_REQUIRE(lanes == 0 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, BLQH_EINVAL,
+                 "unsupported lanes per row");
     BLQH_REQUIRE(count >= 1, BLQH_EINVAL, "empty batch");
     auto* b = new blqh_batch;
     try {
@@ -409,8 +507,14 @@ blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count) {
         b->item_prob.alloc(ip.size());
         BLQH_HIP(hipMemcpy(b->item_prob.get(), ip.data(), ip.size() * 4, hipMemcpyHostToDevice));
         b->grid = (int)std::max<int64_t>(1, std::min<int64_t>(b->nitems, (int64_t)runtime().num_cus * 8));
-        refresh_probs(b);
-        tune_batch(b);
+        b->relayout.alloc(panel_layout_scratch_bytes(b->hs.data(), count));
+        if (lanes) {
+            b->lpr = lanes;
+            refresh_probs(b);
+        } else {
+            refresh_probs(b);
+            tune_batch(b);
+        }
     } catch (...) {
         delete b;
         throw;
@@ -418,18 +522,48 @@ blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count) {
     return b;
 }
 
+// New members of the same shapes (a sweep's next parameter point): the work
+// items, pools, captured replay and lane choice are kept.
+void blqh_batch_rebind_impl(blqh_batch* b, blqh_hamil* const* hs, int count) {
+    BLQH_REQUIRE(count == (int)b->hs.size(), BLQH_EINVAL, "rebind needs as many Hamiltonians as the batch has");
+    int64_t nnz = 0;
+    for (int p = 0; p < count; ++p) {
+        BLQH_REQUIRE(hs[p] != nullptr && hs[p]->pan.built, BLQH_EINVAL, "null hamiltonian in batch");
+        BLQH_REQUIRE(hs[p]->n == b->h_probs[p].n && hs[p]->pan.npan == b->h_probs[p].npan, BLQH_EINVAL,
+                     "rebind needs Hamiltonians of the same sizes, in the same order");
+        nnz += hs[p]->nnz;
+    }
+    b->nnz = nnz;
+    b->hs.assign(hs, hs + count);
+    for (int p = 0; p < count; ++p) b->h_probs[p].real = hs[p]->pan.real ? 1 : 0;
+    const size_t scratch = panel_layout_scratch_bytes(b->hs.data(), count);
+    if (scratch > b->relayout.bytes()) b->relayout.alloc(scratch);
+    refresh_probs(b);
+}
+
 void blqh_batch_destroy_impl(blqh_batch* b) { delete b; }
 
 int blqh_batch_lanes_impl(const blqh_batch* b) { return b->lpr; }
 
 void blqh_batch_set_lanes_impl(blqh_batch* b, int lanes) {
-    BLQH_REQUIRE(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32, BLQH_EINVAL, "unsupported lanes per row");
+    BLQH_REQUIRE(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, BLQH_EINVAL,
+                 "unsupported lanes per row");
     b->lpr = lanes;
 }
 
 void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const uint32_t* seeds,
                              const double* const* starts, blqh_lanczos_result* res, double* const* psi_out) {
     auto t0 = std::chrono::steady_clock::now();
+    // BLQH_BATCH_TIMING=1 prints the host-side phases of every solve to stderr
+    const bool phase_log = std::getenv("BLQH_BATCH_TIMING") != nullptr;
+    auto tp = t0;
+    auto phase = [&](const char* what) {
+        if (!phase_log) return;
+        BLQH_HIP(hipStreamSynchronize(stream()));
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "blqh batch: %-12s %8.2f ms\n", what, std::chrono::duration<double>(now - tp).count() * 1e3);
+        tp = now;
+    };
     hipStream_t st = stream();
     const int P = (int)b->hs.size();
     const int cap = o.lambda;
@@ -441,11 +575,38 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     // member Hamiltonian may have reordered its panels (new buffers) since
     // the batch was created.
     refresh_probs(b);
-    // Krylov blocks of all problems
-    size_t vecs = 0;
-    for (int p = 0; p < P; ++p) vecs += (size_t)ld * b->h_probs[p].n;
-    Pools pl;
-    pl.W.alloc(vecs);
+    // Krylov blocks of all problems and the per-problem scalars (kept for the
+    // next solve with the same lambda)
+    Pools& pl = b->pl;
+    if (b->pool_cap != cap) {
+        if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+        b->pool_cap = -1;
+        b->pl = Pools{};
+        size_t vecs = 0;
+        for (int p = 0; p < P; ++p) vecs += (size_t)ld * b->h_probs[p].n;
+        pl.W.alloc(vecs);
+        pl.probs.alloc(P);
+        pl.active.alloc(P);
+        pl.jbase.alloc(1);
+        pl.mvec.alloc(P);
+        pl.alpha.alloc((size_t)P * cap);
+        pl.beta.alloc((size_t)P * ld);
+        pl.invb.alloc((size_t)P * ld);
+        pl.part1.alloc(b->nitems);
+        pl.part2.alloc(b->nitems);
+        pl.coef.alloc((size_t)P * cap);
+        pl.scale.alloc(P);
+        pl.u.alloc(b->rows);
+        pl.psi.alloc(b->rows);
+        if (b->pin) (void)hipHostFree(b->pin);
+        b->pin = nullptr;
+        const size_t pin_bytes = ((size_t)P * cap + (size_t)P * ld) * 8 + 2 * (size_t)P * 4;
+        BLQH_HIP(hipHostMalloc((void**)&b->pin, pin_bytes, hipHostMallocDefault));
+        b->pool_cap = cap;
+    }
     std::vector<BProb> hp = b->h_probs;
     {
         size_t off = 0;
@@ -454,21 +615,9 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
             off += (size_t)ld * hp[p].n;
         }
     }
-    pl.probs.alloc(P);
-    pl.active.alloc(P);
-    pl.jbase.alloc(1);
-    pl.mvec.alloc(P);
-    pl.alpha.alloc((size_t)P * cap);
-    pl.beta.alloc((size_t)P * ld);
-    pl.invb.alloc((size_t)P * ld);
-    pl.part1.alloc(b->nitems);
-    pl.part2.alloc(b->nitems);
-    pl.coef.alloc((size_t)P * cap);
-    pl.scale.alloc(P);
-    pl.u.alloc(b->rows);
-    pl.psi.alloc(b->rows);
     BLQH_HIP(hipMemcpyAsync(pl.probs.get(), hp.data(), P * sizeof(BProb), hipMemcpyHostToDevice, st));
 
+    phase("setup");
     // start vectors (host mt19937 or caller's), beta_0 = 1
     {
         std::vector<std::vector<double2>> v0(P);
@@ -500,6 +649,7 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
         BLQH_HIP(hipMemcpyAsync(pl.invb.get(), ones.data(), ones.size() * 8, hipMemcpyHostToDevice, st));
         BLQH_HIP(hipStreamSynchronize(st));  // v0 and ones are freed at scope exit
     }
+    phase("start");
     std::vector<int32_t> act(P, 1);
     BLQH_HIP(hipMemcpyAsync(pl.active.get(), act.data(), P * 4, hipMemcpyHostToDevice, st));
     BState S = make_state(b, pl, cap);
@@ -528,36 +678,34 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
         }
         hipLaunchKernelGGL(k_bnext, dim3(1), dim3(64), 0, st, pl.jbase.get(), chk);
     };
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
-    struct GraphGuard {
-        hipGraphExec_t& e;
-        hipGraph_t& g;
+    struct EvList {
         std::vector<hipEvent_t>& ev;
-        ~GraphGuard() {
-            if (e) (void)hipGraphExecDestroy(e);
-            if (g) (void)hipGraphDestroy(g);
+        ~EvList() {
             for (auto x : ev) (void)hipEventDestroy(x);
         }
-    } guard{gexec, graph, ev};
-    if (!timed) {
+    } evguard{ev};
+    const int kinds = (b->has_real ? 2 : 0) | (b->has_complex ? 1 : 0);  // which K1 launches a step has
+    if (!timed && (b->g_chk != chk || b->g_lpr != b->lpr || b->g_cap != cap || b->g_kinds != kinds || !b->gexec)) {
+        if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
         BLQH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         chunk(false, 0);
-        BLQH_HIP(hipStreamEndCapture(st, &graph));
-        BLQH_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        BLQH_HIP(hipStreamEndCapture(st, &b->graph));
+        BLQH_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        b->g_chk = chk;
+        b->g_lpr = b->lpr;
+        b->g_cap = cap;
+        b->g_kinds = kinds;
     }
+    hipGraphExec_t gexec = b->gexec;
     const int zero = 0;
     BLQH_HIP(hipMemcpy(pl.jbase.get(), &zero, 4, hipMemcpyHostToDevice));
 
     // pinned staging: alpha/beta read back per replay, active flags (two slots,
     // so a slot is rewritten only after its copy has run)
-    const size_t pin_bytes = ((size_t)P * cap + (size_t)P * ld) * 8 + 2 * (size_t)P * 4;
-    char* pin = nullptr;
-    BLQH_HIP(hipHostMalloc((void**)&pin, pin_bytes, hipHostMallocDefault));
-    struct PinGuard {
-        char* p;
-        ~PinGuard() { if (p) (void)hipHostFree(p); }
-    } pguard{pin};
+    char* pin = b->pin;
     double* ha = (double*)pin;
     double* hb = ha + (size_t)P * cap;
     int32_t* actpin = (int32_t*)(hb + (size_t)P * ld);
@@ -571,7 +719,10 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     std::vector<std::vector<double>> ritz(P);
     std::vector<int> m(P, 0), steps_run(P, 0);
     std::vector<char> conv(P, 0), live(P, 1);
-    std::vector<double> dE(P, INFINITY), resid(P, INFINITY), svec(cap);
+    std::vector<double> dE(P, INFINITY), resid(P, INFINITY);
+    // host threads for the convergence tests: worth it only for many problems
+    const int nthr = P >= 32 ? (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1;
+    std::vector<std::vector<double>> svecs(nthr, std::vector<double>(cap));
     // Replay r+1 is queued before the host tests replay r, so the host test
     // overlaps the GPU; a problem that converged in replay r runs one more
     // replay of steps that are never used.
@@ -594,9 +745,8 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
         if (more) enqueue(upto);  // speculative next replay
         BLQH_HIP(hipEventSynchronize(evc));
         // the single-problem solver's convergence test, per live problem
-        bool changed = false;
-        for (int p = 0; p < P; ++p) {
-            if (!live[p]) continue;
+        // (problems split over host threads; each writes only its own entries)
+        auto test = [&](int p, std::vector<double>& svec) {
             steps_run[p] = std::min(cap, upto + (more ? chk : 0));
             const double* a = ha + (size_t)p * cap;
             const double* bb = hb + (size_t)p * ld;
@@ -618,12 +768,29 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
                     if (resid[p] < o.res_tol) { conv[p] = 1; break; }
                 }
             }
-            if (conv[p]) {
+        };
+        {
+            const int nt = std::min(nthr, nlive);
+            if (nt <= 1) {
+                for (int p = 0; p < P; ++p)
+                    if (live[p]) test(p, svecs[0]);
+            } else {
+                std::vector<std::thread> th;
+                for (int t = 0; t < nt; ++t)
+                    th.emplace_back([&, t] {
+                        for (int p = t; p < P; p += nt)
+                            if (live[p]) test(p, svecs[t]);
+                    });
+                for (auto& t : th) t.join();
+            }
+        }
+        bool changed = false;
+        for (int p = 0; p < P; ++p)
+            if (live[p] && conv[p]) {
                 live[p] = 0;
                 --nlive;
                 changed = true;
             }
-        }
         done = upto;
         if (!more || nlive == 0) break;
         if (changed) {
@@ -636,6 +803,7 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     }
     BLQH_HIP(hipStreamSynchronize(st));  // a speculative replay may still run
 
+    phase("iterate");
     // pass 2 per problem, then psi = sum_l c_l conj(v_l), normalised, and Re<psi|H psi>
     std::vector<double> hinvb((size_t)P * ld);
     BLQH_HIP(hipMemcpyAsync(hinvb.data(), pl.invb.get(), hinvb.size() * 8, hipMemcpyDeviceToHost, st));
@@ -682,6 +850,7 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
         }
         prof.launches += ntimed;
     }
+    phase("finish");
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (int p = 0; p < P; ++p) {
         double erq = 0;

@@ -67,8 +67,16 @@ struct HostTables {
 };
 HostTables host_tables(const blqh_sector& s, const blqh_params& p);
 void build_panels(blqh_hamil* H);
-// Converts the panel runs to (jagged = true) or from step-major order; no-op when already there.
-void set_panel_layout(blqh_hamil* H, bool jagged);
+// Converts the panel runs to (jagged = true) or from step-major order in place
+// (the panel buffers keep their addresses); no-op when already there.  With a
+// scratch area of at least panel_layout_scratch_bytes(H) the conversion is
+// stream-ordered and allocates nothing; without one it allocates a temporary
+// and synchronises the library stream.
+void set_panel_layout(blqh_hamil* H, bool jagged, void* scratch = nullptr, size_t scratch_bytes = 0);
+size_t panel_layout_scratch_bytes(const blqh_hamil* H);
+// The same for many matrices in one launch (those already in the layout are skipped).
+void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scratch, size_t scratch_bytes);
+size_t panel_layout_scratch_bytes(blqh_hamil* const* hs, int count);
 
 // Row context with the basis' device pointers (tables left null).
 inline RowCtx make_row_ctx(const blqh_basis* B) {

@@ -26,6 +26,8 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <mutex>
+#include <thread>
 
 #include "engine.hpp"
 #include "rowterms.hpp"
@@ -42,29 +44,78 @@ static double vq(const blqh_params& p, int alpha, int qx, int qy) {
     return 2.0 * M_PI / q * std::exp(-q * q / 2.0) * std::pow(1.0 - std::exp(-q * q / 2.0), p.nLL * 2);
 }
 
+// The Coulomb tables depend on the geometry and d only, not on the sector,
+// so a sweep over all (J,K) sectors at one parameter point computes them once.
+// The interlayer lattice sum runs over ~20 nphi/(d+0.01) terms per entry
+// (24,000 at d = 0, nphi = 12); entries are spread over threads, each entry
+// summed in the reference's order.
+namespace {
+struct CoulombKey {
+    int nphi, nspecies, nLL;
+    double lx, ly, d, theta_x;
+    bool operator==(const CoulombKey& o) const {
+        return nphi == o.nphi && nspecies == o.nspecies && nLL == o.nLL && lx == o.lx && ly == o.ly && d == o.d &&
+               theta_x == o.theta_x;
+    }
+};
+struct CoulombEntry {
+    CoulombKey key;
+    std::vector<double> C;
+    double Ec;
+};
+std::mutex g_coulomb_mu;
+std::vector<CoulombEntry> g_coulomb;  // most recent last, at most 8
+
+void coulomb_tables(const blqh_sector& s, const blqh_params& p, std::vector<double>& C, double& Ec) {
+    const CoulombKey key{s.nphi, s.nspecies, p.nLL, p.lx, p.ly, p.d, p.theta_x};
+    {
+        std::lock_guard<std::mutex> lk(g_coulomb_mu);
+        for (const auto& e : g_coulomb)
+            if (e.key == key) {
+                C = e.C;
+                Ec = e.Ec;
+                return;
+            }
+    }
+    const int nphi = s.nphi;
+    const double lx = p.lx, ly = p.ly;
+    Ec = -2.0;
+    for (int i = 0; i < nphi; ++i)
+        for (int j = 0; j < nphi; ++j)
+            if (!(i == 0 && j == 0)) Ec += std::erfc(std::sqrt(M_PI * (i * i * lx / ly + j * j * ly / lx)));
+    Ec /= std::sqrt(lx * ly);
+    C.assign(2 * nphi * nphi, 0.0);
+    auto entry = [&](int e) {
+        const int sx = e / nphi, qy = e % nphi;
+        double V = 0;
+        for (int qx = -nphi; qx <= nphi; ++qx)
+            if (!(qx == 0 && qy == 0)) V += 2.0 * vq(p, 0, qx, qy) * std::cos(2.0 * M_PI * sx * qx / nphi) / (2.0 * lx * ly);
+        C[sx * nphi + qy] = V;
+        V = 0;
+        const double arg = s.nspecies == 2 ? 2.0 * M_PI * sx + 2 * p.theta_x : 2.0 * M_PI * sx - p.theta_x;
+        const int q0 = (int)(-10 * nphi / (p.d + 0.01));
+        for (int qx = q0; qx < 10 * nphi / (p.d + 0.01); ++qx)
+            if (!(qx == 0 && qy == 0)) V += 2.0 * vq(p, 1, qx, qy) * std::cos(arg * qx / nphi) / (2.0 * lx * ly);
+        C[nphi * nphi + sx * nphi + qy] = V;
+    };
+    const int ne = nphi * nphi;
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            for (int e = t; e < ne; e += nt) entry(e);
+        });
+    for (auto& t : th) t.join();
+    std::lock_guard<std::mutex> lk(g_coulomb_mu);
+    if (g_coulomb.size() >= 8) g_coulomb.erase(g_coulomb.begin());
+    g_coulomb.push_back({key, C, Ec});
+}
+}  // namespace
+
 HostTables host_tables(const blqh_sector& s, const blqh_params& p) {
     HostTables H;
     const int nphi = s.nphi;
-    const double lx = p.lx, ly = p.ly;
-    H.Ec = -2.0;
-    for (int i = 0; i < nphi; ++i)
-        for (int j = 0; j < nphi; ++j)
-            if (!(i == 0 && j == 0)) H.Ec += std::erfc(std::sqrt(M_PI * (i * i * lx / ly + j * j * ly / lx)));
-    H.Ec /= std::sqrt(lx * ly);
-    H.C.assign(2 * nphi * nphi, 0.0);
-    for (int sx = 0; sx < nphi; ++sx)
-        for (int qy = 0; qy < nphi; ++qy) {
-            double V = 0;
-            for (int qx = -nphi; qx <= nphi; ++qx)
-                if (!(qx == 0 && qy == 0)) V += 2.0 * vq(p, 0, qx, qy) * std::cos(2.0 * M_PI * sx * qx / nphi) / (2.0 * lx * ly);
-            H.C[sx * nphi + qy] = V;
-            V = 0;
-            const double arg = s.nspecies == 2 ? 2.0 * M_PI * sx + 2 * p.theta_x : 2.0 * M_PI * sx - p.theta_x;
-            const int q0 = (int)(-10 * nphi / (p.d + 0.01));
-            for (int qx = q0; qx < 10 * nphi / (p.d + 0.01); ++qx)
-                if (!(qx == 0 && qy == 0)) V += 2.0 * vq(p, 1, qx, qy) * std::cos(arg * qx / nphi) / (2.0 * lx * ly);
-            H.C[nphi * nphi + sx * nphi + qy] = V;
-        }
+    coulomb_tables(s, p, H.C, H.Ec);
     H.FT.resize(nphi * nphi);
     for (int a = 0; a < nphi; ++a)
         for (int b = 0; b < nphi; ++b)

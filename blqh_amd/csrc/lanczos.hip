@@ -475,14 +475,10 @@ __global__ void __launch_bounds__(WPB * 64) k_sps(PanelView P, const double2* __
 }
 
 // ---- jagged panel SpMV --------------------------------------------------------------
-// Panels in step-major order (Panels::jagged): inside each (64-row slice,
-// panel) run, step t holds entry t of every row that has more than t entries
-// there, rows in order.  Lane l owns row l and walks its own entries; at step t
-// the active lanes read consecutive positions off_t + (active lanes below l),
-// so every matrix load is coalesced, each row is summed in column order in
-// registers, and no value crosses lanes.  Slice ownership and the panel sweep
-// are those of k_sps.  Four steps are issued together so that four loads per
-// stream and four x gathers are in flight per lane.
+// Panels in step-major order (Panels::jagged, read by jag_row_sum): lane l
+// owns row l of a 64-row slice and walks its own entries, every matrix load
+// coalesced, each row summed in column order in registers, no value crossing
+// lanes.  Slice ownership and the panel sweep are those of k_sps.
 template <bool REAL, bool LZ, int WPB, int MAXS>
 __global__ void __launch_bounds__(WPB * 64) k_jag(PanelView P, const double2* __restrict__ x,
                                                   double2* __restrict__ out, const double2* __restrict__ Wp, Scal sc,
@@ -517,48 +513,8 @@ __global__ void __launch_bounds__(WPB * 64) k_jag(PanelView P, const double2* __
                 const uint32_t e = myrow < n ? rp[myrow + 1] : rp[rend];
                 const uint32_t len = e - s;
                 const int64_t run = base + (uint32_t)__builtin_amdgcn_readfirstlane((int)s);  // lane 0: rp[row0]
-                const uint16_t* cr = P.pcol + run;
-                const double* vr = P.pvr + run;
-                const double2* vc = P.pvc + run;
                 double ar = 0.0, ai = 0.0;
-                uint32_t off = 0;
-                for (uint32_t t = 0;; t += 4) {
-                    uint64_t m[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) m[u] = __ballot(t + u < len);
-                    if (m[0] == 0) break;
-                    uint32_t q[4];
-                    q[0] = off + lanes_below(m[0]);
-                    uint32_t o = off + (uint32_t)__popcll(m[0]);
-#pragma unroll
-                    for (int u = 1; u < 4; ++u) {
-                        q[u] = o + lanes_below(m[u]);
-                        o += (uint32_t)__popcll(m[u]);
-                    }
-                    off = o;
-                    int c[4];
-                    double2 v[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const bool act = t + u < len;
-                        c[u] = act ? (int)__builtin_nontemporal_load(&cr[q[u]]) : 0;
-                        if (REAL) v[u] = make_double2(act ? ntload1(&vr[q[u]]) : 0.0, 0.0);
-                        else v[u] = act ? ntload(&vc[q[u]]) : make_double2(0.0, 0.0);
-                    }
-                    double2 xv[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) xv[u] = xp[c[u]];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        if (REAL) {
-                            ar += v[u].x * xv[u].x;
-                            ai += v[u].x * xv[u].y;
-                        } else {
-                            ar += v[u].x * xv[u].x - v[u].y * xv[u].y;
-                            ai += v[u].x * xv[u].y + v[u].y * xv[u].x;
-                        }
-                    }
-                }
+                jag_row_sum<REAL>(P.pcol + run, P.pvr + run, P.pvc + run, xp, len, ar, ai);
                 double2 tacc = my[k * 64 + lane];
                 tacc.x += ar;
                 tacc.y += ai;

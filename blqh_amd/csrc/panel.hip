@@ -78,8 +78,11 @@ __global__ void k_panel_rows(int64_t n, int npan, const int64_t* __restrict__ of
     if (blockIdx.x == 0 && threadIdx.x == 0) pbase[npan] = total;
 }
 
-// max |re| and max |im| of the values (two doubles via atomicMax on the bit pattern of non-negative doubles)
-__global__ void k_absmax(int64_t nnz, const double2* __restrict__ val, unsigned long long* __restrict__ out) {
+// max |re| and max |im| of the values (two doubles via atomicMax on the bit
+// pattern of non-negative doubles, one atomic pair per workgroup)
+__global__ void __launch_bounds__(kBlk) k_absmax(int64_t nnz, const double2* __restrict__ val,
+                                                 unsigned long long* __restrict__ out) {
+    __shared__ double sr[kBlk / 64], si[kBlk / 64];
     double mr = 0, mi = 0;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
         mr = fmax(mr, fabs(val[k].x));
@@ -90,6 +93,15 @@ __global__ void k_absmax(int64_t nnz, const double2* __restrict__ val, unsigned 
         mi = fmax(mi, __shfl_xor(mi, o, 64));
     }
     if ((threadIdx.x & 63) == 0) {
+        sr[threadIdx.x >> 6] = mr;
+        si[threadIdx.x >> 6] = mi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlk / 64; ++w) {
+            mr = fmax(mr, sr[w]);
+            mi = fmax(mi, si[w]);
+        }
         atomicMax(&out[0], (unsigned long long)__double_as_longlong(mr));
         atomicMax(&out[1], (unsigned long long)__double_as_longlong(mi));
     }
@@ -99,42 +111,68 @@ __global__ void k_absmax(int64_t nnz, const double2* __restrict__ val, unsigned 
 // order and step-major ("jagged") order: step t holds entry t of each row of
 // the slice that has more than t entries in the panel, rows in order.  One
 // wavefront per run, lane l <-> row l: at step t the rows still active take
-// consecutive positions, off_t + (active lanes below l).
-template <bool REAL>
-__global__ void k_panel_relayout(int64_t n, int npan, const int64_t* __restrict__ pbase,
-                                 const uint32_t* __restrict__ prow, const uint16_t* __restrict__ scol,
-                                 const double* __restrict__ svr, const double2* __restrict__ svc,
-                                 uint16_t* __restrict__ dcol, double* __restrict__ dvr, double2* __restrict__ dvc,
-                                 int to_jagged) {
+// consecutive positions, off_t + (active lanes below l).  Any number of
+// matrices in one launch (grid.y = matrix); each run is written to the
+// matrix's scratch arrays, four steps of loads in flight per lane, and
+// k_relayout_back copies the scratch over the panel arrays.
+__global__ void __launch_bounds__(kBlk) k_relayout_runs(const RelayoutJob* __restrict__ jobs, int to_jagged) {
+    const RelayoutJob J = jobs[blockIdx.y];
     const int lane = threadIdx.x & 63;
+    const int64_t n = J.n;
     const int64_t nslices = (n + 63) >> 6;
-    const int64_t total = nslices * npan;
+    const int64_t total = nslices * J.npan;
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < total; w += nw) {
         const int p = (int)(w / nslices);
         const int64_t slice = w - (int64_t)p * nslices;
-        const uint32_t* rp = prow + (int64_t)p * (n + 1);
+        const uint32_t* rp = J.prow + (int64_t)p * (n + 1);
         const int64_t row0 = slice * 64, myrow = row0 + lane;
         const int64_t rend = row0 + 64 < n ? row0 + 64 : n;
         const uint32_t s = myrow < n ? rp[myrow] : rp[rend];
         const uint32_t e = myrow < n ? rp[myrow + 1] : rp[rend];
         const uint32_t len = e - s;
-        const int64_t pb = pbase[p];
+        const int64_t pb = J.pbase[p];
         const int64_t run = pb + rp[row0];
         int64_t off = 0;
-        for (uint32_t t = 0;; ++t) {
-            const uint64_t m = __ballot(t < len);
-            if (m == 0) break;
-            if (t < len) {
-                const int64_t jg = run + off + lanes_below(m);
-                const int64_t rm = pb + s + t;
-                const int64_t from = to_jagged ? rm : jg, to = to_jagged ? jg : rm;
-                dcol[to] = scol[from];
-                if (REAL) dvr[to] = svr[from];
-                else dvc[to] = svc[from];
+        for (uint32_t t = 0;; t += 4) {
+            uint64_t m[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) m[u] = __ballot(t + u < len);
+            if (m[0] == 0) break;
+            int64_t from[4], to[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t jg = run + off + lanes_below(m[u]);
+                const int64_t rm = pb + s + t + u;
+                from[u] = to_jagged ? rm : jg;
+                to[u] = to_jagged ? jg : rm;
+                off += __popcll(m[u]);
             }
-            off += __popcll(m);
+            uint16_t c[4] = {0, 0, 0, 0};
+            double2 v[4] = {};
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (t + u < len) {
+                    c[u] = J.col[from[u]];
+                    v[u] = J.real ? make_double2(J.vr[from[u]], 0.0) : J.vc[from[u]];
+                }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (t + u < len) {
+                    J.tcol[to[u]] = c[u];
+                    if (J.real) J.tvr[to[u]] = v[u].x;
+                    else J.tvc[to[u]] = v[u];
+                }
         }
+    }
+}
+
+__global__ void __launch_bounds__(kBlk) k_relayout_back(const RelayoutJob* __restrict__ jobs) {
+    const RelayoutJob J = jobs[blockIdx.y];
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < J.nnz; k += (int64_t)gridDim.x * blockDim.x) {
+        J.col[k] = J.tcol[k];
+        if (J.real) J.vr[k] = J.tvr[k];
+        else J.vc[k] = J.tvc[k];
     }
 }
 
@@ -142,32 +180,80 @@ __global__ void k_panel_relayout(int64_t n, int npan, const int64_t* __restrict_
 
 namespace blqh {
 
-void set_panel_layout(blqh_hamil* H, bool jagged) {
-    Panels& P = H->pan;
-    if (!P.built || P.jagged == jagged) return;
+namespace {
+constexpr size_t kAlign = 256;
+size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+size_t entry_bytes(const blqh_hamil* H) {
+    const size_t m = (size_t)std::max<int64_t>(H->nnz, 1);
+    return align_up(m * sizeof(uint16_t)) + align_up(m * (H->pan.real ? sizeof(double) : sizeof(double2)));
+}
+}  // namespace
+
+size_t panel_layout_scratch_bytes(blqh_hamil* const* hs, int count) {
+    size_t b = align_up((size_t)count * sizeof(RelayoutJob));
+    for (int i = 0; i < count; ++i) b += entry_bytes(hs[i]);
+    return b;
+}
+
+void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scratch, size_t scratch_bytes) {
+    std::vector<blqh_hamil*> todo;
+    for (int i = 0; i < count; ++i)
+        if (hs[i]->pan.built && hs[i]->pan.jagged != jagged) todo.push_back(hs[i]);
+    if (todo.empty()) return;
+    BLQH_REQUIRE(todo.size() <= 65535, BLQH_ERANGE, "too many matrices for one panel relayout");
     hipStream_t st = stream();
-    const int64_t m = std::max<int64_t>(H->nnz, 1);
-    DevBuf<uint16_t> col(m);
-    DevBuf<double> vr;
-    DevBuf<double2> vc;
-    if (P.real) vr.alloc(m);
-    else vc.alloc(m);
-    const int64_t runs = ceil_div(H->n, 64) * P.npan;
-    const int g = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(runs, kBlk / 64), 65536));
-    if (P.real)
-        hipLaunchKernelGGL(k_panel_relayout<true>, dim3(g), dim3(kBlk), 0, st, H->n, P.npan, P.pbase.get(),
-                           P.prow.get(), P.pcol.get(), P.pvr.get(), P.pvc.get(), col.get(), vr.get(), vc.get(),
-                           jagged ? 1 : 0);
-    else
-        hipLaunchKernelGGL(k_panel_relayout<false>, dim3(g), dim3(kBlk), 0, st, H->n, P.npan, P.pbase.get(),
-                           P.prow.get(), P.pcol.get(), P.pvr.get(), P.pvc.get(), col.get(), vr.get(), vc.get(),
-                           jagged ? 1 : 0);
-    BLQH_HIP(hipGetLastError());
+    const size_t need = panel_layout_scratch_bytes(todo.data(), (int)todo.size());
+    DevBuf<char> own;
+    char* sc = (char*)scratch;
+    if (!sc || scratch_bytes < need) {
+        own.alloc(need);
+        sc = own.get();
+    }
+    std::vector<RelayoutJob> jobs(todo.size());
+    size_t off = align_up(todo.size() * sizeof(RelayoutJob));
+    int64_t max_runs = 1, max_nnz = 1;
+    for (size_t i = 0; i < todo.size(); ++i) {
+        blqh_hamil* H = todo[i];
+        RelayoutJob& J = jobs[i];
+        J.n = H->n;
+        J.npan = H->pan.npan;
+        J.real = H->pan.real ? 1 : 0;
+        J.nnz = H->nnz;
+        J.pbase = H->pan.pbase.get();
+        J.prow = H->pan.prow.get();
+        J.col = H->pan.pcol.get();
+        J.vr = H->pan.pvr.get();
+        J.vc = H->pan.pvc.get();
+        const size_t m = (size_t)std::max<int64_t>(H->nnz, 1);
+        J.tcol = (uint16_t*)(sc + off);
+        off += align_up(m * sizeof(uint16_t));
+        J.tvr = J.real ? (double*)(sc + off) : nullptr;
+        J.tvc = J.real ? nullptr : (double2*)(sc + off);
+        off += align_up(m * (J.real ? sizeof(double) : sizeof(double2)));
+        max_runs = std::max<int64_t>(max_runs, ceil_div(H->n, 64) * H->pan.npan);
+        max_nnz = std::max<int64_t>(max_nnz, H->nnz);
+    }
+    // earlier work on the library stream may still read this scratch; then a
+    // synchronous copy of the job table (the host vector goes out of scope)
     BLQH_HIP(hipStreamSynchronize(st));
-    P.pcol = std::move(col);
-    if (P.real) P.pvr = std::move(vr);
-    else P.pvc = std::move(vc);
-    P.jagged = jagged;
+    BLQH_HIP(hipMemcpy(sc, jobs.data(), jobs.size() * sizeof(RelayoutJob), hipMemcpyHostToDevice));
+    const RelayoutJob* dj = (const RelayoutJob*)sc;
+    const int gx = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(max_runs, kBlk / 64), 4096));
+    hipLaunchKernelGGL(k_relayout_runs, dim3(gx, (unsigned)jobs.size()), dim3(kBlk), 0, st, dj, jagged ? 1 : 0);
+    const int gb = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(max_nnz, kBlk), 4096));
+    hipLaunchKernelGGL(k_relayout_back, dim3(gb, (unsigned)jobs.size()), dim3(kBlk), 0, st, dj);
+    BLQH_HIP(hipGetLastError());
+    if (own.get()) BLQH_HIP(hipStreamSynchronize(st));  // the temporary is freed on return
+    for (auto* H : todo) H->pan.jagged = jagged;
+}
+
+void set_panel_layout(blqh_hamil* H, bool jagged, void* scratch, size_t scratch_bytes) {
+    set_panel_layouts(&H, 1, jagged, scratch, scratch_bytes);
+}
+
+size_t panel_layout_scratch_bytes(const blqh_hamil* H) {
+    blqh_hamil* h = const_cast<blqh_hamil*>(H);
+    return panel_layout_scratch_bytes(&h, 1);
 }
 
 void build_panels(blqh_hamil* H) {
@@ -178,7 +264,7 @@ void build_panels(blqh_hamil* H) {
     P.npan = npan;
     DevBuf<unsigned long long> mx(2);
     BLQH_HIP(hipMemsetAsync(mx.get(), 0, 16, st));
-    const int g = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(H->nnz, kBlk), 1), 65536);
+    const int g = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(H->nnz, kBlk * 8), 1), 2048);
     hipLaunchKernelGGL(k_absmax, dim3(g), dim3(kBlk), 0, st, H->nnz, H->val.get(), mx.get());
     unsigned long long hm[2];
     BLQH_HIP(hipMemcpyAsync(hm, mx.get(), 16, hipMemcpyDeviceToHost, st));

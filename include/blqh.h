@@ -146,10 +146,17 @@ int blqh_lanczos(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_r
  * start vectors (NULL array or NULL entry: seeded); psi: per-problem outputs
  * (2*nrows doubles; NULL array or entry: not copied); res[count]. ---- */
 typedef struct blqh_batch blqh_batch;
-int blqh_batch_create(blqh_hamil* const* hs, int32_t count, blqh_batch** out);
+/* lanes: 0 = autotune the batched SpMV on this batch, else as blqh_batch_set_lanes
+ * (a sweep tunes its first batch and passes the choice to the next ones). */
+int blqh_batch_create(blqh_hamil* const* hs, int32_t count, int32_t lanes, blqh_batch** out);
+/* Replace the members by Hamiltonians of the same sizes in the same order (the
+ * next point of a parameter sweep); device pools, the captured replay and the
+ * lane choice are kept. */
+int blqh_batch_rebind(blqh_batch* b, blqh_hamil* const* hs, int32_t count);
 int blqh_batch_lanczos(blqh_batch* b, const blqh_lanczos_opts* o, const uint32_t* seeds,
                        const double* const* starts, blqh_lanczos_result* res, double* const* psi);
-/* Lanes per row of the batched SpMV (4, 8, 16 or 32; autotuned at create). */
+/* Lanes per row of the batched SpMV (4, 8, 16 or 32 over row-major panels;
+ * 64: one lane per row over step-major panels; autotuned at create). */
 int blqh_batch_get_lanes(const blqh_batch* b, int32_t* lanes);
 int blqh_batch_set_lanes(blqh_batch* b, int32_t lanes);
 void blqh_batch_destroy(blqh_batch* b);

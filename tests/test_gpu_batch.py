@@ -100,7 +100,7 @@ def test_lane_counts_agree(batch, batch_result):
     seeds = [11 + i for i in range(len(PROBLEMS))]
     keep = batch.lanes
     try:
-        for lanes in (4, 32):
+        for lanes in (4, 32, 64):
             batch.lanes = lanes
             r = batch.lanczos(lam=250, seeds=seeds, check_every=4, want_psi=False)
             for a, b in zip(r, batch_result):
@@ -109,6 +109,20 @@ def test_lane_counts_agree(batch, batch_result):
         batch.lanes = keep
     with pytest.raises(blqh_amd.BlqhError):
         batch.lanes = 12
+    with pytest.raises(blqh_amd.BlqhError):
+        blqh_amd.Batch(batch.hams, lanes=12)
+
+
+def test_batch_created_with_fixed_lanes(hams, batch_result):
+    b = blqh_amd.Batch([H for _, _, H in hams], lanes=16)
+    assert b.lanes == 16
+    r = b.lanczos(lam=250, seeds=[11 + i for i in range(len(PROBLEMS))], check_every=4, want_psi=False)
+    for a, c in zip(r, batch_result):
+        assert a["E0"] == pytest.approx(c["E0"], rel=1e-11, abs=0)
+    # a second solve with another lambda re-allocates the pools
+    r2 = b.lanczos(lam=260, seeds=[11 + i for i in range(len(PROBLEMS))], check_every=4, want_psi=False)
+    for a, c in zip(r2, r):
+        assert a["E0"] == c["E0"] and a["iterations"] == c["iterations"]
 
 
 def test_fixed_lambda_runs_exactly_lambda_steps(hams, batch):
@@ -135,3 +149,23 @@ def test_single_problem_batch(hams):
     r = blqh_amd.Batch([H]).lanczos(lam=200, seeds=5)
     s = H.lanczos(lam=200, seed=5)
     assert r[0]["E0"] == pytest.approx(s["E0"], rel=1e-10, abs=0)
+
+
+def test_rebind_to_next_parameter_point(hams, batch):
+    """A sweep's next point: same sectors, new d; the rebound batch gives what
+    a freshly created batch gives, and a size mismatch is refused."""
+    new = []
+    for (name, S, nphi, nel, nu, nsu, J, K, d, dsas), (_, B, _) in zip(PROBLEMS, hams):
+        new.append(blqh_amd.Hamiltonian(B, blqh_amd.params(nphi, d=d + 0.25, delta_sas=dsas)))
+    seeds = [11 + i for i in range(len(PROBLEMS))]
+    fresh = blqh_amd.Batch(new, lanes=batch.lanes).lanczos(lam=250, seeds=seeds, check_every=4, want_psi=False)
+    old_members = batch.hams
+    try:
+        batch.rebind(new)
+        r = batch.lanczos(lam=250, seeds=seeds, check_every=4, want_psi=False)
+        for a, b in zip(r, fresh):
+            assert a["E0"] == b["E0"] and a["iterations"] == b["iterations"]
+        with pytest.raises(blqh_amd.BlqhError):
+            batch.rebind(new[::-1])
+    finally:
+        batch.rebind(old_members)
