@@ -51,6 +51,16 @@ class LanczosResult(C.Structure):
                 ("residual", C.c_double)]
 
 
+class MlpTrainOpts(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("lr_final", C.c_float),
+                ("seed", C.c_uint32)]
+
+
+class MlpTrainResult(C.Structure):
+    _fields_ = [("loss_first", C.c_double), ("loss_last", C.c_double), ("samples", C.c_int64),
+                ("steps", C.c_int32), ("seconds", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol here must exist in libblqh.so
 _SIGS = {
     "blqh_last_error": (C.c_char_p, []),
@@ -76,6 +86,16 @@ _SIGS = {
     "blqh_batch_get_lanes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "blqh_batch_set_lanes": (C.c_int, [C.c_void_p, C.c_int32]),
     "blqh_batch_destroy": (None, [C.c_void_p]),
+    "blqh_mlp_create": (C.c_int, [C.c_int32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "blqh_mlp_nparams": (C.c_int, [C.c_void_p]),
+    "blqh_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "blqh_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "blqh_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
+    "blqh_mlp_train": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.POINTER(MlpTrainOpts), C.POINTER(MlpTrainResult)]),
+    "blqh_mlp_start_vector": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p]),
+    "blqh_mlp_destroy": (None, [C.c_void_p]),
     "blqh_dense_ground_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]),
     "blqh_start_vector": (C.c_int, [C.c_int64, C.c_uint32, C.c_void_p]),
     "blqh_tridiag_lowest": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
@@ -340,6 +360,67 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             self._lib.blqh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        if getattr(self, "_lib", None) is not None:
+            self.close()
+
+
+class Mlp:
+    """Learned warm start (blqh_mlp_*): FP32 MLP over occupation bits and
+    (d, Delta_SAS) predicting log|psi| and the sign of psi."""
+
+    def __init__(self, nbits, seed=0):
+        self._lib = lib()
+        h = C.c_void_p()
+        _check(self._lib.blqh_mlp_create(nbits, seed, C.byref(h)))
+        self.h = h
+        self.nbits = nbits
+        self.nparams = self._lib.blqh_mlp_nparams(self.h)
+
+    @property
+    def params(self):
+        out = np.empty(self.nparams, np.float32)
+        _check(self._lib.blqh_mlp_get_params(self.h, _p(out)))
+        return out
+
+    @params.setter
+    def params(self, v):
+        v = np.ascontiguousarray(v, np.float32)
+        assert v.size == self.nparams
+        _check(self._lib.blqh_mlp_set_params(self.h, _p(v)))
+
+    def loss_grad(self, words, d, dsas, ya, ys):
+        """Mean loss and gradient over explicit states (all arrays of one length)."""
+        w = np.ascontiguousarray(words, np.uint64)
+        f = [np.ascontiguousarray(a, np.float32) for a in (d, dsas, ya, ys)]
+        loss = C.c_double()
+        g = np.empty(self.nparams, np.float32)
+        _check(self._lib.blqh_mlp_loss_grad(self.h, len(w), _p(w), _p(f[0]), _p(f[1]), _p(f[2]), _p(f[3]),
+                                            C.byref(loss), _p(g)))
+        return loss.value, g
+
+    def train(self, sets, steps=2000, batch=4096, lr=3e-3, lr_final=3e-4, seed=1):
+        """sets: list of (Basis, d, dsas, psi).  Returns a dict of the run."""
+        n = len(sets)
+        bases = (C.c_void_p * n)(*[b.h.value for b, _, _, _ in sets])
+        par = np.ascontiguousarray([[d, s] for _, d, s, _ in sets], np.float64)
+        keep = [np.ascontiguousarray(p, np.complex128) for _, _, _, p in sets]
+        psi = (C.c_void_p * n)(*[k.ctypes.data for k in keep])
+        o = MlpTrainOpts(steps, batch, lr, lr_final, seed)
+        r = MlpTrainResult()
+        _check(self._lib.blqh_mlp_train(self.h, n, bases, _p(par), psi, C.byref(o), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in MlpTrainResult._fields_}
+
+    def start_vector(self, basis, d, dsas):
+        out = np.empty(basis.n, np.complex128)
+        _check(self._lib.blqh_mlp_start_vector(self.h, basis.h, d, dsas, _p(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.blqh_mlp_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -21,6 +21,16 @@ int blqh_batch_lanes_impl(const blqh_batch* b);
 void blqh_batch_set_lanes_impl(blqh_batch* b, int lanes);
 void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const uint32_t* seeds,
                              const double* const* starts, blqh_lanczos_result* res, double* const* psi_out);
+blqh_mlp* blqh_mlp_create_impl(int nbits, uint32_t seed);
+void blqh_mlp_destroy_impl(blqh_mlp* M);
+int blqh_mlp_nparams_impl(const blqh_mlp* M);
+void blqh_mlp_get_params_impl(const blqh_mlp* M, float* out);
+void blqh_mlp_set_params_impl(blqh_mlp* M, const float* in);
+void blqh_mlp_loss_grad_impl(blqh_mlp* M, int64_t ns, const uint64_t* words, const float* pd, const float* ps,
+                             const float* ya, const float* ys, double* loss, float* grad);
+void blqh_mlp_train_impl(blqh_mlp* M, int nsets, const blqh_basis* const* bases, const double* params,
+                         const double* const* psi, const blqh_mlp_train_opts& o, blqh_mlp_train_result& r);
+void blqh_mlp_start_vector_impl(const blqh_mlp* M, const blqh_basis* B, double d, double dsas, double* out);
 void blqh_dense_ground_state_impl(const blqh_hamil* H, double* e0, double* e_printed, double* vec);
 void blqh_device_copy_impl(const void* src, void* dst, int64_t bytes);
 namespace blqh {
@@ -245,6 +255,54 @@ int blqh_batch_set_lanes(blqh_batch* b, int32_t lanes) {
 
 void blqh_batch_destroy(blqh_batch* b) {
     if (b) blqh_batch_destroy_impl(b);
+}
+
+int blqh_mlp_create(int32_t nbits, uint32_t seed, blqh_mlp** out) {
+    if (!out) return fail(BLQH_EINVAL, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        blqh::runtime();
+        *out = blqh_mlp_create_impl(nbits, seed);
+    });
+}
+
+int blqh_mlp_nparams(const blqh_mlp* m) {
+    if (!m) return fail(BLQH_EINVAL, "null network");
+    return blqh_mlp_nparams_impl(m);
+}
+
+int blqh_mlp_get_params(const blqh_mlp* m, float* out) {
+    if (!m || !out) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] { blqh_mlp_get_params_impl(m, out); });
+}
+
+int blqh_mlp_set_params(blqh_mlp* m, const float* in) {
+    if (!m || !in) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] { blqh_mlp_set_params_impl(m, in); });
+}
+
+int blqh_mlp_loss_grad(blqh_mlp* m, int64_t count, const uint64_t* words, const float* d, const float* dsas,
+                       const float* ya, const float* ys, double* loss, float* grad) {
+    if (!m || !words || !d || !dsas || !ya || !ys) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] { blqh_mlp_loss_grad_impl(m, count, words, d, dsas, ya, ys, loss, grad); });
+}
+
+int blqh_mlp_train(blqh_mlp* m, int32_t nsets, const blqh_basis* const* bases, const double* params,
+                   const double* const* psi, const blqh_mlp_train_opts* o, blqh_mlp_train_result* r) {
+    if (!m || !o || !r) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] {
+        std::memset(r, 0, sizeof(*r));
+        blqh_mlp_train_impl(m, nsets, bases, params, psi, *o, *r);
+    });
+}
+
+int blqh_mlp_start_vector(const blqh_mlp* m, const blqh_basis* b, double d, double dsas, double* out) {
+    if (!m || !b || !out) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] { blqh_mlp_start_vector_impl(m, b, d, dsas, out); });
+}
+
+void blqh_mlp_destroy(blqh_mlp* m) {
+    if (m) blqh_mlp_destroy_impl(m);
 }
 
 int blqh_dense_ground_state(const blqh_hamil* h, double* e0, double* e_printed, double* vec) {

@@ -161,6 +161,46 @@ int blqh_batch_get_lanes(const blqh_batch* b, int32_t* lanes);
 int blqh_batch_set_lanes(blqh_batch* b, int32_t lanes);
 void blqh_batch_destroy(blqh_batch* b);
 
+/* ---- Learned warm start (no counterpart in the reference: it supplies the
+ *      start vector that lanczos_hamiltonian.cpp:466-483 draws at random).
+ * An FP32 MLP over the nbits occupation bits of a basis word (as +-1) plus
+ * (d/3, 10 Delta_SAS): two hidden layers of 64 tanh units, outputs log|psi|
+ * and the logit of sign(psi) > 0.  Trained on the GPU on ground states the
+ * engine computed; its output is passed as blqh_lanczos_opts.start (or the
+ * batch's starts).  Parameters, flat FP32: W1[nbits+2][64], b1[64],
+ * W2[64][64], b2[64], W3[64][2], b3[2]. ---- */
+typedef struct blqh_mlp blqh_mlp;
+typedef struct blqh_mlp_train_opts {
+    int32_t steps;    /* optimiser steps (Adam) */
+    int32_t batch;    /* states per step, drawn with probability |psi|^2 from a uniformly chosen set */
+    float lr;         /* learning rate of the first step */
+    float lr_final;   /* learning rate of the last step (geometric decay); <= 0: constant */
+    uint32_t seed;    /* sampling seed */
+} blqh_mlp_train_opts;
+typedef struct blqh_mlp_train_result {
+    double loss_first;  /* minibatch loss before the first step */
+    double loss_last;   /* mean minibatch loss of the last (up to) 20 steps */
+    int64_t samples;    /* states in the training sets */
+    int32_t steps;
+    double seconds;
+} blqh_mlp_train_result;
+int blqh_mlp_create(int32_t nbits, uint32_t seed, blqh_mlp** out);  /* Glorot init from mt19937(seed) */
+int blqh_mlp_nparams(const blqh_mlp* m);
+int blqh_mlp_get_params(const blqh_mlp* m, float* out);
+int blqh_mlp_set_params(blqh_mlp* m, const float* in);             /* also resets the optimiser state */
+/* Mean loss over explicit states and its gradient (grad may be NULL):
+ * L = mean_s (o0_s - ya_s)^2 + softplus(o1_s) - ys_s o1_s. */
+int blqh_mlp_loss_grad(blqh_mlp* m, int64_t count, const uint64_t* words, const float* d, const float* dsas,
+                       const float* ya, const float* ys, double* loss, float* grad);
+/* Train on `nsets` ground states: set k = (bases[k], d = params[2k],
+ * Delta_SAS = params[2k+1], psi[k] = 2*nrows doubles as returned by
+ * blqh_lanczos); each state is gauge-fixed to real first. */
+int blqh_mlp_train(blqh_mlp* m, int32_t nsets, const blqh_basis* const* bases, const double* params,
+                   const double* const* psi, const blqh_mlp_train_opts* o, blqh_mlp_train_result* r);
+/* Normalised real start vector (2*nrows doubles) for a basis at (d, Delta_SAS). */
+int blqh_mlp_start_vector(const blqh_mlp* m, const blqh_basis* b, double d, double dsas, double* out);
+void blqh_mlp_destroy(blqh_mlp* m);
+
 /* Dense cross-check: hamil::diag + ground_state_energy
  * (BLQH_without_spin/hamiltonian.cpp:341-353,413-432).  Diagonalises the dense
  * matrix with rocSOLVER zheevd in the reference's convention (row-major H read
