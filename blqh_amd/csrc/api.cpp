@@ -190,6 +190,17 @@ int blqh_hamil_set_spmv(blqh_hamil* h, int32_t kind, int32_t lanes, int32_t thre
     if (kind == 3)
         ok = h->pan.built && lanes == 64 && (threads == 256 || threads == 512 || threads == 1024) && wpc >= 1 &&
              wpc <= 4 && (rows == 1 || rows == 2 || rows == 4 || rows == 8);
+    if (kind == 4) {
+        // the column-tile layout is built here on first use (one workgroup per CU)
+        int rc = guarded([&] { ok = blqh::build_tiles(h, blqh::runtime().num_cus); });
+        if (rc != BLQH_OK) return rc;
+        if (ok) {
+            lanes = 64;
+            threads = 1024;
+            wpc = 1;
+            rows = h->til.R;
+        }
+    }
     if (!ok) return fail(BLQH_EINVAL, "unsupported SpMV variant");
     h->spmv_kind = kind;
     h->spmv_lpr = kind < 0 ? 0 : lanes;

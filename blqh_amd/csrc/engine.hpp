@@ -33,6 +33,7 @@ struct blqh_hamil {
     blqh::DevBuf<int32_t> col;
     blqh::DevBuf<double2> val;
     blqh::Panels pan;                // panel-major copy for the SpMV hot loop
+    blqh::Tiles til;                 // column-tile copy (SpMV kind 4), built on demand
     int spmv_kind = -1;              // autotuned SpMV: 0 = row-major CSR, 1 = panel-major (-1 = not tuned)
     int spmv_lpr = 0;                // lanes per row of the chosen kernel
     int spmv_threads = 0;            // workgroup size of the panel kernel
@@ -67,6 +68,9 @@ struct HostTables {
 };
 HostTables host_tables(const blqh_sector& s, const blqh_params& p);
 void build_panels(blqh_hamil* H);
+// Column-tile layout for SpMV kind 4 on `G` workgroups; returns false (and
+// builds nothing) when the matrix does not fit it.
+bool build_tiles(blqh_hamil* H, int G);
 // Converts the panel runs to (jagged = true) or from step-major order in place
 // (the panel buffers keep their addresses); no-op when already there.  With a
 // scratch area of at least panel_layout_scratch_bytes(H) the conversion is

@@ -544,6 +544,121 @@ __global__ void __launch_bounds__(WPB * 64) k_jag(PanelView P, const double2* __
     if (LZ) grid_reduce(dacc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
 }
 
+// ---- column-tile SpMV (kind 4) ---------------------------------------------------------
+// Workgroup b (16 wavefronts) owns a row block; lane l of wave w keeps the
+// sums of its R rows (one per owned slice) in registers for the whole
+// product.  The columns are swept in tiles of kTileCols: the x tile is staged
+// in LDS (the next tile is loaded into registers while the current one is
+// consumed).  Within a tile the wave's entries are stored step-major across
+// its R rows (tiles.hip), so at step j the wave issues the loads of all R
+// rows before using any: enough bytes in flight per wave to stream HBM, and
+// every gather is an LDS read.  HBM traffic: the matrix stream, the u8 entry
+// counts, x once per XCD and the vectors of the epilogue.
+constexpr int kT4Threads = kTileWaves * 64;
+constexpr int kT4Pre = kTileCols / kT4Threads;  // x entries per thread per tile
+
+template <bool REAL, bool LZ, int R>
+__global__ void __launch_bounds__(kT4Threads) k_t4(TileView V, const double2* __restrict__ x,
+                                                   double2* __restrict__ out, const double2* __restrict__ Wp, Scal sc,
+                                                   int j, double* partials, unsigned* ticket) {
+    __shared__ double2 xs[kTileCols];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t n = V.n;
+    const int64_t nslices = (n + 63) >> 6;
+    const int64_t wid = (int64_t)blockIdx.x * kTileWaves + wv;
+    const int64_t s0 = (int64_t)blockIdx.x * V.SB;
+    const int64_t s1 = s0 + V.SB < nslices ? s0 + V.SB : nslices;
+    double ar[R], ai[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) ar[r] = ai[r] = 0.0;
+    int64_t ptr = V.wstart[wid];
+    const uint8_t* L = V.len + wid * (int64_t)V.ntiles * V.R * 64 + lane;
+    double2 pre[kT4Pre];
+    auto load_tile = [&](int t) {
+#pragma unroll
+        for (int k = 0; k < kT4Pre; ++k) {
+            const int64_t c = (int64_t)t * kTileCols + k * kT4Threads + threadIdx.x;
+            pre[k] = c < n ? x[c] : make_double2(0.0, 0.0);
+        }
+    };
+    load_tile(0);
+    for (int t = 0; t < V.ntiles; ++t) {
+        __syncthreads();  // every wave is done with the previous tile
+#pragma unroll
+        for (int k = 0; k < kT4Pre; ++k) xs[k * kT4Threads + threadIdx.x] = pre[k];
+        __syncthreads();
+        if (t + 1 < V.ntiles) load_tile(t + 1);  // in flight while this tile is consumed
+        uint32_t l[R];
+        uint32_t lmax = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            l[r] = r < V.R ? (uint32_t)L[((int64_t)t * V.R + r) * 64] : 0u;
+            lmax = l[r] > lmax ? l[r] : lmax;
+        }
+        const uint16_t* cr = V.col + ptr;
+        const double* vr = V.vr + ptr;
+        const double2* vc = V.vc + ptr;
+        uint32_t off = 0;
+        for (uint32_t st = 0; __ballot(st < lmax) != 0; ++st) {
+            uint32_t q[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint64_t m = __ballot(st < l[r]);
+                q[r] = off + lanes_below(m);
+                off += (uint32_t)__popcll(m);
+            }
+            int c[R];
+            double2 v[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const bool act = st < l[r];
+                c[r] = act ? (int)__builtin_nontemporal_load(&cr[q[r]]) : 0;
+                if (REAL) v[r] = make_double2(act ? ntload1(&vr[q[r]]) : 0.0, 0.0);
+                else v[r] = act ? ntload(&vc[q[r]]) : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double2 xv = xs[c[r]];
+                if (REAL) {
+                    ar[r] += v[r].x * xv.x;
+                    ai[r] += v[r].x * xv.y;
+                } else {
+                    ar[r] += v[r].x * xv.x - v[r].y * xv.y;
+                    ai[r] += v[r].x * xv.y + v[r].y * xv.x;
+                }
+            }
+        }
+        ptr += off;
+    }
+    double ib = 0, cp = 0;
+    if (LZ) {
+        ib = sc.invb[j];
+        cp = j > 0 ? sc.beta[j] * sc.invb[j - 1] : 0.0;
+    }
+    double dacc = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int64_t slice = s0 + wv + (int64_t)kTileWaves * r;
+        const int64_t row = slice * 64 + lane;
+        if (r < V.R && slice < s1 && row < n) {
+            if (LZ) {
+                double ur = ib * ar[r], ui = ib * ai[r];
+                if (j > 0) {
+                    const double2 q = Wp[row];
+                    ur -= cp * q.x;
+                    ui -= cp * q.y;
+                }
+                out[row] = make_double2(ur, ui);
+                const double2 w = x[row];
+                dacc += w.x * ur + w.y * ui;
+            } else {
+                out[row] = make_double2(ar[r], ai[r]);
+            }
+        }
+    }
+    if (LZ) grid_reduce(dacc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
+}
+
 // ---- launch helpers -----------------------------------------------------------
 int vec_grid(int64_t n) {
     const int ncu = runtime().num_cus;
@@ -673,8 +788,38 @@ void jag_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2
     throw Error(BLQH_EINTERNAL, "unsupported jagged SpMV plan");
 }
 
+// smallest compiled slice count per wave that covers the layout's R
+int t4_R(int R) {
+    for (int c : {1, 2, 3, 4, 6, 8, 9, 10, 12, 16})
+        if (c >= R) return c;
+    return -1;
+}
+
+template <bool LZ>
+void t4_dispatch(const blqh_hamil* H, const double2* x, double2* out, const double2* Wp, const Scal& sc, int j,
+                 double* partials, unsigned* ticket, hipStream_t st) {
+    const Tiles& T = H->til;
+    BLQH_REQUIRE(T.built, BLQH_EINTERNAL, "column-tile SpMV without its layout");
+    const TileView V = tview(T, H->n);
+    const dim3 grid(T.G), block(kT4Threads);
+#define BLQH_T4_CASE(RR)                                                                                    \
+    if (t4_R(T.R) == RR) {                                                                                  \
+        if (T.real)                                                                                         \
+            hipLaunchKernelGGL((k_t4<true, LZ, RR>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_t4<false, LZ, RR>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+        return;                                                                                             \
+    }
+    BLQH_T4_CASE(1) BLQH_T4_CASE(2) BLQH_T4_CASE(3) BLQH_T4_CASE(4) BLQH_T4_CASE(6) BLQH_T4_CASE(8)
+    BLQH_T4_CASE(9) BLQH_T4_CASE(10) BLQH_T4_CASE(12) BLQH_T4_CASE(16)
+#undef BLQH_T4_CASE
+    throw Error(BLQH_EINTERNAL, "unsupported column-tile SpMV layout");
+}
+
 void launch_spmv_plan(const blqh_hamil* H, const Plan& pl, const double2* x, double2* y, hipStream_t st) {
-    if (pl.kind == 3) {
+    if (pl.kind == 4) {
+        t4_dispatch<false>(H, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
+    } else if (pl.kind == 3) {
         jag_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
     } else if (pl.kind == 2) {
         sps_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
@@ -688,12 +833,15 @@ void launch_spmv_plan(const blqh_hamil* H, const Plan& pl, const double2* x, dou
 
 // grid size (for partial-sum buffers) of the fused K1 under a plan
 int lz1_grid(const blqh_hamil* H, const Plan& pl) {
+    if (pl.kind == 4) return H->til.G;
     return pl.kind >= 2 ? sps_grid(H, pl) : pl.kind == 1 ? panel_launch(H, pl).grid : spmv_grid(H->n, pl.lpr);
 }
 
 void launch_lz1(const blqh_hamil* H, const Plan& pl, const double2* W, const double2* Wp, double2* u, const Scal& sc,
                 int j, double* partials, unsigned* ticket, hipStream_t st) {
-    if (pl.kind == 3) {
+    if (pl.kind == 4) {
+        t4_dispatch<true>(H, W, u, Wp, sc, j, partials, ticket, st);
+    } else if (pl.kind == 3) {
         jag_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
     } else if (pl.kind == 2) {
         sps_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
@@ -711,7 +859,7 @@ void tune(blqh_hamil* H) {
     if (const char* env = std::getenv("BLQH_SPMV_PLAN")) {
         Plan p{};
         if (std::sscanf(env, "%d,%d,%d,%d,%d", &p.kind, &p.lpr, &p.threads, &p.wpc, &p.rows) == 5 &&
-            (p.kind == 0 || H->pan.built)) {
+            (p.kind == 0 || H->pan.built) && (p.kind != 4 || build_tiles(H, runtime().num_cus))) {
             H->spmv_kind = p.kind;
             H->spmv_lpr = p.lpr;
             H->spmv_threads = p.threads;
@@ -740,6 +888,9 @@ void tune(blqh_hamil* H) {
                 while (ms < need && ms < 8) ms <<= 1;
                 cands.push_back(Plan{2, 64, pt, wpc, ms});
             }
+    // column tiles: an x tile per LDS, worth it when x does not fit the caches
+    if (H->pan.built && H->n > 4 * kTileCols && build_tiles(H, runtime().num_cus))
+        cands.push_back(Plan{4, 64, kT4Threads, 1, H->til.R});
     if (H->pan.built)  // jagged kernel last: one layout conversion
         for (int pt : {256, 512, 1024})
             for (int wpc : {1, 2, 4}) {
@@ -785,10 +936,23 @@ void tune(blqh_hamil* H) {
 }
 
 // Tunes on first use (unless a variant is pinned) and puts the panels in the
-// entry order the chosen kernel reads.
+// entry order the chosen kernel reads; the column-tile copy is released when
+// another variant won.
 void autotune(blqh_hamil* H) {
-    if (H->spmv_kind < 0) tune(H);
-    if (H->spmv_kind >= 1) set_panel_layout(H, H->spmv_kind == 3);
+    if (H->spmv_kind < 0) {
+        tune(H);
+        if (H->spmv_kind != 4 && H->til.built) {
+            Tiles& T = H->til;
+            T.len.release();
+            T.wstart.release();
+            T.col.release();
+            T.vr.release();
+            T.vc.release();
+            T.built = false;
+            T.tried = false;
+        }
+    }
+    if (H->spmv_kind >= 1 && H->spmv_kind <= 3) set_panel_layout(H, H->spmv_kind == 3);
 }
 
 }  // namespace

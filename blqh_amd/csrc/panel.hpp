@@ -52,6 +52,44 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Column tiles (SpMV kind 4): the columns are cut into tiles of kTileCols
+// (an x tile of 128 KiB, staged in LDS), the rows into G blocks of SB 64-row
+// slices, one block per workgroup of kTileWaves wavefronts; wave w of block b
+// owns the block's slices w, w + 16, ... (R of them) and reads one contiguous
+// stream: for every tile t, for every owned slice r, the (slice, tile) run in
+// step-major order (see jag_row_sum).  len[wave][t][r][lane] holds each row's
+// entry count in the run (u8; the layout is not built when a count exceeds
+// 255), wstart[wave] the stream's first entry.
+constexpr int kTileCols = 8192;
+constexpr int kTileWaves = 16;
+constexpr int kTileMaxR = 16;
+
+struct Tiles {
+    bool built = false;
+    bool tried = false;   // build attempted (it may be unavailable for this matrix)
+    bool real = false;
+    int G = 0, SB = 0, R = 0, ntiles = 0;
+    DevBuf<uint8_t> len;     // [G * kTileWaves][ntiles][R][64]
+    DevBuf<int64_t> wstart;  // [G * kTileWaves]
+    DevBuf<uint16_t> col;    // [nnz] column - tile base
+    DevBuf<double> vr;       // [nnz] when real
+    DevBuf<double2> vc;      // [nnz] when complex
+};
+
+struct TileView {
+    int64_t n;
+    int G, SB, R, ntiles;
+    const uint8_t* len;
+    const int64_t* wstart;
+    const uint16_t* col;
+    const double* vr;
+    const double2* vc;
+};
+
+inline TileView tview(const Tiles& T, int64_t n) {
+    return TileView{n, T.G, T.SB, T.R, T.ntiles, T.len.get(), T.wstart.get(), T.col.get(), T.vr.get(), T.vc.get()};
+}
+
 // Row sums of one (64-row slice, panel) run in step-major order: step t
 // holds entry t of every row with more than t entries, rows in order.  Lane l
 // owns row l (`len` entries); at step t the active lanes read consecutive
@@ -59,54 +97,62 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
 // first entry, xp at the panel's slice of x.  Four steps per iteration, so
 // four loads per stream and four x gathers are in flight per lane.
 template <bool REAL>
-__device__ __forceinline__ void jag_row_sum(const uint16_t* __restrict__ cr, const double* __restrict__ vr,
-                                            const double2* __restrict__ vc, const double2* __restrict__ xp,
-                                            uint32_t len, double& ar, double& ai) {
+__device__ __forceinline__ uint32_t jag_row_sum(const uint16_t* __restrict__ cr, const double* __restrict__ vr,
+                                                const double2* __restrict__ vc, const double2* __restrict__ xp,
+                                                uint32_t len, double& ar, double& ai) {
     typedef double d2v_t __attribute__((ext_vector_type(2)));
+    // Software pipeline over groups of four steps: the column and value loads
+    // of group g+1 are issued before group g's gathers and products, so a
+    // wavefront always has the next group's loads in flight.
     uint32_t off = 0;
-    for (uint32_t t = 0;; t += 4) {
+    auto issue = [&](uint32_t t, int* c, double2* v, bool& any) {
         uint64_t m[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) m[u] = __ballot(t + u < len);
-        if (m[0] == 0) break;
-        uint32_t q[4];
-        q[0] = off + lanes_below(m[0]);
-        uint32_t o = off + (uint32_t)__popcll(m[0]);
-#pragma unroll
-        for (int u = 1; u < 4; ++u) {
-            q[u] = o + lanes_below(m[u]);
-            o += (uint32_t)__popcll(m[u]);
-        }
-        off = o;
-        int c[4];
-        double2 v[4];
+        any = m[0] != 0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
+            const uint32_t q = off + lanes_below(m[u]);
+            off += (uint32_t)__popcll(m[u]);
             const bool act = t + u < len;
-            c[u] = act ? (int)__builtin_nontemporal_load(&cr[q[u]]) : 0;
+            c[u] = act ? (int)__builtin_nontemporal_load(&cr[q]) : 0;
             if (REAL) {
-                v[u] = make_double2(act ? __builtin_nontemporal_load(&vr[q[u]]) : 0.0, 0.0);
+                v[u] = make_double2(act ? __builtin_nontemporal_load(&vr[q]) : 0.0, 0.0);
             } else if (act) {
-                const d2v_t w = __builtin_nontemporal_load(reinterpret_cast<const d2v_t*>(&vc[q[u]]));
+                const d2v_t w = __builtin_nontemporal_load(reinterpret_cast<const d2v_t*>(&vc[q]));
                 v[u] = make_double2(w.x, w.y);
             } else {
                 v[u] = make_double2(0.0, 0.0);
             }
         }
+    };
+    int c0[4], c1[4];
+    double2 v0[4], v1[4];
+    bool any0, any1;
+    issue(0, c0, v0, any0);
+    for (uint32_t t = 4; any0; t += 4) {
+        issue(t, c1, v1, any1);  // next group in flight
         double2 xv[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xv[u] = xp[c[u]];
+        for (int u = 0; u < 4; ++u) xv[u] = xp[c0[u]];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (REAL) {
-                ar += v[u].x * xv[u].x;
-                ai += v[u].x * xv[u].y;
+                ar += v0[u].x * xv[u].x;
+                ai += v0[u].x * xv[u].y;
             } else {
-                ar += v[u].x * xv[u].x - v[u].y * xv[u].y;
-                ai += v[u].x * xv[u].y + v[u].y * xv[u].x;
+                ar += v0[u].x * xv[u].x - v0[u].y * xv[u].y;
+                ai += v0[u].x * xv[u].y + v0[u].y * xv[u].x;
             }
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c0[u] = c1[u];
+            v0[u] = v1[u];
+        }
+        any0 = any1;
     }
+    return off;  // entries of the whole run (wave-uniform)
 }
 
 inline PanelView view(const Panels& P, int64_t n) {

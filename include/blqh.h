@@ -95,6 +95,11 @@ int blqh_spmv_device(const blqh_hamil* h, const void* x_dev, void* y_dev); /* de
  * row, coalesced, lanes = 64) with `threads` 256, 512 or 1024, `wgs_per_cu`
  * 1-4 and `rows` 1, 2, 4 or 8.  Switching between kind 3 and kinds 1-2
  * reorders the panel entries on the device (temporary copy of the entries).
+ * kind 4: column tiles of 8192 staged in LDS, one workgroup of 1024 threads
+ * per CU owning a row block (a separate copy of the matrix, built on first
+ * use; `lanes`, `threads`, `wgs_per_cu` and `rows` are set by the layout and
+ * reported by get; BLQH_EINVAL when a row has more than 255 entries in one
+ * tile or the matrix has more than 4.2M rows).
  * get: kind = -1 until
  * the autotuner has run (first SpMV or solve).  set: pins a variant (returns
  * BLQH_EINVAL for an unsupported one); kind = -1 re-arms the autotuner. */

@@ -12,7 +12,8 @@ pytestmark = pytest.mark.gpu
 PLANS = [(0, l, 256, 0, 0) for l in (4, 8, 16, 32, 64)] + \
         [(1, l, t, w, u) for l in (4, 8, 16) for t in (512, 1024) for w in (1, 2) for u in (2, 4)] + \
         [(2, 64, t, w, u) for t in (512, 1024) for w in (1, 2, 4) for u in (1, 2, 4, 8)] + \
-        [(3, 64, t, w, u) for t in (256, 512, 1024) for w in (1, 2, 4) for u in (1, 2, 4, 8)]
+        [(3, 64, t, w, u) for t in (256, 512, 1024) for w in (1, 2, 4) for u in (1, 2, 4, 8)] + \
+        [(4, 64, 1024, 1, 0)]
 
 SECTORS = [
     ("real_4panels", (2, 14, 14, -1, -1, 0, 0), dict(d=1.0, delta_sas=0.05)),
@@ -76,6 +77,9 @@ def test_solver_independent_of_variant(built):
     H.set_spmv(-1)
     e_auto = H.lanczos(lam=300, seed=4, want_psi=False)["E0"]
     assert e_pan == pytest.approx(e_csr, rel=1e-11)
+    H.set_spmv(4, 64, 1024, 1, 0)
+    e_t4 = H.lanczos(lam=300, seed=4, want_psi=False)["E0"]
+    assert e_t4 == pytest.approx(e_csr, rel=1e-11)
     H.set_spmv(3, 64, 512, 2, 2)
     e_jag = H.lanczos(lam=300, seed=4, want_psi=False)["E0"]
     assert e_auto == pytest.approx(e_csr, rel=1e-11)
