@@ -51,6 +51,11 @@ class LanczosResult(C.Structure):
                 ("residual", C.c_double)]
 
 
+class Observables(C.Structure):
+    _fields_ = [("pseudospin_sz", C.c_double), ("pseudospin_sx", C.c_double), ("sz", C.c_double),
+                ("upper_sz", C.c_double), ("lower_sz", C.c_double), ("occupation", (C.c_double * 64) * 2)]
+
+
 class MlpTrainOpts(C.Structure):
     _fields_ = [("steps", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("lr_final", C.c_float),
                 ("seed", C.c_uint32)]
@@ -96,6 +101,12 @@ _SIGS = {
                                  C.POINTER(MlpTrainOpts), C.POINTER(MlpTrainResult)]),
     "blqh_mlp_start_vector": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p]),
     "blqh_mlp_destroy": (None, [C.c_void_p]),
+    "blqh_compute_observables": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Observables)]),
+    "blqh_tridiag_spectrum": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "blqh_spectrum_distinct": (C.c_int, [C.c_int32, C.c_void_p, C.c_int32, C.c_double, C.c_void_p]),
+    "blqh_record_save": (C.c_int, [C.c_char_p, C.c_int64, C.c_int32, C.c_uint32] + [C.c_void_p] * 5),
+    "blqh_record_load": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_uint32)] + [C.c_void_p] * 5),
     "blqh_dense_ground_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]),
     "blqh_start_vector": (C.c_int, [C.c_int64, C.c_uint32, C.c_void_p]),
     "blqh_tridiag_lowest": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
@@ -426,6 +437,54 @@ class Mlp:
     def __del__(self):
         if getattr(self, "_lib", None) is not None:
             self.close()
+
+
+def observables(basis, psi):
+    """Ground-state observables of psi in `basis` (dict; occupation is [2][nphi])."""
+    v = np.ascontiguousarray(psi, np.complex128)
+    o = Observables()
+    _check(lib().blqh_compute_observables(basis.h, _p(v), C.byref(o)))
+    nphi = basis.sector.nphi
+    occ = np.array([[o.occupation[a][j] for j in range(nphi)] for a in range(2)])
+    return {"pseudospin_sz": o.pseudospin_sz, "pseudospin_sx": o.pseudospin_sx, "sz": o.sz,
+            "upper_sz": o.upper_sz, "lower_sz": o.lower_sz, "occupation": occ}
+
+
+def tridiag_spectrum(alpha, beta):
+    """Eigenpairs of the Lanczos tridiagonal: alpha[m], beta[m+1] as returned by
+    lanczos() (beta[0] = 1).  Returns (eigenvalues, psi_n0, psi_0)."""
+    a = np.ascontiguousarray(alpha, np.float64)
+    m = len(a)
+    b = np.ascontiguousarray(beta, np.float64)[1:m]
+    eig, first, vec0 = np.empty(m), np.empty(m), np.empty(m)
+    _check(lib().blqh_tridiag_spectrum(m, _p(a), _p(b) if m > 1 else None, _p(eig), _p(first), _p(vec0)))
+    return eig, first, vec0
+
+
+def spectrum_distinct(eig, kmax=7, gap=1e-5):
+    """Ghost-filtered low-lying levels (eig[0], then jumps > gap), at most kmax."""
+    e = np.ascontiguousarray(eig, np.float64)
+    out = np.empty(max(kmax, 1))
+    n = lib().blqh_spectrum_distinct(len(e), _p(e), kmax, gap, _p(out))
+    _check(0 if n >= 0 else n)
+    return out[:n]
+
+
+def record_save(path, nhilbert, seed, eig, beta, alpha, psi0, psin0):
+    m = len(alpha)
+    arrs = [np.ascontiguousarray(v, np.float64) for v in (eig, beta, alpha, psi0, psin0)]
+    assert len(arrs[1]) == m + 1 and all(len(v) == m for i, v in enumerate(arrs) if i != 1)
+    _check(lib().blqh_record_save(os.fsencode(path), nhilbert, m, seed, *[_p(v) for v in arrs]))
+
+
+def record_load(path, cap=100000):
+    n, m, sd = C.c_int64(), C.c_int32(), C.c_uint32()
+    eig, beta, alpha, psi0, psin0 = (np.empty(cap), np.empty(cap + 1), np.empty(cap), np.empty(cap), np.empty(cap))
+    _check(lib().blqh_record_load(os.fsencode(path), cap, C.byref(n), C.byref(m), C.byref(sd), _p(eig), _p(beta),
+                                  _p(alpha), _p(psi0), _p(psin0)))
+    k = m.value
+    return {"nhilbert": n.value, "lambda": k, "seed": sd.value, "eigenvalues": eig[:k], "norm": beta[:k + 1],
+            "overlap": alpha[:k], "psi_0": psi0[:k], "psi_n0": psin0[:k]}
 
 
 def start_vector(n, seed):

@@ -21,7 +21,7 @@ ARCH = "gfx950"
 # Packed-FP32 instructions are disabled for all device code.
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"),
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
-SOURCES = ["basis.hip", "hamil.hip", "panel.hip", "tiles.hip", "lanczos.hip", "batch.hip", "mlp.hip", "dense.hip", "api.cpp"]
+SOURCES = ["basis.hip", "hamil.hip", "panel.hip", "tiles.hip", "lanczos.hip", "batch.hip", "mlp.hip", "obs.hip", "dense.hip", "records.cpp", "api.cpp"]
 
 
 def _compile(src):

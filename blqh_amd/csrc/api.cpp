@@ -31,6 +31,14 @@ void blqh_mlp_loss_grad_impl(blqh_mlp* M, int64_t ns, const uint64_t* words, con
 void blqh_mlp_train_impl(blqh_mlp* M, int nsets, const blqh_basis* const* bases, const double* params,
                          const double* const* psi, const blqh_mlp_train_opts& o, blqh_mlp_train_result& r);
 void blqh_mlp_start_vector_impl(const blqh_mlp* M, const blqh_basis* B, double d, double dsas, double* out);
+void blqh_tridiag_spectrum_impl(int m, const double* alpha, const double* offdiag, double* eig, double* first,
+                                double* vec0);
+int blqh_spectrum_distinct_impl(int m, const double* eig, int kmax, double gap, double* out);
+void blqh_record_save_impl(const char* path, int64_t nhilbert, int m, uint32_t seed, const double* eig,
+                           const double* beta, const double* alpha, const double* psi0, const double* psin0);
+void blqh_record_load_impl(const char* path, int cap, int64_t* nhilbert, int* m, uint32_t* seed, double* eig,
+                           double* beta, double* alpha, double* psi0, double* psin0);
+void blqh_observables_impl(const blqh_basis* B, const double* psi_host, blqh_observables& out);
 void blqh_dense_ground_state_impl(const blqh_hamil* H, double* e0, double* e_printed, double* vec);
 void blqh_device_copy_impl(const void* src, void* dst, int64_t bytes);
 namespace blqh {
@@ -314,6 +322,34 @@ int blqh_mlp_start_vector(const blqh_mlp* m, const blqh_basis* b, double d, doub
 
 void blqh_mlp_destroy(blqh_mlp* m) {
     if (m) blqh_mlp_destroy_impl(m);
+}
+
+int blqh_compute_observables(const blqh_basis* b, const double* psi, blqh_observables* out) {
+    if (!b || !psi || !out) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] { blqh_observables_impl(b, psi, *out); });
+}
+
+int blqh_tridiag_spectrum(int32_t m, const double* alpha, const double* offdiag, double* eig, double* first,
+                          double* vec0) {
+    if (m < 1 || !alpha || (m > 1 && !offdiag)) return fail(BLQH_EINVAL, "bad argument");
+    return guarded([&] { blqh_tridiag_spectrum_impl(m, alpha, offdiag, eig, first, vec0); });
+}
+
+int blqh_spectrum_distinct(int32_t m, const double* eig, int32_t kmax, double gap, double* out) {
+    if (m < 0 || kmax < 0 || (m > 0 && (!eig || !out))) return fail(BLQH_EINVAL, "bad argument");
+    return blqh_spectrum_distinct_impl(m, eig, kmax, gap, out);
+}
+
+int blqh_record_save(const char* path, int64_t nhilbert, int32_t m, uint32_t seed, const double* eig,
+                     const double* beta, const double* alpha, const double* psi0, const double* psin0) {
+    if (!path || m < 1 || !eig || !beta || !alpha || !psi0 || !psin0) return fail(BLQH_EINVAL, "bad argument");
+    return guarded([&] { blqh_record_save_impl(path, nhilbert, m, seed, eig, beta, alpha, psi0, psin0); });
+}
+
+int blqh_record_load(const char* path, int32_t cap, int64_t* nhilbert, int32_t* m, uint32_t* seed, double* eig,
+                     double* beta, double* alpha, double* psi0, double* psin0) {
+    if (!path || !nhilbert || !m || !seed || cap < 1) return fail(BLQH_EINVAL, "bad argument");
+    return guarded([&] { blqh_record_load_impl(path, cap, nhilbert, m, seed, eig, beta, alpha, psi0, psin0); });
 }
 
 int blqh_dense_ground_state(const blqh_hamil* h, double* e0, double* e_printed, double* vec) {

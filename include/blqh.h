@@ -223,6 +223,40 @@ int blqh_start_vector(int64_t n, uint32_t seed, double* out);
  * the reference's diag() pass 2 (lanczos_hamiltonian.cpp:581-606). */
 int blqh_tridiag_lowest(int32_t m, const double* alpha, const double* offdiag, double* e0, double* vec);
 
+/* ---- Ground-state observables: lhamil::occupatation_number, pseudospin_Sz,
+ *      pseudospin_Sx (BLQH_without_spin/lanczos_hamiltonian.cpp:658-735) and
+ *      the spinful Sz, upper_Sz, down_Sz (BLQH_with_spin/lanczos_hamiltonian.cpp:1217-1500),
+ *      of a state psi (2*nrows doubles, e.g. from blqh_lanczos) in basis b. ---- */
+typedef struct blqh_observables {
+    double pseudospin_sz;       /* (2 <N_upper> - Ne) / (2 Ne) */
+    double pseudospin_sx;       /* |<psi|T|psi>| / Ne, T the interlayer tunnelling with 1/2 per pair */
+    double sz;                  /* spinful: (2 <N_up> - Ne) / (2 Ne); spinless: 0 */
+    double upper_sz, lower_sz;  /* spinful upper_Sz / down_Sz; spinless: 0 */
+    double occupation[2][64];   /* [a][j]: orbital j of layer a (spinful: species 0 / 1) */
+} blqh_observables;
+int blqh_compute_observables(const blqh_basis* b, const double* psi, blqh_observables* out);
+
+/* ---- Spectra and the Lanczos record ----
+ * All eigenpairs of the Lanczos tridiagonal (alpha[m], off-diagonal beta_1..beta_{m-1}):
+ * the reference's diag() (BLQH_without_spin/lanczos_hamiltonian.cpp:581-606).
+ * eig[m] ascending; first[m] = first component of every eigenvector (psi_n0);
+ * vec0[m] = eigenvector of eig[0] (psi_0, first component >= 0); any may be NULL. */
+int blqh_tridiag_spectrum(int32_t m, const double* alpha, const double* offdiag, double* eig, double* first,
+                          double* vec0);
+/* Low-lying distinct levels: eig[0], then every eig[i] with eig[i] - eig[i-1] > gap,
+ * at most kmax (the ghost filter of BLQH_without_spin/main.cpp:100-110, gap 1e-5).
+ * Returns the count (>= 0) or an error code. */
+int blqh_spectrum_distinct(int32_t m, const double* eig, int32_t kmax, double gap, double* out);
+/* The record of lhamil::save_to_file / read_from_file
+ * (BLQH_without_spin/lanczos_hamiltonian.cpp:738-819): nHilbert, lambda (= m),
+ * seed, eigenvalues[m], norm = beta[m+1] (beta[0] = 1), overlap = alpha[m],
+ * psi_0[m], psi_n0[m], in the reference's text format.  load: arrays of
+ * capacity cap (beta cap+1); BLQH_ERANGE when the record is longer. */
+int blqh_record_save(const char* path, int64_t nhilbert, int32_t m, uint32_t seed, const double* eig,
+                     const double* beta, const double* alpha, const double* psi0, const double* psin0);
+int blqh_record_load(const char* path, int32_t cap, int64_t* nhilbert, int32_t* m, uint32_t* seed, double* eig,
+                     double* beta, double* alpha, double* psi0, double* psin0);
+
 /* Kernel timing (engine instrumentation; no counterpart in the reference).
  * When enabled, every fused Lanczos SpMV launch (the solver's dominant kernel)
  * is bracketed by HIP events on the library stream; blqh_profile_read returns

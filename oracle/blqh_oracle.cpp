@@ -528,6 +528,97 @@ void orc_basis_export(void* h, uint64_t* id, int16_t* period) {
     if (period) std::memcpy(period, B->period.data(), B->period.size() * sizeof(int16_t));
 }
 int64_t orc_basis_find(void* h, uint64_t c) { return ((Basis*)h)->find(c); }
+
+// Ground-state observables of a state psi (interleaved re/im) in basis h,
+// loop for loop as lhamil::occupatation_number, pseudospin_Sz, pseudospin_Sx
+// (BLQH_without_spin/lanczos_hamiltonian.cpp:658-735) and the spinful
+// occupatation_number, pseudospin_Sz, Sz, upper_Sz, down_Sz, pseudospin_Sx
+// (BLQH_with_spin/lanczos_hamiltonian.cpp:1217-1500).
+// out: [0] pseudospin Sz, [1] pseudospin Sx, [2] Sz, [3] upper Sz, [4] lower Sz,
+// [5 + a*nphi + j] occupation number of orbital j in layer a.
+void orc_observables(void* h, const double* psi_in, double* out) {
+    const Basis& B = *(const Basis*)h;
+    const orc_sector& s = B.s;
+    const int nphi = s.nphi, K = s.K;
+    const long n = (long)B.id.size();
+    const cplx* psi = (const cplx*)psi_in;
+    std::vector<cplx> FT((size_t)nphi * nphi);
+    for (int a = 0; a < nphi; ++a)
+        for (int b = 0; b < nphi; ++b)
+            FT[a * nphi + b] = cplx(std::cos(2 * M_PI * (a - b) * K / nphi), std::sin(2 * M_PI * (a - b) * K / nphi));
+    const uint64_t low = (1ULL << nphi) - 1;
+    auto nel_of = [&](uint64_t c, int sp) { return pc((c >> (sp * nphi)) & low); };
+    // occupation numbers
+    for (int a = 0; a < 2; ++a)
+        for (int j = 0; j < nphi; ++j) {
+            const uint64_t mask = a == 0 ? (1ULL << j) : (1ULL << (j + nphi));
+            cplx occ = 0;
+            for (long i = 0; i < n; ++i) {
+                const int D = K < 0 ? 1 : B.period[i];
+                for (int q = 0; q < D; ++q) {
+                    int sign = 1;
+                    const uint64_t lb = q == 0 ? B.id[i] : translate(s, B.id[i], q, sign);
+                    if ((mask & lb) == mask) occ += std::conj(psi[i]) * psi[i] * FT[q * nphi + q] / (1.0 * D);
+                }
+            }
+            out[5 + a * nphi + j] = occ.real();
+        }
+    // pseudospin Sz, Sz, upper / lower Sz
+    double up = 0, su = 0, us = 0, ls = 0;
+    for (long i = 0; i < n; ++i) {
+        const double w = std::norm(psi[i]);
+        if (s.S == 2) {
+            up += nel_of(B.id[i], 0) * w;
+        } else {
+            const double n0 = nel_of(B.id[i], 0), n1 = nel_of(B.id[i], 1), n2 = nel_of(B.id[i], 2),
+                         n3 = nel_of(B.id[i], 3);
+            up += (n0 + n1) * w;
+            su += (n0 + n2) * w;
+            us += (2.0 * n0 - (n0 + n1)) / (2.0 * (n0 + n1) + 1e-8) * w;
+            ls += (2.0 * n2 - (n2 + n3)) / (2.0 * (n2 + n3) + 1e-8) * w;
+        }
+    }
+    out[0] = (2.0 * up - s.nel) / (2.0 * s.nel);
+    out[2] = s.S == 4 ? (2.0 * su - s.nel) / (2.0 * s.nel) : 0.0;
+    out[3] = s.S == 4 ? us : 0.0;
+    out[4] = s.S == 4 ? ls : 0.0;
+    // pseudospin Sx: tunnelling pairs, coefficient 1/2 (spinless) or 1/4 per loop (spinful)
+    struct Loop { int nbeg, nend, off; double coef; };
+    std::vector<Loop> loops;
+    if (s.S == 2) loops.push_back({0, nphi, nphi, 0.5});
+    else {
+        loops = {{0, nphi, 2 * nphi, 0.25}, {nphi, 2 * nphi, 2 * nphi, 0.25},
+                 {2 * nphi, 3 * nphi, -2 * nphi, 0.25}, {3 * nphi, 4 * nphi, -2 * nphi, 0.25},
+                 {0, nphi, 3 * nphi, 0.25}, {3 * nphi, 4 * nphi, -3 * nphi, 0.25},
+                 {nphi, 2 * nphi, nphi, 0.25}, {2 * nphi, 3 * nphi, -nphi, 0.25}};
+    }
+    cplx sx = 0;
+    for (long i = 0; i < n; ++i) {
+        const int Dl = K < 0 ? 1 : B.period[i];
+        for (int ql = 0; ql < Dl; ++ql) {
+            int signl = 1;
+            const uint64_t lb = ql == 0 ? B.id[i] : translate(s, B.id[i], ql, signl);
+            for (const Loop& lp : loops)
+                for (int m = lp.nbeg; m < lp.nend; ++m) {
+                    const int nt = m + lp.off;
+                    const uint64_t mask = (1ULL << m) + (1ULL << nt);
+                    const uint64_t Kn = mask & lb;
+                    if (Kn == mask || Kn == 0) continue;
+                    const uint64_t r0 = lb - Kn + (Kn ^ mask);
+                    const int Dr = K < 0 ? 1 : period_of(s, r0);
+                    for (int qr = 0; qr < Dr; ++qr) {
+                        int signr = 1;
+                        const uint64_t rb = qr == 0 ? r0 : inv_translate(s, r0, qr, signr);
+                        const long j = B.find(rb);
+                        if (j < 0) continue;
+                        const int sign = (nt > m ? sign1(lb, m, nt) : sign1(lb, nt, m)) * signl * signr;
+                        sx += FT[ql * nphi + qr] * std::conj(psi[i]) * psi[j] * (sign * lp.coef / std::sqrt(Dl * Dr));
+                    }
+                }
+        }
+    }
+    out[1] = std::abs(sx) / s.nel;
+}
 void orc_basis_free(void* h) { delete (Basis*)h; }
 
 // Number of J-sector configurations before orbit reduction (the reference's

@@ -81,6 +81,8 @@ def lib():
         L.orc_spmv.argtypes = [vp, vp, vp]
         L.orc_start_vector.argtypes = [i64, C.c_uint, vp]
         L.orc_tridiag_eigh.argtypes = [C.c_int, vp, vp, vp, vp]
+        L.orc_observables.restype = None
+        L.orc_observables.argtypes = [vp, vp, vp]
         L.orc_lanczos.restype = C.c_double
         L.orc_lanczos.argtypes = [vp, C.c_int, C.c_uint, vp, vp, vp, vp]
         _lib = L
@@ -275,3 +277,13 @@ def ref_basis(sec):
         if n >= 0:
             return ids[:n].copy(), per[:n].copy()
         cap = -n
+
+
+def observables(basis, psi):
+    """The reference's observables of psi in `basis` (dict)."""
+    nphi = basis.sec.nphi
+    psi = np.ascontiguousarray(psi, np.complex128)
+    out = np.empty(5 + 2 * nphi)
+    lib().orc_observables(basis.h, _p(psi), _p(out))
+    return {"pseudospin_sz": out[0], "pseudospin_sx": out[1], "sz": out[2], "upper_sz": out[3], "lower_sz": out[4],
+            "occupation": out[5:].reshape(2, nphi)}
