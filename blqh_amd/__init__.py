@@ -42,7 +42,8 @@ class Params(C.Structure):
 
 class LanczosOpts(C.Structure):
     _fields_ = [("lam", C.c_int32), ("seed", C.c_uint32), ("tol", C.c_double), ("check_every", C.c_int32),
-                ("store_basis", C.c_int32), ("start", C.c_void_p), ("res_tol", C.c_double)]
+                ("store_basis", C.c_int32), ("start", C.c_void_p), ("res_tol", C.c_double),
+                ("skip_states", C.c_int32)]
 
 
 class LanczosResult(C.Structure):
@@ -53,7 +54,8 @@ class LanczosResult(C.Structure):
 
 class Observables(C.Structure):
     _fields_ = [("pseudospin_sz", C.c_double), ("pseudospin_sx", C.c_double), ("sz", C.c_double),
-                ("upper_sz", C.c_double), ("lower_sz", C.c_double), ("occupation", (C.c_double * 64) * 2)]
+                ("upper_sz", C.c_double), ("lower_sz", C.c_double), ("occupation", (C.c_double * 64) * 2),
+                ("spinflip", C.c_double)]
 
 
 class MlpTrainOpts(C.Structure):
@@ -103,6 +105,9 @@ _SIGS = {
     "blqh_mlp_destroy": (None, [C.c_void_p]),
     "blqh_compute_observables": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Observables)]),
     "blqh_tridiag_spectrum": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "blqh_tridiag_vectors": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
+    "blqh_lanczos_reconstruct": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "blqh_spectrum_distinct": (C.c_int, [C.c_int32, C.c_void_p, C.c_int32, C.c_double, C.c_void_p]),
     "blqh_record_save": (C.c_int, [C.c_char_p, C.c_int64, C.c_int32, C.c_uint32] + [C.c_void_p] * 5),
     "blqh_record_load": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
@@ -274,10 +279,11 @@ class Hamiltonian:
         return y
 
     def lanczos(self, lam=200, seed=0, tol=1e-10, res_tol=1e-7, check_every=1, store_basis=-1, start=None,
-                want_psi=True):
+                want_psi=True, skip_states=False):
         """Ground state.  Stops when the lowest Ritz value moves by < tol and the
-        Ritz residual is < res_tol (tol <= 0: exactly lam steps)."""
-        opts = LanczosOpts(lam, seed, tol, check_every, store_basis, None, res_tol)
+        Ritz residual is < res_tol (tol <= 0: exactly lam steps).  skip_states:
+        passes 1 and 2 only (alpha, beta, E0; no psi, E_rq is NaN)."""
+        opts = LanczosOpts(lam, seed, tol, check_every, store_basis, None, res_tol, 1 if skip_states else 0)
         keep = None
         if start is not None:
             keep = np.ascontiguousarray(start, np.complex128)
@@ -290,6 +296,54 @@ class Hamiltonian:
         out = {k: getattr(res, k) for k, _ in LanczosResult._fields_}
         out.update(psi=psi, alpha=a[:res.iterations], beta=b[:res.iterations + 1])
         return out
+
+    def reconstruct(self, coefs, alpha=None, beta=None, seed=0, start=None):
+        """Pass 3 (eigenstates_reconstruction): regenerate the Lanczos vectors of
+        len(coefs[0]) steps from `seed` (or `start`) and return the normalised
+        states psi_s = sum_l coefs[s][l] conj(v_l), their Rayleigh quotients and
+        the largest relative deviation of the regenerated alpha / beta from the
+        given ones (beta[0] = 1 as returned by lanczos())."""
+        c = np.ascontiguousarray(np.atleast_2d(coefs), np.float64)
+        ns, m = c.shape
+        a = None if alpha is None else np.ascontiguousarray(alpha, np.float64)
+        b = None if beta is None else np.ascontiguousarray(beta, np.float64)
+        assert a is None or len(a) >= m - 1
+        assert b is None or len(b) >= m
+        keep = None if start is None else np.ascontiguousarray(start, np.complex128)
+        assert keep is None or len(keep) == self.n
+        psi = np.empty((ns, self.n), np.complex128)
+        erq = np.empty(ns)
+        dev = C.c_double()
+        _check(lib().blqh_lanczos_reconstruct(self.h, m, seed, _p(keep) if keep is not None else None,
+                                              _p(a) if a is not None else None, _p(b) if b is not None else None,
+                                              ns, _p(c), _p(psi), _p(erq), C.byref(dev)))
+        return psi, erq, dev.value
+
+    def eigenstates(self, lam=200, seed=0, tol=1e-10, res_tol=1e-7, check_every=1, start=None, e1_gap=0.0):
+        """The spinful reference's three passes with the first excited state
+        (coeff_explicit_update, diag, eigenstates_reconstruction,
+        ground_state_energy, first_excited_state_energy): psi_0 and psi_1 are
+        rebuilt in one replay.  The excited Krylov vector is eigenvector 1 of
+        the tridiagonal, as the reference's diag() selects in effect (its
+        1e-2 gap test reads the eigenvalues before they are filled); e1_gap > 0
+        picks the first eigenvalue more than e1_gap above E0 instead."""
+        r = self.lanczos(lam, seed, tol, res_tol, check_every, 0, start, want_psi=False, skip_states=True)
+        a, b = r["alpha"], r["beta"]
+        m = len(a)
+        if m < 2:
+            raise BlqhError(-1, "the first excited state needs at least 2 Lanczos vectors")
+        eig, _, _ = tridiag_spectrum(a, b)
+        k1 = 1
+        if e1_gap > 0:
+            big = np.nonzero(np.abs(eig - eig[0]) > e1_gap)[0]
+            if len(big) == 0:
+                raise BlqhError(-1, "no eigenvalue above E0 + e1_gap in the Krylov space")
+            k1 = int(big[0])
+        v = tridiag_vectors(a, b, 0, k1 + 1)[[0, k1]]
+        psi, erq, dev = self.reconstruct(v, a, b, seed=seed, start=start)
+        r.update(psi=psi[0], psi1=psi[1], E_rq=erq[0], E1_rq=erq[1], E1=eig[k1], e1_index=k1, replay_dev=dev,
+                 eigenvalues=eig)
+        return r
 
     def dense_ground_state(self):
         e0 = C.c_double()
@@ -447,7 +501,7 @@ def observables(basis, psi):
     nphi = basis.sector.nphi
     occ = np.array([[o.occupation[a][j] for j in range(nphi)] for a in range(2)])
     return {"pseudospin_sz": o.pseudospin_sz, "pseudospin_sx": o.pseudospin_sx, "sz": o.sz,
-            "upper_sz": o.upper_sz, "lower_sz": o.lower_sz, "occupation": occ}
+            "upper_sz": o.upper_sz, "lower_sz": o.lower_sz, "occupation": occ, "spinflip": o.spinflip}
 
 
 def tridiag_spectrum(alpha, beta):
@@ -459,6 +513,17 @@ def tridiag_spectrum(alpha, beta):
     eig, first, vec0 = np.empty(m), np.empty(m), np.empty(m)
     _check(lib().blqh_tridiag_spectrum(m, _p(a), _p(b) if m > 1 else None, _p(eig), _p(first), _p(vec0)))
     return eig, first, vec0
+
+
+def tridiag_vectors(alpha, beta, k0, count):
+    """Eigenvectors k0..k0+count-1 of the Lanczos tridiagonal (alpha[m], beta[m+1],
+    beta[0] = 1), rows of the returned (count, m) array, first components >= 0."""
+    a = np.ascontiguousarray(alpha, np.float64)
+    m = len(a)
+    b = np.ascontiguousarray(beta, np.float64)[1:m]
+    out = np.empty((count, m))
+    _check(lib().blqh_tridiag_vectors(m, _p(a), _p(b) if m > 1 else None, k0, count, _p(out)))
+    return out
 
 
 def spectrum_distinct(eig, kmax=7, gap=1e-5):

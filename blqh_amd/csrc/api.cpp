@@ -19,6 +19,10 @@ void blqh_batch_destroy_impl(blqh_batch* b);
 void blqh_batch_rebind_impl(blqh_batch* b, blqh_hamil* const* hs, int count);
 int blqh_batch_lanes_impl(const blqh_batch* b);
 void blqh_batch_set_lanes_impl(blqh_batch* b, int lanes);
+void blqh_lanczos_reconstruct_impl(blqh_hamil* H, int m, uint32_t seed, const double* start, const double* alpha,
+                                   const double* beta, int ns, const double* coefs, double* psi, double* erq,
+                                   double* dev);
+void blqh_tridiag_vectors_impl(int m, const double* alpha, const double* offdiag, int k0, int count, double* vecs);
 void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const uint32_t* seeds,
                              const double* const* starts, blqh_lanczos_result* res, double* const* psi_out);
 blqh_mlp* blqh_mlp_create_impl(int nbits, uint32_t seed);
@@ -244,6 +248,16 @@ int blqh_lanczos(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_r
     });
 }
 
+int blqh_lanczos_reconstruct(const blqh_hamil* h, int32_t m, uint32_t seed, const double* start,
+                             const double* alpha, const double* beta, int32_t ns, const double* coefs, double* psi,
+                             double* erq, double* max_dev) {
+    if (!h || !coefs) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] {
+        blqh_lanczos_reconstruct_impl(const_cast<blqh_hamil*>(h), m, seed, start, alpha, beta, ns, coefs, psi, erq,
+                                      max_dev);
+    });
+}
+
 int blqh_batch_create(blqh_hamil* const* hs, int32_t count, int32_t lanes, blqh_batch** out) {
     if (!hs || !out || count < 1) return fail(BLQH_EINVAL, "bad argument");
     *out = nullptr;
@@ -333,6 +347,12 @@ int blqh_tridiag_spectrum(int32_t m, const double* alpha, const double* offdiag,
                           double* vec0) {
     if (m < 1 || !alpha || (m > 1 && !offdiag)) return fail(BLQH_EINVAL, "bad argument");
     return guarded([&] { blqh_tridiag_spectrum_impl(m, alpha, offdiag, eig, first, vec0); });
+}
+
+int blqh_tridiag_vectors(int32_t m, const double* alpha, const double* offdiag, int32_t k0, int32_t count,
+                         double* vecs) {
+    if (m < 1 || !alpha || !vecs || (m > 1 && !offdiag)) return fail(BLQH_EINVAL, "bad argument");
+    return guarded([&] { blqh_tridiag_vectors_impl(m, alpha, offdiag, k0, count, vecs); });
 }
 
 int blqh_spectrum_distinct(int32_t m, const double* eig, int32_t kmax, double gap, double* out) {

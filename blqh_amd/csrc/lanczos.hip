@@ -141,24 +141,26 @@ __global__ void __launch_bounds__(kThreads) k_lz1(int64_t n, const int64_t* __re
     grid_reduce(acc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
 }
 
-// K2: Wn = u - alpha[j] invb[j] W; beta[j+1] = |Wn|; optional psi += coef[j] invb[j] conj(W)
+// K2: Wn = u - alpha[j] invb[j] W; beta[j+1] = |Wn|; for s < ns:
+// psi_s += coef[s*ldc + j] invb[j] conj(W)   (psi_s = psi + s*n)
 __global__ void __launch_bounds__(kThreads) k_lz2(int64_t n, const double2* __restrict__ u,
                                                    const double2* __restrict__ W, double2* __restrict__ Wn, Scal sc,
-                                                   int j, double2* __restrict__ psi, const double* __restrict__ coef,
-                                                   double* partials, unsigned* ticket) {
+                                                   int j, int ns, double2* __restrict__ psi,
+                                                   const double* __restrict__ coef, int ldc, double* partials,
+                                                   unsigned* ticket) {
     const double a = sc.alpha[j] * sc.invb[j];
-    const double cj = psi ? coef[j] * sc.invb[j] : 0.0;
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
         const double2 w = W[i], x = u[i];
         const double2 r = make_double2(x.x - a * w.x, x.y - a * w.y);
         Wn[i] = r;
         acc += r.x * r.x + r.y * r.y;
-        if (psi) {
-            double2 p = psi[i];
+        for (int s = 0; s < ns; ++s) {
+            const double cj = coef[(int64_t)s * ldc + j] * sc.invb[j];
+            double2 p = psi[(int64_t)s * n + i];
             p.x += cj * w.x;
             p.y -= cj * w.y;
-            psi[i] = p;
+            psi[(int64_t)s * n + i] = p;
         }
     }
     grid_reduce(acc, partials, ticket, [&](double s) {
@@ -1082,6 +1084,22 @@ void blqh_spmv_device_impl(blqh_hamil* H, const void* x, void* y) {
     BLQH_HIP(hipGetLastError());
 }
 
+namespace {
+// Start vector of pass 1 and pass 3: the caller's (normalised here) or the seeded mt19937 one.
+void make_start(int64_t n, uint32_t seed, const double* start, std::vector<double2>& v0) {
+    if (start) {
+        v0.assign((const double2*)start, (const double2*)start + n);
+        double s = 0;
+        for (auto& z : v0) s += z.x * z.x + z.y * z.y;
+        BLQH_REQUIRE(s > 0 && std::isfinite(s), BLQH_EINVAL, "start vector is zero or not finite");
+        s = std::sqrt(s);
+        for (auto& z : v0) { z.x /= s; z.y /= s; }
+    } else {
+        blqh::start_vector(n, seed, v0);
+    }
+}
+}  // namespace
+
 void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_result& res, double* psi_out,
                        double* alpha_out, double* beta_out) {
     using namespace blqh;
@@ -1098,7 +1116,8 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
     size_t free_b = 0, total_b = 0;
     BLQH_HIP(hipMemGetInfo(&free_b, &total_b));
     const size_t need = (size_t)(cap + 1) * n * sizeof(double2);
-    bool store = o.store_basis > 0 || (o.store_basis < 0 && need < free_b / 2);
+    const bool skip = o.skip_states != 0;
+    bool store = !skip && (o.store_basis > 0 || (o.store_basis < 0 && need < free_b / 2));
     const int nvec = store ? cap + 1 : 3;
     DevBuf<double2> Wb((size_t)nvec * n), u(n), psi(n);
     DevBuf<double> sc_buf(3 * (size_t)cap + 8), coef(cap + 1), red(4);
@@ -1111,18 +1130,8 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
     BLQH_HIP(hipMemcpyAsync(sc.beta, &one, 8, hipMemcpyHostToDevice, st));
     BLQH_HIP(hipMemcpyAsync(sc.invb, &one, 8, hipMemcpyHostToDevice, st));
 
-    // start vector
     std::vector<double2> v0;
-    if (o.start) {
-        v0.assign((const double2*)o.start, (const double2*)o.start + n);
-        double s = 0;
-        for (auto& z : v0) s += z.x * z.x + z.y * z.y;
-        BLQH_REQUIRE(s > 0 && std::isfinite(s), BLQH_EINVAL, "start vector is zero or not finite");
-        s = std::sqrt(s);
-        for (auto& z : v0) { z.x /= s; z.y /= s; }
-    } else {
-        start_vector(n, o.seed, v0);
-    }
+    make_start(n, o.seed, o.start, v0);
     auto slot = [&](int j) -> double2* { return Wb.get() + (size_t)(store ? j : j % 3) * n; };
     BLQH_HIP(hipMemcpyAsync(slot(0), v0.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
 
@@ -1142,8 +1151,8 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
         }
         launch_lz1(H, plan, W, Wp, u.get(), sc, j, partials.get(), ticket.get(), st);
         if (prof.on) BLQH_HIP(hipEventRecord(prof.ev[2 * ntimed++ + 1], st));
-        hipLaunchKernelGGL(k_lz2, dim3(gv), dim3(kThreads), 0, st, n, u.get(), W, slot(j + 1), sc, j, psi_acc, cf,
-                           partials.get(), ticket.get());
+        hipLaunchKernelGGL(k_lz2, dim3(gv), dim3(kThreads), 0, st, n, u.get(), W, slot(j + 1), sc, j,
+                           psi_acc ? 1 : 0, psi_acc, cf, cap, partials.get(), ticket.get());
     };
 
     // pass 1
@@ -1187,6 +1196,29 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
     // pass 2: lowest eigenpair of T_m
     std::vector<double> c(m);
     const double E0 = tridiag_lowest(m, ha.data(), hb.data() + 1, c.data());
+    if (alpha_out) { std::fill(alpha_out, alpha_out + cap, 0.0); std::copy(ha.begin(), ha.begin() + m, alpha_out); }
+    if (beta_out) { std::fill(beta_out, beta_out + cap + 1, 0.0); std::copy(hb.begin(), hb.begin() + m + 1, beta_out); }
+    res.E0 = E0;
+    res.dE = dE;
+    res.residual = std::isfinite(resid) ? resid : hb[m] * std::fabs(c[m - 1]);
+    res.iterations = m;
+    res.converged = conv ? 1 : 0;
+    if (skip) {
+        res.E_rq = NAN;
+        res.passes = 1;
+        BLQH_HIP(hipStreamSynchronize(st));
+        if (prof.on) {
+            for (int k = 0; k < ntimed; ++k) {
+                float ms = 0;
+                BLQH_HIP(hipEventElapsedTime(&ms, prof.ev[2 * k], prof.ev[2 * k + 1]));
+                prof.ms += ms;
+            }
+            prof.launches += ntimed;
+            prof.bytes += (double)ntimed * (20.0 * H->nnz + 56.0 * n + 8.0);
+        }
+        res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return;
+    }
     // pass 3
     std::vector<double> invb(m);
     {
@@ -1237,13 +1269,87 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
         prof.launches += ntimed;
         prof.bytes += (double)ntimed * (20.0 * H->nnz + 56.0 * n + 8.0);
     }
-    if (alpha_out) { std::fill(alpha_out, alpha_out + cap, 0.0); std::copy(ha.begin(), ha.begin() + m, alpha_out); }
-    if (beta_out) { std::fill(beta_out, beta_out + cap + 1, 0.0); std::copy(hb.begin(), hb.begin() + m + 1, beta_out); }
-    res.E0 = E0;
     res.E_rq = erq;
-    res.dE = dE;
-    res.residual = std::isfinite(resid) ? resid : hb[m] * std::fabs(c[m - 1]);
-    res.iterations = m;
-    res.converged = conv ? 1 : 0;
     res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void blqh_lanczos_reconstruct_impl(blqh_hamil* H, int m, uint32_t seed, const double* start, const double* alpha,
+                                   const double* beta, int ns, const double* coefs, double* psi_out, double* erq_out,
+                                   double* dev_out) {
+    using namespace blqh;
+    hipStream_t st = stream();
+    const int64_t n = H->n;
+    BLQH_REQUIRE(m >= 1 && m <= 100000, BLQH_EINVAL, "lambda out of range");
+    BLQH_REQUIRE(ns >= 1 && ns <= 16, BLQH_EINVAL, "between 1 and 16 states per replay");
+    BLQH_REQUIRE(coefs != nullptr, BLQH_EINVAL, "coefficients missing");
+    for (int s = 0; s < ns; ++s) {
+        bool any = false;
+        for (int l = 0; l < m; ++l) {
+            const double c = coefs[(size_t)s * m + l];
+            BLQH_REQUIRE(std::isfinite(c), BLQH_EINVAL, "coefficients not finite");
+            any |= c != 0.0;
+        }
+        BLQH_REQUIRE(any, BLQH_EINVAL, "a coefficient vector is zero");
+    }
+    autotune(H);
+    const Plan plan = plan_of(H);
+    DevBuf<double2> Wb(3 * (size_t)n), u(n), psi((size_t)ns * n);
+    DevBuf<double> sc_buf(3 * (size_t)m + 8), coef((size_t)ns * m), red(2 * ns + 2);
+    const int gmax = std::max(std::max(lz1_grid(H, plan), spmv_grid(n, 4)), vec_grid(n));
+    DevBuf<double> partials(gmax);
+    DevBuf<unsigned> ticket(1);
+    BLQH_HIP(hipMemsetAsync(ticket.get(), 0, sizeof(unsigned), st));
+    Scal sc{sc_buf.get(), sc_buf.get() + m, sc_buf.get() + 2 * m + 2};
+    const double one = 1.0;
+    BLQH_HIP(hipMemcpyAsync(sc.beta, &one, 8, hipMemcpyHostToDevice, st));
+    BLQH_HIP(hipMemcpyAsync(sc.invb, &one, 8, hipMemcpyHostToDevice, st));
+    std::vector<double2> v0;
+    make_start(n, seed, start, v0);
+    auto slot = [&](int j) -> double2* { return Wb.get() + (size_t)(j % 3) * n; };
+    BLQH_HIP(hipMemcpyAsync(slot(0), v0.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
+    BLQH_HIP(hipMemcpyAsync(coef.get(), coefs, (size_t)ns * m * 8, hipMemcpyHostToDevice, st));
+    BLQH_HIP(hipMemsetAsync(psi.get(), 0, (size_t)ns * n * sizeof(double2), st));
+    const int gv = vec_grid(n);
+    for (int j = 0; j + 1 < m; ++j) {
+        double2* W = slot(j);
+        double2* Wp = j > 0 ? slot(j - 1) : slot(j);
+        launch_lz1(H, plan, W, Wp, u.get(), sc, j, partials.get(), ticket.get(), st);
+        hipLaunchKernelGGL(k_lz2, dim3(gv), dim3(kThreads), 0, st, n, u.get(), W, slot(j + 1), sc, j, ns, psi.get(),
+                           coef.get(), m, partials.get(), ticket.get());
+    }
+    BLQH_HIP(hipGetLastError());
+    std::vector<double> ra(m), rb(m + 1);
+    BLQH_HIP(hipMemcpyAsync(ra.data(), sc.alpha, m * 8, hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipMemcpyAsync(rb.data(), sc.invb, m * 8, hipMemcpyDeviceToHost, st));
+    std::vector<double> rbeta(m);
+    BLQH_HIP(hipMemcpyAsync(rbeta.data(), sc.beta, m * 8, hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    for (int s = 0; s < ns; ++s)  // the last vector v_{m-1} = invb[m-1] W_{m-1}
+        hipLaunchKernelGGL(k_accum, dim3(gv), dim3(kThreads), 0, st, n, slot(m - 1),
+                           coefs[(size_t)s * m + m - 1] * rb[m - 1], psi.get() + (size_t)s * n);
+    double dev = 0.0;
+    for (int j = 0; j + 1 < m; ++j) {
+        if (alpha) dev = std::max(dev, std::fabs(ra[j] - alpha[j]) / std::max(1.0, std::fabs(alpha[j])));
+        if (beta) dev = std::max(dev, std::fabs(rbeta[j + 1] - beta[j + 1]) / std::max(1.0, std::fabs(beta[j + 1])));
+    }
+    BLQH_REQUIRE(std::isfinite(dev), BLQH_EINTERNAL, "replay produced non-finite coefficients");
+    for (int s = 0; s < ns; ++s) {
+        double2* p = psi.get() + (size_t)s * n;
+        hipLaunchKernelGGL(k_norm2, dim3(gv), dim3(kThreads), 0, st, n, p, red.get() + 2 * s, partials.get(),
+                           ticket.get());
+        hipLaunchKernelGGL(k_scale, dim3(gv), dim3(kThreads), 0, st, n, p, red.get() + 2 * s);
+        launch_spmv_plan(H, plan, p, u.get(), st);
+        hipLaunchKernelGGL(k_redot, dim3(gv), dim3(kThreads), 0, st, n, p, u.get(), red.get() + 2 * s + 1,
+                           partials.get(), ticket.get());
+    }
+    BLQH_HIP(hipGetLastError());
+    std::vector<double> hred(2 * ns);
+    BLQH_HIP(hipMemcpyAsync(hred.data(), red.get(), 2 * ns * 8, hipMemcpyDeviceToHost, st));
+    if (psi_out) BLQH_HIP(hipMemcpyAsync(psi_out, psi.get(), (size_t)ns * n * sizeof(double2), hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    for (int s = 0; s < ns; ++s) {
+        BLQH_REQUIRE(hred[2 * s] > 0, BLQH_EINVAL, "a reconstructed state is zero (coefficients all zero?)");
+        if (erq_out) erq_out[s] = hred[2 * s + 1];
+    }
+    if (dev_out) *dev_out = dev;
 }

@@ -11,9 +11,11 @@
 //   Sz (spinful)     = (2 <N_spin-up> - Ne) / (2 Ne)   (species 0, 2)
 //   upper / lower Sz = < (2 n_0 - n_U) / (2 n_U + 1e-8) >, < (2 n_2 - n_L) / (2 n_L + 1e-8) >
 //   pseudospin Sx    = | <psi| T |psi> | / Ne
+//   spin flip        = | <psi| T_sf |psi> | / Ne   (spinful spinflip_tunneling, :1519-1592)
 // T is the tunnelling operator of the Hamiltonian with coefficient +1/2 per
 // orbital pair (spinless: n <-> n + nphi; spinful: species pairs (0,2), (1,3),
-// (0,3), (1,2), each visited from both ends with 1/4), evaluated row by row
+// (0,3), (1,2), each visited from both ends with 1/4); T_sf keeps the
+// spin-flip pairs (0,3) and (1,2), each visited once with 1/2.  Both are evaluated row by row
 // with the Hamiltonian build's hop term (translation images, representative
 // lookup, fermion signs, FT phases).  Sums run per thread in row order and
 // per block in a fixed order; the host adds the block partials in order.
@@ -144,6 +146,16 @@ std::vector<HopTerm> sx_hops(const Geo& g) {
     return h;
 }
 
+// spin-flip part of the interlayer tunnelling (spinful only; coefficient 1/2 per pair)
+std::vector<HopTerm> sf_hops(const Geo& g) {
+    std::vector<HopTerm> h;
+    const int f = g.nphi;
+    auto add = [&](int a, int b) { h.push_back({a, b, a < b ? a : b, a < b ? b : a, 0.5, 1.0, 0.0, 0}); };
+    for (int n = 0; n < f; ++n) add(n, n + 3 * f);      // upper-up <-> lower-down
+    for (int n = f; n < 2 * f; ++n) add(n, n + f);      // upper-down <-> lower-up
+    return h;
+}
+
 }  // namespace
 
 void blqh_observables_impl(const blqh_basis* B, const double* psi_host, blqh_observables& out) {
@@ -161,30 +173,42 @@ void blqh_observables_impl(const blqh_basis* B, const double* psi_host, blqh_obs
             ft[a * f + b] = make_double2(std::cos(2.0 * M_PI * (a - b) * K / f), std::sin(2.0 * M_PI * (a - b) * K / f));
     DevBuf<double2> dft(ft.size());
     BLQH_HIP(hipMemcpyAsync(dft.get(), ft.data(), ft.size() * sizeof(double2), hipMemcpyHostToDevice, st));
-    const std::vector<HopTerm> hops = sx_hops(g);
+    std::vector<HopTerm> hops = sx_hops(g);
+    const int nsx = (int)hops.size();
+    if (g.S == 4) {
+        const std::vector<HopTerm> sf = sf_hops(g);
+        hops.insert(hops.end(), sf.begin(), sf.end());
+    }
+    const int nsf = (int)hops.size() - nsx;
     DevBuf<HopTerm> dh(hops.size());
     BLQH_HIP(hipMemcpyAsync(dh.get(), hops.data(), hops.size() * sizeof(HopTerm), hipMemcpyHostToDevice, st));
     RowCtx c = make_row_ctx(B);
     c.FT = dft.get();
     const int G = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kT), 2048));
-    DevBuf<double2> ph(G);
+    DevBuf<double2> ph(2 * (size_t)G);
     DevBuf<double> po((size_t)G * 2 * f), pz((size_t)G * 4);
-    hipLaunchKernelGGL(k_obs_hops, dim3(G), dim3(kT), 0, st, c, n, B->id.get(), dh.get(), (int)hops.size(), psi.get(),
-                       ph.get());
+    hipLaunchKernelGGL(k_obs_hops, dim3(G), dim3(kT), 0, st, c, n, B->id.get(), dh.get(), nsx, psi.get(), ph.get());
+    if (nsf)
+        hipLaunchKernelGGL(k_obs_hops, dim3(G), dim3(kT), 0, st, c, n, B->id.get(), dh.get() + nsx, nsf, psi.get(),
+                           ph.get() + G);
     hipLaunchKernelGGL(k_obs_occ, dim3(G, 2 * f), dim3(kT), 0, st, g, K, n, B->id.get(), B->period.get(), psi.get(),
                        po.get());
     hipLaunchKernelGGL(k_obs_sz, dim3(G), dim3(kT), 0, st, g, n, B->id.get(), psi.get(), pz.get());
     BLQH_HIP(hipGetLastError());
-    std::vector<double2> hh(G);
+    std::vector<double2> hh(2 * (size_t)G);
     std::vector<double> ho((size_t)G * 2 * f), hz((size_t)G * 4);
-    BLQH_HIP(hipMemcpyAsync(hh.data(), ph.get(), G * sizeof(double2), hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipMemcpyAsync(hh.data(), ph.get(), (nsf ? 2 : 1) * G * sizeof(double2), hipMemcpyDeviceToHost, st));
     BLQH_HIP(hipMemcpyAsync(ho.data(), po.get(), ho.size() * 8, hipMemcpyDeviceToHost, st));
     BLQH_HIP(hipMemcpyAsync(hz.data(), pz.get(), hz.size() * 8, hipMemcpyDeviceToHost, st));
     BLQH_HIP(hipStreamSynchronize(st));
-    double tr = 0, ti = 0, z[4] = {0, 0, 0, 0};
+    double tr = 0, ti = 0, fr = 0, fi = 0, z[4] = {0, 0, 0, 0};
     for (int b = 0; b < G; ++b) {
         tr += hh[b].x;
         ti += hh[b].y;
+        if (nsf) {
+            fr += hh[G + b].x;
+            fi += hh[G + b].y;
+        }
         for (int k = 0; k < 4; ++k) z[k] += hz[4 * b + k];
     }
     std::memset(&out, 0, sizeof(out));
@@ -195,6 +219,7 @@ void blqh_observables_impl(const blqh_basis* B, const double* psi_host, blqh_obs
         out.sz = (2.0 * z[1] - ne) / (2.0 * ne);
         out.upper_sz = z[2];
         out.lower_sz = z[3];
+        out.spinflip = std::hypot(fr, fi) / ne;
     }
     for (int y = 0; y < 2 * f; ++y) {
         double s = 0;

@@ -127,6 +127,18 @@ void blqh_tridiag_spectrum_impl(int m, const double* alpha, const double* offdia
     }
 }
 
+void blqh_tridiag_vectors_impl(int m, const double* alpha, const double* offdiag, int k0, int count, double* vecs) {
+    BLQH_REQUIRE(m >= 1 && k0 >= 0 && count >= 1 && k0 + count <= m, BLQH_EINVAL, "eigenvector index out of range");
+    std::vector<double> d(alpha, alpha + m), e(m > 1 ? m - 1 : 1, 0.0), z;
+    if (m > 1) std::memcpy(e.data(), offdiag, (m - 1) * sizeof(double));
+    blqh::tridiag_eigen(m, d, e, z);
+    for (int k = k0; k < k0 + count; ++k) {
+        const double* col = z.data() + (size_t)m * k;
+        const double sg = col[0] < 0 ? -1.0 : 1.0;
+        for (int i = 0; i < m; ++i) vecs[(size_t)(k - k0) * m + i] = sg * col[i];
+    }
+}
+
 int blqh_spectrum_distinct_impl(int m, const double* eig, int kmax, double gap, double* out) {
     // eigenvalues[0], then eigenvalues[i] with eigenvalues[i] - eigenvalues[i-1] > gap
     int count = 0;

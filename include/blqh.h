@@ -119,6 +119,9 @@ typedef struct blqh_lanczos_opts {
     const double* start;  /* optional start vector (2*nrows doubles), normalised here; NULL = seeded mt19937 */
     double res_tol;       /* also require the Ritz residual beta_{m} |s_m| < res_tol (bounds the state's
                              error; <= 0: energy criterion only) */
+    int32_t skip_states;  /* 1: stop after passes 1 and 2 (alpha, beta, E0; psi untouched, E_rq = NaN),
+                             for callers that reconstruct states with blqh_lanczos_reconstruct;
+                             0: reconstruct the ground state.  Ignored by blqh_batch_lanczos. */
 } blqh_lanczos_opts;
 
 typedef struct blqh_lanczos_result {
@@ -137,6 +140,26 @@ typedef struct blqh_lanczos_result {
  * lambda+1 doubles (beta[0] = 1 as in the reference). */
 int blqh_lanczos(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_result* r,
                  double* psi, double* alpha, double* beta);
+
+/* Pass 3 on its own: eigenstates_reconstruction of both reference trees
+ * (BLQH_without_spin/lanczos_hamiltonian.cpp:610-644; the spinful one,
+ * BLQH_with_spin/lanczos_hamiltonian.cpp:1143-1191, builds psir_0 and psir_1
+ * in one replay).  The Lanczos vectors v_0..v_{m-1} are regenerated from the
+ * start vector (seeded mt19937 when start is NULL, else start, normalised),
+ * and for each of the ns coefficient vectors coefs[s*m .. s*m+m-1] the state
+ * psi_s = sum_l coefs[s*m+l] conj(v_l) is formed, normalised and written to
+ * psi[s*2*nrows ..] (complex, 2*nrows doubles each; psi may be NULL), with
+ * erq[s] = Re<psi_s|H psi_s> (ground_state_energy / first_excited_state_energy,
+ * :1193-1215).  The replay recomputes alpha and beta with the engine's own
+ * recurrence rather than reusing the stored values; *max_dev (may be NULL)
+ * receives the largest relative difference between the recomputed and the
+ * given alpha[0..m-2] / beta[1..m-1] (either array may be NULL), which is 0
+ * for coefficients from blqh_lanczos with the same start and about 1e-8 for a
+ * record read back from its 8-digit text (resume of pass 3 from a record,
+ * read_from_file, :768-819).  1 <= ns <= 16. */
+int blqh_lanczos_reconstruct(const blqh_hamil* h, int32_t m, uint32_t seed, const double* start,
+                             const double* alpha, const double* beta, int32_t ns, const double* coefs, double* psi,
+                             double* erq, double* max_dev);
 
 /* ---- Batched solves: the sweeps over sectors and parameter points of
  *      BLQH_without_spin/main.cpp:76-165 and BLQH_with_spin/main.cpp:158-302,
@@ -233,6 +256,9 @@ typedef struct blqh_observables {
     double sz;                  /* spinful: (2 <N_up> - Ne) / (2 Ne); spinless: 0 */
     double upper_sz, lower_sz;  /* spinful upper_Sz / down_Sz; spinless: 0 */
     double occupation[2][64];   /* [a][j]: orbital j of layer a (spinful: species 0 / 1) */
+    double spinflip;            /* spinful spinflip_tunneling (BLQH_with_spin/lanczos_hamiltonian.cpp:1519-1592):
+                                   |<psi|T_sf|psi>| / Ne, T_sf the spin-flip interlayer hops with 1/2
+                                   per pair (upper-up <-> lower-down, upper-down <-> lower-up); spinless: 0 */
 } blqh_observables;
 int blqh_compute_observables(const blqh_basis* b, const double* psi, blqh_observables* out);
 
@@ -243,6 +269,11 @@ int blqh_compute_observables(const blqh_basis* b, const double* psi, blqh_observ
  * vec0[m] = eigenvector of eig[0] (psi_0, first component >= 0); any may be NULL. */
 int blqh_tridiag_spectrum(int32_t m, const double* alpha, const double* offdiag, double* eig, double* first,
                           double* vec0);
+/* Eigenvectors k0 .. k0+count-1 (ascending eigenvalues) of the same tridiagonal,
+ * each with first component >= 0, into vecs[(k - k0)*m ..]: the Krylov
+ * coefficients psi_0 / psi_1 of diag() (BLQH_with_spin/lanczos_hamiltonian.cpp:1106-1139). */
+int blqh_tridiag_vectors(int32_t m, const double* alpha, const double* offdiag, int32_t k0, int32_t count,
+                         double* vecs);
 /* Low-lying distinct levels: eig[0], then every eig[i] with eig[i] - eig[i-1] > gap,
  * at most kmax (the ghost filter of BLQH_without_spin/main.cpp:100-110, gap 1e-5).
  * Returns the count (>= 0) or an error code. */

@@ -535,7 +535,8 @@ int64_t orc_basis_find(void* h, uint64_t c) { return ((Basis*)h)->find(c); }
 // occupatation_number, pseudospin_Sz, Sz, upper_Sz, down_Sz, pseudospin_Sx
 // (BLQH_with_spin/lanczos_hamiltonian.cpp:1217-1500).
 // out: [0] pseudospin Sz, [1] pseudospin Sx, [2] Sz, [3] upper Sz, [4] lower Sz,
-// [5 + a*nphi + j] occupation number of orbital j in layer a.
+// [5 + a*nphi + j] occupation number of orbital j in layer a, [5 + 2*nphi]
+// spin-flip tunnelling (spinful spinflip_tunneling, :1519-1592; spinless 0).
 void orc_observables(void* h, const double* psi_in, double* out) {
     const Basis& B = *(const Basis*)h;
     const orc_sector& s = B.s;
@@ -592,32 +593,38 @@ void orc_observables(void* h, const double* psi_in, double* out) {
                  {0, nphi, 3 * nphi, 0.25}, {3 * nphi, 4 * nphi, -3 * nphi, 0.25},
                  {nphi, 2 * nphi, nphi, 0.25}, {2 * nphi, 3 * nphi, -nphi, 0.25}};
     }
-    cplx sx = 0;
-    for (long i = 0; i < n; ++i) {
-        const int Dl = K < 0 ? 1 : B.period[i];
-        for (int ql = 0; ql < Dl; ++ql) {
-            int signl = 1;
-            const uint64_t lb = ql == 0 ? B.id[i] : translate(s, B.id[i], ql, signl);
-            for (const Loop& lp : loops)
-                for (int m = lp.nbeg; m < lp.nend; ++m) {
-                    const int nt = m + lp.off;
-                    const uint64_t mask = (1ULL << m) + (1ULL << nt);
-                    const uint64_t Kn = mask & lb;
-                    if (Kn == mask || Kn == 0) continue;
-                    const uint64_t r0 = lb - Kn + (Kn ^ mask);
-                    const int Dr = K < 0 ? 1 : period_of(s, r0);
-                    for (int qr = 0; qr < Dr; ++qr) {
-                        int signr = 1;
-                        const uint64_t rb = qr == 0 ? r0 : inv_translate(s, r0, qr, signr);
-                        const long j = B.find(rb);
-                        if (j < 0) continue;
-                        const int sign = (nt > m ? sign1(lb, m, nt) : sign1(lb, nt, m)) * signl * signr;
-                        sx += FT[ql * nphi + qr] * std::conj(psi[i]) * psi[j] * (sign * lp.coef / std::sqrt(Dl * Dr));
+    auto expect = [&](const std::vector<Loop>& lps) {
+        cplx acc = 0;
+        for (long i = 0; i < n; ++i) {
+            const int Dl = K < 0 ? 1 : B.period[i];
+            for (int ql = 0; ql < Dl; ++ql) {
+                int signl = 1;
+                const uint64_t lb = ql == 0 ? B.id[i] : translate(s, B.id[i], ql, signl);
+                for (const Loop& lp : lps)
+                    for (int m = lp.nbeg; m < lp.nend; ++m) {
+                        const int nt = m + lp.off;
+                        const uint64_t mask = (1ULL << m) + (1ULL << nt);
+                        const uint64_t Kn = mask & lb;
+                        if (Kn == mask || Kn == 0) continue;
+                        const uint64_t r0 = lb - Kn + (Kn ^ mask);
+                        const int Dr = K < 0 ? 1 : period_of(s, r0);
+                        for (int qr = 0; qr < Dr; ++qr) {
+                            int signr = 1;
+                            const uint64_t rb = qr == 0 ? r0 : inv_translate(s, r0, qr, signr);
+                            const long j = B.find(rb);
+                            if (j < 0) continue;
+                            const int sign = (nt > m ? sign1(lb, m, nt) : sign1(lb, nt, m)) * signl * signr;
+                            acc += FT[ql * nphi + qr] * std::conj(psi[i]) * psi[j] *
+                                   (sign * lp.coef / std::sqrt(Dl * Dr));
+                        }
                     }
-                }
+            }
         }
-    }
-    out[1] = std::abs(sx) / s.nel;
+        return std::abs(acc) / s.nel;
+    };
+    out[1] = expect(loops);
+    // spin-flip tunnelling: upper-up <-> lower-down and upper-down <-> lower-up, 1/2 each (:1519-1592)
+    out[5 + 2 * nphi] = s.S == 4 ? expect({{0, nphi, 3 * nphi, 0.5}, {nphi, 2 * nphi, nphi, 0.5}}) : 0.0;
 }
 void orc_basis_free(void* h) { delete (Basis*)h; }
 

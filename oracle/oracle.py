@@ -283,7 +283,7 @@ def observables(basis, psi):
     """The reference's observables of psi in `basis` (dict)."""
     nphi = basis.sec.nphi
     psi = np.ascontiguousarray(psi, np.complex128)
-    out = np.empty(5 + 2 * nphi)
+    out = np.empty(6 + 2 * nphi)
     lib().orc_observables(basis.h, _p(psi), _p(out))
     return {"pseudospin_sz": out[0], "pseudospin_sx": out[1], "sz": out[2], "upper_sz": out[3], "lower_sz": out[4],
-            "occupation": out[5:].reshape(2, nphi)}
+            "occupation": out[5:5 + 2 * nphi].reshape(2, nphi), "spinflip": out[5 + 2 * nphi]}

@@ -45,3 +45,34 @@ def test_fixed_layer_counts_have_no_tunnelling():
     s = O.sector(S=4, nphi=4, nel=4, nel_up=2, nel_su=2, J=0, K=1)
     B, w, psi = _ground(s, 4, 1.0, 0.05)
     assert O.observables(B, psi)["pseudospin_sx"] == 0.0
+
+
+def test_spinflip_is_half_of_sx_under_lower_layer_spin_flip_symmetry():
+    """Exchanging spin up and down in the lower layer maps the same-spin hops
+    onto the spin-flip hops and leaves H invariant when Delta_Z = 0 and
+    theta_B = 0 (nphi even, so both tunnelling kinds carry the same
+    empty-orbital sign convention).  A non-degenerate ground state of a sector
+    with all spin counts is then an eigenvector of that exchange, so
+    <T_sf> = <T_ss> and spinflip = Sx / 2."""
+    s = O.sector(S=4, nphi=4, nel=4, nel_up=-1, nel_su=-1, J=0, K=1)
+    B, w, psi = _ground(s, 4, 1.0, 0.05)
+    assert w[1] - w[0] > 1e-6
+    ob = O.observables(B, psi)
+    assert ob["pseudospin_sx"] > 0.1
+    assert ob["spinflip"] == pytest.approx(ob["pseudospin_sx"] / 2, rel=1e-10)
+
+
+def test_spinflip_vanishes_when_spins_are_polarised():
+    """With every electron spin-up (nel_su = nel) each spin-flip target leaves
+    the basis, while the same-spin tunnelling still acts."""
+    s = O.sector(S=4, nphi=4, nel=4, nel_up=-1, nel_su=4, J=2, K=2)
+    B, w, psi = _ground(s, 4, 1.0, 0.05)
+    ob = O.observables(B, psi)
+    assert ob["spinflip"] == 0.0
+    assert ob["pseudospin_sx"] > 0.3
+
+
+def test_spinless_has_no_spinflip():
+    s = O.sector(S=2, nphi=6, nel=6, nel_up=-1, J=3, K=3)
+    B, w, psi = _ground(s, 6, 1.0, 0.05)
+    assert O.observables(B, psi)["spinflip"] == 0.0
