@@ -390,7 +390,8 @@ void refresh_probs(blqh_batch* b) {
     b->has_real = b->has_complex = false;
     for (blqh_hamil* H : b->hs) (H->pan.real ? b->has_real : b->has_complex) = true;
     // k_b1j reads step-major runs, k_b1 row-major ones: one relayout launch for all members
-    set_panel_layouts(b->hs.data(), (int)b->hs.size(), b->lpr == 64, b->relayout.get(), b->relayout.bytes());
+    // plain step order: a rebind re-lays out every member at each parameter point
+    set_panel_layouts(b->hs.data(), (int)b->hs.size(), b->lpr == 64, b->relayout.get(), b->relayout.bytes(), false);
     for (size_t p = 0; p < b->hs.size(); ++p) {
         blqh_hamil* H = b->hs[p];
         BProb& d = b->h_probs[p];

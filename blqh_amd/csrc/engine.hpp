@@ -72,14 +72,18 @@ void build_panels(blqh_hamil* H);
 // builds nothing) when the matrix does not fit it.
 bool build_tiles(blqh_hamil* H, int G);
 // Converts the panel runs to (jagged = true) or from step-major order in place
-// (the panel buffers keep their addresses); no-op when already there.  With a
-// scratch area of at least panel_layout_scratch_bytes(H) the conversion is
-// stream-ordered and allocates nothing; without one it allocates a temporary
-// and synchronises the library stream.
+// (the panel buffers keep their addresses); no-op when already there.  The
+// step-major order of set_panel_layout is the greedy one (steps that share x
+// cache lines across rows; it sizes its LDS from the runs, allocates and
+// synchronises).  With greedy = false and a scratch area of at least
+// panel_layout_scratch_bytes the plain step-major conversion is
+// stream-ordered and allocates nothing; without a scratch area it allocates a
+// temporary and synchronises the library stream.
 void set_panel_layout(blqh_hamil* H, bool jagged, void* scratch = nullptr, size_t scratch_bytes = 0);
 size_t panel_layout_scratch_bytes(const blqh_hamil* H);
 // The same for many matrices in one launch (those already in the layout are skipped).
-void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scratch, size_t scratch_bytes);
+void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scratch, size_t scratch_bytes,
+                       bool greedy);
 size_t panel_layout_scratch_bytes(blqh_hamil* const* hs, int count);
 
 // Row context with the basis' device pointers (tables left null).

@@ -18,7 +18,10 @@
 // of the largest entry (the K = 0 sectors are exactly real; K = nphi/2 sectors
 // carry sin(pi n) rounding residue of 1e-16).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "engine.hpp"
@@ -55,6 +58,25 @@ __global__ void k_panel_scatter(int64_t n, const int64_t* __restrict__ rowptr, c
         for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
             const int q = col[k] >> kPanelShift;
             if (q != p) { p = q; dst = off[(int64_t)q * n + r]; }
+            pcol[dst] = (uint16_t)(col[k] & (kPanelWidth - 1));
+            if (real) pvr[dst] = val[k].x; else pvc[dst] = val[k];
+            ++dst;
+        }
+    }
+}
+
+// The row-major panel arrays again from the CSR, in place (columns ascending
+// in every row, as built): run offsets from pbase / prow.
+__global__ void k_panel_rescatter(int64_t n, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                  const double2* __restrict__ val, const int64_t* __restrict__ pbase,
+                                  const uint32_t* __restrict__ prow, int real, uint16_t* __restrict__ pcol,
+                                  double* __restrict__ pvr, double2* __restrict__ pvc) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        int p = -1;
+        int64_t dst = 0;
+        for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+            const int q = col[k] >> kPanelShift;
+            if (q != p) { p = q; dst = pbase[q] + prow[(int64_t)q * (n + 1) + r]; }
             pcol[dst] = (uint16_t)(col[k] & (kPanelWidth - 1));
             if (real) pvr[dst] = val[k].x; else pvc[dst] = val[k];
             ++dst;
@@ -115,7 +137,8 @@ __global__ void __launch_bounds__(kBlk) k_absmax(int64_t nnz, const double2* __r
 // matrices in one launch (grid.y = matrix); each run is written to the
 // matrix's scratch arrays, four steps of loads in flight per lane, and
 // k_relayout_back copies the scratch over the panel arrays.
-__global__ void __launch_bounds__(kBlk) k_relayout_runs(const RelayoutJob* __restrict__ jobs, int to_jagged) {
+__global__ void __launch_bounds__(kBlk) k_relayout_runs(const RelayoutJob* __restrict__ jobs, int to_jagged,
+                                                       int64_t lo_e) {
     const RelayoutJob J = jobs[blockIdx.y];
     const int lane = threadIdx.x & 63;
     const int64_t n = J.n;
@@ -133,6 +156,7 @@ __global__ void __launch_bounds__(kBlk) k_relayout_runs(const RelayoutJob* __res
         const uint32_t len = e - s;
         const int64_t pb = J.pbase[p];
         const int64_t run = pb + rp[row0];
+        if ((int64_t)rp[rend] - rp[row0] <= lo_e) continue;  // ordered by the greedy kernel
         int64_t off = 0;
         for (uint32_t t = 0;; t += 4) {
             uint64_t m[4];
@@ -167,6 +191,222 @@ __global__ void __launch_bounds__(kBlk) k_relayout_runs(const RelayoutJob* __res
     }
 }
 
+// Greedy step order for the jagged layout (replaces the row order inside each
+// step).  One wavefront per (slice, panel) run.  At step t every lane with
+// more than t entries takes one of its entries; the lanes are served line by
+// line: repeatedly the x cache line (kLineCols columns, 128 bytes of complex
+// x) on which the most not-yet-served lanes still hold an entry is chosen
+// (ties: the lowest line), and each of those lanes takes its lowest-column
+// untaken entry on it.  Lanes gathering from one line at one step share one
+// L1-to-L2 request.  The step positions are those of the plain jagged order
+// (off_t + active lanes below l), so the SpMV kernels read either order; the
+// order depends only on the run's (row, column) set, so converting back and
+// forth reproduces it.  LDS per run (cap a power of two >= 64 entries):
+// mask[cap] u64 (lanes holding an untaken entry per line), ecol[cap] u16
+// (panel column per entry), dl[cap] u16 (sort buffer, then line index per
+// entry), lines[cap] u16 (sorted distinct lines, then the live-line list),
+// taken[cap] u8.  Runs of more than cap entries keep the plain order.
+constexpr int kLineShift = 3;
+
+__device__ void jag_copy_plain(const RelayoutJob& J, int64_t run, int64_t pb, uint32_t s, uint32_t len) {
+    int64_t off = 0;
+    for (uint32_t t = 0;; ++t) {
+        const uint64_t m = __ballot(t < len);
+        if (m == 0) break;
+        if (t < len) {
+            const int64_t to = run + off + lanes_below(m), from = pb + s + t;
+            J.tcol[to] = J.col[from];
+            if (J.real) J.tvr[to] = J.vr[from];
+            else J.tvc[to] = J.vc[from];
+        }
+        off += __popcll(m);
+    }
+}
+
+// This launch orders the runs with lo < E <= cap entries; runs above cap keep
+// the plain order when plain_big is set and are left alone otherwise.
+__global__ void __launch_bounds__(64) k_relayout_greedy(const RelayoutJob* __restrict__ jobs, int lo_e, int cap,
+                                                        int plain_big, int* __restrict__ err) {
+    extern __shared__ __align__(16) unsigned char lds_greedy[];
+    uint64_t* mask = (uint64_t*)lds_greedy;
+    uint16_t* ecol = (uint16_t*)(mask + cap);
+    uint16_t* dl = ecol + cap;
+    uint16_t* lines = dl + cap;
+    uint8_t* taken = (uint8_t*)(lines + cap);
+    const RelayoutJob J = jobs[blockIdx.y];
+    const int lane = threadIdx.x;
+    const int64_t n = J.n;
+    const int64_t nslices = (n + 63) >> 6;
+    const int64_t total = nslices * J.npan;
+    for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        const int p = (int)(w / nslices);
+        const int64_t slice = w - (int64_t)p * nslices;
+        const uint32_t* rp = J.prow + (int64_t)p * (n + 1);
+        const int64_t row0 = slice * 64, myrow = row0 + lane;
+        const int64_t rend = row0 + 64 < n ? row0 + 64 : n;
+        const uint32_t r0 = rp[row0];
+        const uint32_t s = myrow < n ? rp[myrow] : rp[rend];
+        const uint32_t e = myrow < n ? rp[myrow + 1] : rp[rend];
+        const uint32_t len = e - s;
+        const int64_t pb = J.pbase[p];
+        const int64_t run = pb + r0;
+        const int64_t E = (int64_t)rp[rend] - r0;
+        if (E <= lo_e) continue;
+        if (E > cap) {
+            if (plain_big) jag_copy_plain(J, run, pb, s, len);
+            continue;
+        }
+        const int a = (int)(s - r0);
+        uint32_t maxlen = len;
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t y = __shfl_xor(maxlen, o, 64);
+            maxlen = y > maxlen ? y : maxlen;
+        }
+        int P2 = 64;
+        while (P2 < E) P2 <<= 1;
+        __syncthreads();  // the previous run's LDS reads are done
+        for (int k = lane; k < P2; k += 64) {
+            if (k < E) {
+                const uint16_t c = J.col[run + k];
+                ecol[k] = c;
+                dl[k] = (uint16_t)(c >> kLineShift);
+                taken[k] = 0;
+            } else {
+                dl[k] = 0xFFFF;
+            }
+        }
+        __syncthreads();
+        // bitonic sort of dl[0, P2)
+        for (int size = 2; size <= P2; size <<= 1)
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int i = lane; i < P2 / 2; i += 64) {
+                    const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+                    const bool up = (lo & size) == 0;
+                    const uint16_t x = dl[lo], y = dl[hi];
+                    if ((x > y) == up) {
+                        dl[lo] = y;
+                        dl[hi] = x;
+                    }
+                }
+                __syncthreads();
+            }
+        // distinct lines
+        int D = 0;
+        for (int k0 = 0; k0 < E; k0 += 64) {
+            const int k = k0 + lane;
+            const bool first = k < E && (k == 0 || dl[k] != dl[k - 1]);
+            const uint64_t b = __ballot(first);
+            if (first) lines[D + lanes_below(b)] = dl[k];
+            D += __popcll(b);
+        }
+        __syncthreads();
+        for (int d = lane; d < D; d += 64) mask[d] = 0;
+        __syncthreads();
+        // line index of each of the lane's entries, lane bit on the line
+        for (int k = a; k < a + (int)len; ++k) {
+            const uint16_t ln = (uint16_t)(ecol[k] >> kLineShift);
+            int lo = 0, hi = D - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (lines[mid] < ln) lo = mid + 1;
+                else hi = mid;
+            }
+            dl[k] = (uint16_t)lo;
+            atomicOr((unsigned long long*)&mask[lo], 1ull << lane);
+        }
+        __syncthreads();
+        // lines[] becomes the list of live line indices
+        for (int d = lane; d < D; d += 64) lines[d] = (uint16_t)d;
+        int live = D;
+        __syncthreads();
+        int64_t off = 0;
+        for (uint32_t t = 0; t < maxlen; ++t) {
+            const uint64_t act = __ballot(t < len);
+            uint64_t U = act;
+            int mine = -1;
+            while (U) {
+                int key = -1;
+                for (int i = lane; i < live; i += 64) {
+                    const int d = lines[i];
+                    const int pc = __popcll(mask[d] & U);
+                    const int kk = pc ? (pc << 16) | (0xFFFF - d) : -1;
+                    key = kk > key ? kk : key;
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    const int y = __shfl_xor(key, o, 64);
+                    key = y > key ? y : key;
+                }
+                if (key < 0) {  // no line serves an active lane: cannot happen
+                    if (lane == 0) atomicOr(err, 1);
+                    break;
+                }
+                const int bd = 0xFFFF - (key & 0xFFFF);
+                const uint64_t take = mask[bd] & U;
+                __syncthreads();  // every lane has read mask[bd]
+                if ((take >> lane) & 1) {
+                    int kb = -1, cnt = 0;
+                    for (int k = a; k < a + (int)len; ++k)
+                        if (!taken[k] && dl[k] == bd) {
+                            ++cnt;
+                            if (kb < 0 || ecol[k] < ecol[kb]) kb = k;
+                        }
+                    if (kb < 0) {
+                        atomicOr(err, 2);
+                    } else {
+                        taken[kb] = 1;
+                        mine = kb;
+                        if (cnt == 1) atomicAnd((unsigned long long*)&mask[bd], ~(1ull << lane));
+                    }
+                }
+                U &= ~take;
+                __syncthreads();
+            }
+            if (t < len && mine >= 0) {
+                const int64_t to = run + off + lanes_below(act), from = run + mine;
+                J.tcol[to] = J.col[from];
+                if (J.real) J.tvr[to] = J.vr[from];
+                else J.tvc[to] = J.vc[from];
+            }
+            off += __popcll(act);
+            // drop exhausted lines from the live list (in place, order kept)
+            int nl = 0;
+            for (int i0 = 0; i0 < live; i0 += 64) {
+                const int i = i0 + lane;
+                const int d = i < live ? lines[i] : 0;
+                const bool keep = i < live && mask[d] != 0;
+                const uint64_t b = __ballot(keep);
+                __syncthreads();
+                if (keep) lines[nl + lanes_below(b)] = (uint16_t)d;
+                nl += __popcll(b);
+                __syncthreads();
+            }
+            live = nl;
+        }
+    }
+}
+
+// Largest (slice, panel) run of every job: sizes the greedy order's LDS.
+__global__ void __launch_bounds__(kBlk) k_run_max(const RelayoutJob* __restrict__ jobs, unsigned* __restrict__ out) {
+    const RelayoutJob J = jobs[blockIdx.y];
+    const int64_t n = J.n;
+    const int64_t nslices = (n + 63) >> 6;
+    const int64_t total = nslices * J.npan;
+    unsigned m = 0;
+    for (int64_t w = (int64_t)blockIdx.x * kBlk + threadIdx.x; w < total; w += (int64_t)gridDim.x * kBlk) {
+        const int p = (int)(w / nslices);
+        const int64_t slice = w - (int64_t)p * nslices;
+        const uint32_t* rp = J.prow + (int64_t)p * (n + 1);
+        const int64_t row0 = slice * 64, rend = row0 + 64 < n ? row0 + 64 : n;
+        const unsigned E = rp[rend] - rp[row0];
+        m = E > m ? E : m;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned y = __shfl_xor(m, o, 64);
+        m = y > m ? y : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
 __global__ void __launch_bounds__(kBlk) k_relayout_back(const RelayoutJob* __restrict__ jobs) {
     const RelayoutJob J = jobs[blockIdx.y];
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < J.nnz; k += (int64_t)gridDim.x * blockDim.x) {
@@ -182,6 +422,52 @@ namespace blqh {
 
 namespace {
 constexpr size_t kAlign = 256;
+
+size_t greedy_lds_bytes(int cap) { return (size_t)cap * (8 + 2 + 2 + 2 + 1); }
+
+// Most entries per run that the greedy jagged order can hold in LDS (0: plain
+// order, BLQH_JAGGED_ORDER=plain): 8192 (120 KiB of LDS per workgroup) where
+// the device allows it, else 4096 (60 KiB).
+int greedy_cap_max() {
+    static int cap = -1;
+    if (cap >= 0) return cap;
+    const char* env = std::getenv("BLQH_JAGGED_ORDER");
+    if (env && std::strcmp(env, "plain") == 0) return cap = 0;
+    int dev = 0, maxlds = 0;
+    BLQH_HIP(hipGetDevice(&dev));
+    BLQH_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    cap = 4096;
+    if ((size_t)maxlds >= greedy_lds_bytes(8192) &&
+        hipFuncSetAttribute((const void*)k_relayout_greedy, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)greedy_lds_bytes(8192)) == hipSuccess)
+        cap = 8192;
+    (void)hipGetLastError();
+    return cap;
+}
+
+// Runs of up to kGreedySmall entries are ordered with 30 KiB of LDS per
+// workgroup (several workgroups per CU); larger ones in a second launch.
+constexpr int kGreedySmall = 2048;
+
+// LDS capacity for the large runs of these jobs: the smallest power of two
+// >= 64 holding their largest run, at most greedy_cap_max() (larger runs keep
+// the plain order).
+int greedy_cap(const RelayoutJob* dj, int njobs, int64_t max_runs, hipStream_t st, unsigned& max_e) {
+    max_e = 0;
+    const int cmax = greedy_cap_max();
+    if (cmax == 0) return 0;
+    DevBuf<unsigned> mx(1);
+    BLQH_HIP(hipMemsetAsync(mx.get(), 0, sizeof(unsigned), st));
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(max_runs, kBlk), 1024));
+    hipLaunchKernelGGL(k_run_max, dim3(g, (unsigned)njobs), dim3(kBlk), 0, st, dj, mx.get());
+    unsigned h = 0;
+    BLQH_HIP(hipMemcpyAsync(&h, mx.get(), sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    max_e = h;
+    int cap = 64;
+    while (cap < cmax && (unsigned)cap < h) cap <<= 1;
+    return cap;
+}
 size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 size_t entry_bytes(const blqh_hamil* H) {
     const size_t m = (size_t)std::max<int64_t>(H->nnz, 1);
@@ -195,11 +481,12 @@ size_t panel_layout_scratch_bytes(blqh_hamil* const* hs, int count) {
     return b;
 }
 
-void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scratch, size_t scratch_bytes) {
-    std::vector<blqh_hamil*> todo;
-    for (int i = 0; i < count; ++i)
-        if (hs[i]->pan.built && hs[i]->pan.jagged != jagged) todo.push_back(hs[i]);
-    if (todo.empty()) return;
+namespace {
+
+// One conversion of every matrix in todo: row-major -> step-major (greedy or
+// plain order) when jagged, else step-major (either order) -> row-major.
+void relayout_pass(const std::vector<blqh_hamil*>& todo, bool jagged, bool greedy, void* scratch,
+                   size_t scratch_bytes) {
     BLQH_REQUIRE(todo.size() <= 65535, BLQH_ERANGE, "too many matrices for one panel relayout");
     hipStream_t st = stream();
     const size_t need = panel_layout_scratch_bytes(todo.data(), (int)todo.size());
@@ -238,17 +525,81 @@ void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scra
     BLQH_HIP(hipStreamSynchronize(st));
     BLQH_HIP(hipMemcpy(sc, jobs.data(), jobs.size() * sizeof(RelayoutJob), hipMemcpyHostToDevice));
     const RelayoutJob* dj = (const RelayoutJob*)sc;
+    const auto t0 = std::chrono::steady_clock::now();
+    unsigned max_e = 0;
+    const int cap = jagged && greedy ? greedy_cap(dj, (int)jobs.size(), max_runs, st, max_e) : 0;
+    DevBuf<int> err;
     const int gx = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(max_runs, kBlk / 64), 4096));
-    hipLaunchKernelGGL(k_relayout_runs, dim3(gx, (unsigned)jobs.size()), dim3(kBlk), 0, st, dj, jagged ? 1 : 0);
+    if (cap > 0) {
+        err.alloc(1);
+        BLQH_HIP(hipMemsetAsync(err.get(), 0, sizeof(int), st));
+        const int gg = (int)std::max<int64_t>(1, std::min<int64_t>(max_runs, 8192));
+        const int small = std::min(cap, kGreedySmall);
+        hipLaunchKernelGGL(k_relayout_greedy, dim3(gg, (unsigned)jobs.size()), dim3(64), greedy_lds_bytes(small), st,
+                           dj, 0, small, 0, err.get());
+        if (cap > small)
+            hipLaunchKernelGGL(k_relayout_greedy, dim3(gg, (unsigned)jobs.size()), dim3(64), greedy_lds_bytes(cap),
+                               st, dj, small, cap, 0, err.get());
+        if (max_e > (unsigned)cap)  // runs too large for LDS keep the plain order
+            hipLaunchKernelGGL(k_relayout_runs, dim3(gx, (unsigned)jobs.size()), dim3(kBlk), 0, st, dj, 1,
+                               (int64_t)cap);
+    } else {
+        hipLaunchKernelGGL(k_relayout_runs, dim3(gx, (unsigned)jobs.size()), dim3(kBlk), 0, st, dj, jagged ? 1 : 0,
+                           (int64_t)-1);
+    }
     const int gb = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(max_nnz, kBlk), 4096));
     hipLaunchKernelGGL(k_relayout_back, dim3(gb, (unsigned)jobs.size()), dim3(kBlk), 0, st, dj);
     BLQH_HIP(hipGetLastError());
+    if (cap > 0) {
+        int herr = 0;
+        BLQH_HIP(hipMemcpyAsync(&herr, err.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+        BLQH_HIP(hipStreamSynchronize(st));
+        BLQH_REQUIRE(herr == 0, BLQH_EINTERNAL, "greedy jagged relayout left a lane without an entry");
+        if (std::getenv("BLQH_TUNE_LOG"))
+            std::fprintf(stderr, "blqh panel relayout: %zu matrices, greedy step order (cap %d): %.1f ms\n",
+                         todo.size(), cap,
+                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
     if (own.get()) BLQH_HIP(hipStreamSynchronize(st));  // the temporary is freed on return
-    for (auto* H : todo) H->pan.jagged = jagged;
+    for (auto* H : todo) {
+        H->pan.jagged = jagged;
+        H->pan.greedy = jagged && greedy;
+    }
+}
+
+}  // namespace
+
+void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scratch, size_t scratch_bytes,
+                       bool greedy) {
+    // Every matrix ends in the requested order whatever happened to it before,
+    // so a given caller always sums in the same order: the row-major order is
+    // always the build's (ascending columns), the plain step order is derived
+    // from it, and a greedy layout leaves step-major order by a fresh scatter
+    // from the CSR.
+    std::vector<blqh_hamil*> todo, back;
+    hipStream_t st = stream();
+    for (int i = 0; i < count; ++i) {
+        blqh_hamil* H = hs[i];
+        Panels& P = H->pan;
+        if (!P.built || (P.jagged == jagged && (!jagged || P.greedy == greedy))) continue;
+        if (P.jagged && P.greedy) {
+            const int gr = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(H->n, kBlk), 1), 65536);
+            hipLaunchKernelGGL(k_panel_rescatter, dim3(gr), dim3(kBlk), 0, st, H->n, H->rowptr.get(), H->col.get(),
+                               H->val.get(), P.pbase.get(), P.prow.get(), P.real ? 1 : 0, P.pcol.get(),
+                               P.pvr.get(), P.pvc.get());
+            BLQH_HIP(hipGetLastError());
+            P.jagged = P.greedy = false;
+        } else if (P.jagged) {
+            back.push_back(H);  // plain step order -> row-major by relayout
+        }
+        if (jagged) todo.push_back(H);
+    }
+    if (!back.empty()) relayout_pass(back, false, false, scratch, scratch_bytes);
+    if (!todo.empty()) relayout_pass(todo, jagged, greedy, scratch, scratch_bytes);
 }
 
 void set_panel_layout(blqh_hamil* H, bool jagged, void* scratch, size_t scratch_bytes) {
-    set_panel_layouts(&H, 1, jagged, scratch, scratch_bytes);
+    set_panel_layouts(&H, 1, jagged, scratch, scratch_bytes, true);
 }
 
 size_t panel_layout_scratch_bytes(const blqh_hamil* H) {

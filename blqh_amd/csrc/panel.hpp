@@ -14,6 +14,7 @@ struct Panels {
     bool built = false;
     bool real = false;
     bool jagged = false;     // entry order inside each (64-row slice, panel) run: row-major or step-major
+    bool greedy = false;     // step-major with the greedy step order (set_panel_layout), else the plain one
     int npan = 0;
     DevBuf<int64_t> pbase;   // [npan + 1]
     DevBuf<uint32_t> prow;   // [npan][n + 1]
