@@ -202,6 +202,9 @@ int blqh_hamil_set_spmv(blqh_hamil* h, int32_t kind, int32_t lanes, int32_t thre
     if (kind == 3)
         ok = h->pan.built && lanes == 64 && (threads == 256 || threads == 512 || threads == 1024) && wpc >= 1 &&
              wpc <= 4 && (rows == 1 || rows == 2 || rows == 4 || rows == 8);
+    if (kind == 5)
+        ok = h->pan.built && (lanes == 2 || lanes == 4 || lanes == 8) && (threads == 512 || threads == 1024) &&
+             wpc >= 1 && wpc <= 4 && (rows == 1 || rows == 2 || rows == 4) && lanes * rows <= 8;
     if (kind == 4) {
         // the column-tile layout is built here on first use (one workgroup per CU)
         int rc = guarded([&] { ok = blqh::build_tiles(h, blqh::runtime().num_cus); });

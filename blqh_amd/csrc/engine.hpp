@@ -79,11 +79,12 @@ bool build_tiles(blqh_hamil* H, int G);
 // panel_layout_scratch_bytes the plain step-major conversion is
 // stream-ordered and allocates nothing; without a scratch area it allocates a
 // temporary and synchronises the library stream.
-void set_panel_layout(blqh_hamil* H, bool jagged, void* scratch = nullptr, size_t scratch_bytes = 0);
+// rows > 1: step-major with that many rows per lane (plain order), for SpMV kind 5.
+void set_panel_layout(blqh_hamil* H, bool jagged, void* scratch = nullptr, size_t scratch_bytes = 0, int rows = 1);
 size_t panel_layout_scratch_bytes(const blqh_hamil* H);
 // The same for many matrices in one launch (those already in the layout are skipped).
 void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scratch, size_t scratch_bytes,
-                       bool greedy);
+                       bool greedy, int rows = 1);
 size_t panel_layout_scratch_bytes(blqh_hamil* const* hs, int count);
 
 // Row context with the basis' device pointers (tables left null).

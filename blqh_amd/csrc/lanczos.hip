@@ -546,6 +546,75 @@ __global__ void __launch_bounds__(WPB * 64) k_jag(PanelView P, const double2* __
     if (LZ) grid_reduce(dacc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
 }
 
+// SpMV kind 5: as k_jag, but lane l of a wave sums rows l, l+64, ..,
+// l+64(R-1) of a 64R-row super-slice in one stream per panel (panels laid out
+// with R rows per lane), so the lanes' stream lengths even out and fewer lane
+// slots idle per step.  MAXS super-slices per wave; R * MAXS <= 8.
+template <bool REAL, bool LZ, int WPB, int R, int MAXS>
+__global__ void __launch_bounds__(WPB * 64) k_jagr(PanelView P, const double2* __restrict__ x,
+                                                   double2* __restrict__ out, const double2* __restrict__ Wp, Scal sc,
+                                                   int j, double* partials, unsigned* ticket) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t n = P.n;
+    const int64_t nss = (n + 64 * R - 1) / (64 * R);
+    const int64_t W = (int64_t)gridDim.x * WPB;
+    const int64_t wid = (int64_t)blockIdx.x * WPB + wv;
+    double ib = 0, cp = 0;
+    if (LZ) {
+        ib = sc.invb[j];
+        cp = j > 0 ? sc.beta[j] * sc.invb[j - 1] : 0.0;
+    }
+    __shared__ double2 yl[WPB * MAXS * R * 64];  // row sums of this wavefront's super-slices
+    double2* my = yl + wv * MAXS * R * 64;
+    double dacc = 0.0;
+    for (int64_t r0 = 0; r0 < nss; r0 += W * MAXS) {
+        for (int k = 0; k < MAXS * R; ++k) my[k * 64 + lane] = make_double2(0.0, 0.0);
+        for (int p = 0; p < P.npan; ++p) {
+            __syncthreads();  // keep the workgroup on one panel
+            const uint32_t* rp = P.prow + (int64_t)p * (n + 1);
+            const int64_t base = P.pbase[p];
+            const double2* xp = x + (int64_t)p * kPanelWidth;
+#pragma unroll 1
+            for (int k = 0; k < MAXS; ++k) {
+                const int64_t ss = r0 + wid + (int64_t)k * W;
+                if (ss >= nss) continue;  // wave-uniform
+                const int64_t row0 = ss * 64 * R;
+                uint32_t cum[R];
+                uint32_t c = 0;
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const int64_t row = row0 + lane + 64 * q;
+                    c += row < n ? rp[row + 1] - rp[row] : 0u;
+                    cum[q] = c;
+                }
+                const int64_t run = base + rp[row0];
+                jag_rows_sum<REAL, R>(P.pcol + run, P.pvr + run, P.pvc + run, xp, cum, my + k * R * 64 + lane);
+            }
+        }
+        for (int k = 0; k < MAXS; ++k)
+            for (int q = 0; q < R; ++q) {
+                const int64_t ss = r0 + wid + (int64_t)k * W;
+                const int64_t myrow = ss * 64 * R + lane + 64 * q;
+                if (ss >= nss || myrow >= n) continue;
+                const double2 yv = my[(k * R + q) * 64 + lane];
+                if (LZ) {
+                    double ur = ib * yv.x, ui = ib * yv.y;
+                    if (j > 0) {
+                        const double2 w = Wp[myrow];
+                        ur -= cp * w.x;
+                        ui -= cp * w.y;
+                    }
+                    out[myrow] = make_double2(ur, ui);
+                    const double2 w = x[myrow];
+                    dacc += w.x * ur + w.y * ui;
+                } else {
+                    out[myrow] = yv;
+                }
+            }
+    }
+    if (LZ) grid_reduce(dacc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
+}
+
 // ---- column-tile SpMV (kind 4) ---------------------------------------------------------
 // Workgroup b (16 wavefronts) owns a row block; lane l of wave w keeps the
 // sums of its R rows (one per owned slice) in registers for the whole
@@ -772,7 +841,7 @@ void sps_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2
 template <bool LZ>
 void jag_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2* out, const double2* Wp,
                   const Scal& sc, int j, double* partials, unsigned* ticket, hipStream_t st) {
-    BLQH_REQUIRE(H->pan.jagged, BLQH_EINTERNAL, "jagged SpMV on a row-major panel layout");
+    BLQH_REQUIRE(H->pan.jagged && H->pan.rows == 1, BLQH_EINTERNAL, "jagged SpMV without its panel layout");
     const PanelView V = view(H->pan, H->n);
     const dim3 grid(sps_grid(H, pl)), block(pl.threads);
 #define BLQH_JAG_CASE(WPB, MAXS)                                                                            \
@@ -788,6 +857,35 @@ void jag_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2
     BLQH_JAG_CASE(16, 1) BLQH_JAG_CASE(16, 2) BLQH_JAG_CASE(16, 4) BLQH_JAG_CASE(16, 8)
 #undef BLQH_JAG_CASE
     throw Error(BLQH_EINTERNAL, "unsupported jagged SpMV plan");
+}
+
+int jagr_grid(const blqh_hamil* H, const Plan& pl) {
+    const int64_t nss = ceil_div(H->n, (int64_t)64 * pl.lpr);
+    const int64_t groups = ceil_div(nss, (int64_t)(pl.threads / 64) * pl.rows);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(groups, (int64_t)runtime().num_cus * pl.wpc));
+}
+
+template <bool LZ>
+void jagr_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2* out, const double2* Wp,
+                   const Scal& sc, int j, double* partials, unsigned* ticket, hipStream_t st) {
+    BLQH_REQUIRE(H->pan.jagged && H->pan.rows == pl.lpr, BLQH_EINTERNAL,
+                 "multi-row jagged SpMV without its panel layout");
+    const PanelView V = view(H->pan, H->n);
+    const dim3 grid(jagr_grid(H, pl)), block(pl.threads);
+#define BLQH_JAGR_CASE(WPB, R, MAXS)                                                                            \
+    if (pl.threads == WPB * 64 && pl.lpr == R && pl.rows == MAXS) {                                             \
+        if (H->pan.real)                                                                                        \
+            hipLaunchKernelGGL((k_jagr<true, LZ, WPB, R, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_jagr<false, LZ, WPB, R, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+        return;                                                                                                 \
+    }
+    BLQH_JAGR_CASE(8, 2, 1) BLQH_JAGR_CASE(8, 2, 2) BLQH_JAGR_CASE(8, 2, 4)
+    BLQH_JAGR_CASE(8, 4, 1) BLQH_JAGR_CASE(8, 4, 2) BLQH_JAGR_CASE(8, 8, 1)
+    BLQH_JAGR_CASE(16, 2, 1) BLQH_JAGR_CASE(16, 2, 2) BLQH_JAGR_CASE(16, 2, 4)
+    BLQH_JAGR_CASE(16, 4, 1) BLQH_JAGR_CASE(16, 4, 2) BLQH_JAGR_CASE(16, 8, 1)
+#undef BLQH_JAGR_CASE
+    throw Error(BLQH_EINTERNAL, "unsupported multi-row jagged SpMV plan");
 }
 
 // smallest compiled slice count per wave that covers the layout's R
@@ -819,7 +917,9 @@ void t4_dispatch(const blqh_hamil* H, const double2* x, double2* out, const doub
 }
 
 void launch_spmv_plan(const blqh_hamil* H, const Plan& pl, const double2* x, double2* y, hipStream_t st) {
-    if (pl.kind == 4) {
+    if (pl.kind == 5) {
+        jagr_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
+    } else if (pl.kind == 4) {
         t4_dispatch<false>(H, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
     } else if (pl.kind == 3) {
         jag_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
@@ -836,12 +936,15 @@ void launch_spmv_plan(const blqh_hamil* H, const Plan& pl, const double2* x, dou
 // grid size (for partial-sum buffers) of the fused K1 under a plan
 int lz1_grid(const blqh_hamil* H, const Plan& pl) {
     if (pl.kind == 4) return H->til.G;
+    if (pl.kind == 5) return jagr_grid(H, pl);
     return pl.kind >= 2 ? sps_grid(H, pl) : pl.kind == 1 ? panel_launch(H, pl).grid : spmv_grid(H->n, pl.lpr);
 }
 
 void launch_lz1(const blqh_hamil* H, const Plan& pl, const double2* W, const double2* Wp, double2* u, const Scal& sc,
                 int j, double* partials, unsigned* ticket, hipStream_t st) {
-    if (pl.kind == 4) {
+    if (pl.kind == 5) {
+        jagr_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
+    } else if (pl.kind == 4) {
         t4_dispatch<true>(H, W, u, Wp, sc, j, partials, ticket, st);
     } else if (pl.kind == 3) {
         jag_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
@@ -902,6 +1005,16 @@ void tune(blqh_hamil* H) {
                 while (ms < need && ms < 8) ms <<= 1;
                 cands.push_back(Plan{3, 64, pt, wpc, ms});
             }
+    if (H->pan.built)  // R rows per lane: one layout conversion per R
+        for (int R : {2, 4, 8})
+            for (int pt : {512, 1024})
+                for (int wpc : {1, 2}) {
+                    const int64_t waves = (int64_t)runtime().num_cus * wpc * (pt / 64);
+                    const int64_t need = ceil_div(ceil_div(H->n, (int64_t)64 * R), waves);
+                    int ms = 1;
+                    while (ms < need && ms * R < 8) ms <<= 1;
+                    cands.push_back(Plan{5, R, pt, wpc, ms});
+                }
     std::vector<double2> hx(H->n);
     std::mt19937 rng(12345);
     for (auto& z : hx) z = make_double2(rng() * 1.0 / rng.max() - 0.5, rng() * 1.0 / rng.max() - 0.5);
@@ -915,7 +1028,7 @@ void tune(blqh_hamil* H) {
     const int reps = H->nnz > 50000000 ? 3 : 10;
     const bool log = std::getenv("BLQH_TUNE_LOG") != nullptr;  // print the timing of every variant
     for (const Plan& pl : cands) {
-        if (pl.kind >= 1) set_panel_layout(H, pl.kind == 3);
+        if (pl.kind >= 1) set_panel_layout(H, pl.kind == 3 || pl.kind == 5, nullptr, 0, pl.kind == 5 ? pl.lpr : 1);
         launch_spmv_plan(H, pl, x.get(), y.get(), st);  // warm
         BLQH_HIP(hipEventRecord(e0, st));
         for (int r = 0; r < reps; ++r) launch_spmv_plan(H, pl, x.get(), y.get(), st);
@@ -954,7 +1067,8 @@ void autotune(blqh_hamil* H) {
             T.tried = false;
         }
     }
-    if (H->spmv_kind >= 1 && H->spmv_kind <= 3) set_panel_layout(H, H->spmv_kind == 3);
+    if ((H->spmv_kind >= 1 && H->spmv_kind <= 3) || H->spmv_kind == 5)
+        set_panel_layout(H, H->spmv_kind == 3 || H->spmv_kind == 5, nullptr, 0, H->spmv_kind == 5 ? H->spmv_lpr : 1);
 }
 
 }  // namespace

@@ -385,6 +385,73 @@ __global__ void __launch_bounds__(64) k_relayout_greedy(const RelayoutJob* __res
     }
 }
 
+// Step-major order with R rows per lane (SpMV kind 5): the run of a
+// 64R-row super-slice in a panel (its rows' entries, contiguous in the
+// panel) is stored so that lane l walks rows l, l+64, .., l+64(R-1) one after
+// the other as a single stream, and step t holds entry t of every lane's
+// stream (active lanes in order).  Summing R rows per lane evens the lanes'
+// stream lengths, so fewer lane-slots per step stay idle.
+template <int R>
+__global__ void __launch_bounds__(kBlk) k_relayout_rows(const RelayoutJob* __restrict__ jobs) {
+    const RelayoutJob J = jobs[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int64_t n = J.n;
+    const int64_t nss = (n + 64 * R - 1) / (64 * R);
+    const int64_t total = nss * J.npan;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < total; w += nw) {
+        const int p = (int)(w / nss);
+        const int64_t ss = w - (int64_t)p * nss;
+        const uint32_t* rp = J.prow + (int64_t)p * (n + 1);
+        const int64_t row0 = ss * 64 * R;
+        const int64_t pb = J.pbase[p];
+        const int64_t run = pb + rp[row0];
+        uint32_t st[R], ln[R];
+        uint32_t L = 0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int64_t row = row0 + lane + 64 * q;
+            st[q] = row < n ? rp[row] : 0;
+            ln[q] = row < n ? rp[row + 1] - st[q] : 0;
+            L += ln[q];
+        }
+        int cur = 0;
+        uint32_t k = 0;
+        int64_t off = 0;
+        for (uint32_t t = 0;; ++t) {
+            const uint64_t m = __ballot(t < L);
+            if (m == 0) break;
+            if (t < L) {
+                uint32_t lc = 0, sc = 0;
+                bool ok = true;
+                for (;;) {  // the lane's current row (rows without entries skipped; t < L keeps cur < R)
+#pragma unroll
+                    for (int q = 0; q < R; ++q)
+                        if (q == cur) {
+                            lc = ln[q];
+                            sc = st[q];
+                        }
+                    if (k < lc) break;
+                    ++cur;
+                    k = 0;
+                    if (cur >= R) {
+                        ok = false;
+                        break;
+                    }
+                }
+                if (ok) {
+                    const int64_t to = run + off + lanes_below(m), from = pb + sc + k;
+                    J.tcol[to] = J.col[from];
+                    if (J.real) J.tvr[to] = J.vr[from];
+                    else J.tvc[to] = J.vc[from];
+                    ++k;
+                }
+            }
+            off += __popcll(m);
+        }
+    }
+}
+
 // Largest (slice, panel) run of every job: sizes the greedy order's LDS.
 __global__ void __launch_bounds__(kBlk) k_run_max(const RelayoutJob* __restrict__ jobs, unsigned* __restrict__ out) {
     const RelayoutJob J = jobs[blockIdx.y];
@@ -484,8 +551,9 @@ size_t panel_layout_scratch_bytes(blqh_hamil* const* hs, int count) {
 namespace {
 
 // One conversion of every matrix in todo: row-major -> step-major (greedy or
-// plain order) when jagged, else step-major (either order) -> row-major.
-void relayout_pass(const std::vector<blqh_hamil*>& todo, bool jagged, bool greedy, void* scratch,
+// plain order, `rows` rows per lane) when jagged, else plain step-major with
+// one row per lane -> row-major.
+void relayout_pass(const std::vector<blqh_hamil*>& todo, bool jagged, bool greedy, int rows, void* scratch,
                    size_t scratch_bytes) {
     BLQH_REQUIRE(todo.size() <= 65535, BLQH_ERANGE, "too many matrices for one panel relayout");
     hipStream_t st = stream();
@@ -527,10 +595,16 @@ void relayout_pass(const std::vector<blqh_hamil*>& todo, bool jagged, bool greed
     const RelayoutJob* dj = (const RelayoutJob*)sc;
     const auto t0 = std::chrono::steady_clock::now();
     unsigned max_e = 0;
-    const int cap = jagged && greedy ? greedy_cap(dj, (int)jobs.size(), max_runs, st, max_e) : 0;
+    const int cap = jagged && greedy && rows == 1 ? greedy_cap(dj, (int)jobs.size(), max_runs, st, max_e) : 0;
     DevBuf<int> err;
     const int gx = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(max_runs, kBlk / 64), 4096));
-    if (cap > 0) {
+    if (jagged && rows > 1) {
+        const dim3 g(gx, (unsigned)jobs.size());
+        if (rows == 2) hipLaunchKernelGGL(k_relayout_rows<2>, g, dim3(kBlk), 0, st, dj);
+        else if (rows == 4) hipLaunchKernelGGL(k_relayout_rows<4>, g, dim3(kBlk), 0, st, dj);
+        else if (rows == 8) hipLaunchKernelGGL(k_relayout_rows<8>, g, dim3(kBlk), 0, st, dj);
+        else throw Error(BLQH_EINTERNAL, "unsupported rows per lane");
+    } else if (cap > 0) {
         err.alloc(1);
         BLQH_HIP(hipMemsetAsync(err.get(), 0, sizeof(int), st));
         const int gg = (int)std::max<int64_t>(1, std::min<int64_t>(max_runs, 8192));
@@ -563,14 +637,17 @@ void relayout_pass(const std::vector<blqh_hamil*>& todo, bool jagged, bool greed
     if (own.get()) BLQH_HIP(hipStreamSynchronize(st));  // the temporary is freed on return
     for (auto* H : todo) {
         H->pan.jagged = jagged;
-        H->pan.greedy = jagged && greedy;
+        H->pan.greedy = jagged && greedy && rows == 1;
+        H->pan.rows = jagged ? rows : 1;
     }
 }
 
 }  // namespace
 
 void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scratch, size_t scratch_bytes,
-                       bool greedy) {
+                       bool greedy, int rows) {
+    if (!jagged) rows = 1;
+    if (rows > 1) greedy = false;
     // Every matrix ends in the requested order whatever happened to it before,
     // so a given caller always sums in the same order: the row-major order is
     // always the build's (ascending columns), the plain step order is derived
@@ -581,25 +658,26 @@ void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scra
     for (int i = 0; i < count; ++i) {
         blqh_hamil* H = hs[i];
         Panels& P = H->pan;
-        if (!P.built || (P.jagged == jagged && (!jagged || P.greedy == greedy))) continue;
-        if (P.jagged && P.greedy) {
+        if (!P.built || (P.jagged == jagged && (!jagged || (P.greedy == greedy && P.rows == rows)))) continue;
+        if (P.jagged && (P.greedy || P.rows > 1)) {
             const int gr = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(H->n, kBlk), 1), 65536);
             hipLaunchKernelGGL(k_panel_rescatter, dim3(gr), dim3(kBlk), 0, st, H->n, H->rowptr.get(), H->col.get(),
                                H->val.get(), P.pbase.get(), P.prow.get(), P.real ? 1 : 0, P.pcol.get(),
                                P.pvr.get(), P.pvc.get());
             BLQH_HIP(hipGetLastError());
             P.jagged = P.greedy = false;
+            P.rows = 1;
         } else if (P.jagged) {
             back.push_back(H);  // plain step order -> row-major by relayout
         }
         if (jagged) todo.push_back(H);
     }
-    if (!back.empty()) relayout_pass(back, false, false, scratch, scratch_bytes);
-    if (!todo.empty()) relayout_pass(todo, jagged, greedy, scratch, scratch_bytes);
+    if (!back.empty()) relayout_pass(back, false, false, 1, scratch, scratch_bytes);
+    if (!todo.empty()) relayout_pass(todo, jagged, greedy, rows, scratch, scratch_bytes);
 }
 
-void set_panel_layout(blqh_hamil* H, bool jagged, void* scratch, size_t scratch_bytes) {
-    set_panel_layouts(&H, 1, jagged, scratch, scratch_bytes, true);
+void set_panel_layout(blqh_hamil* H, bool jagged, void* scratch, size_t scratch_bytes, int rows) {
+    set_panel_layouts(&H, 1, jagged, scratch, scratch_bytes, true, rows);
 }
 
 size_t panel_layout_scratch_bytes(const blqh_hamil* H) {

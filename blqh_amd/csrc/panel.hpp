@@ -15,6 +15,7 @@ struct Panels {
     bool real = false;
     bool jagged = false;     // entry order inside each (64-row slice, panel) run: row-major or step-major
     bool greedy = false;     // step-major with the greedy step order (set_panel_layout), else the plain one
+    int rows = 1;            // step-major: rows per lane (1, or 2 / 4 / 8 for SpMV kind 5)
     int npan = 0;
     DevBuf<int64_t> pbase;   // [npan + 1]
     DevBuf<uint32_t> prow;   // [npan][n + 1]
@@ -154,6 +155,91 @@ __device__ __forceinline__ uint32_t jag_row_sum(const uint16_t* __restrict__ cr,
         any0 = any1;
     }
     return off;  // entries of the whole run (wave-uniform)
+}
+
+// The same over a lane stream that covers R rows one after the other (SpMV
+// kind 5, panel runs with R rows per lane): cum[q] = the lane's entries in
+// rows 0..q of its stream.  The running sum is added to acc[q * 64] (this
+// lane's LDS slot of row q) when the stream passes the end of row q; the row
+// boundaries stay in registers (static indices, selected by compare).
+template <bool REAL, int R>
+__device__ __forceinline__ void jag_rows_sum(const uint16_t* __restrict__ cr, const double* __restrict__ vr,
+                                             const double2* __restrict__ vc, const double2* __restrict__ xp,
+                                             const uint32_t (&cum)[R], double2* acc) {
+    typedef double d2v_t __attribute__((ext_vector_type(2)));
+    const uint32_t len = cum[R - 1];
+    int cur = 0;
+    uint32_t bnd = cum[0];
+    double ar = 0.0, ai = 0.0;
+    uint32_t off = 0;
+    auto issue = [&](uint32_t t, int* c, double2* v, bool& any) {
+        uint64_t m[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) m[u] = __ballot(t + u < len);
+        any = m[0] != 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t q = off + lanes_below(m[u]);
+            off += (uint32_t)__popcll(m[u]);
+            const bool act = t + u < len;
+            c[u] = act ? (int)__builtin_nontemporal_load(&cr[q]) : 0;
+            if (REAL) {
+                v[u] = make_double2(act ? __builtin_nontemporal_load(&vr[q]) : 0.0, 0.0);
+            } else if (act) {
+                const d2v_t w = __builtin_nontemporal_load(reinterpret_cast<const d2v_t*>(&vc[q]));
+                v[u] = make_double2(w.x, w.y);
+            } else {
+                v[u] = make_double2(0.0, 0.0);
+            }
+        }
+    };
+    int c0[4], c1[4];
+    double2 v0[4], v1[4];
+    bool any0, any1;
+    issue(0, c0, v0, any0);
+    for (uint32_t t = 4; any0; t += 4) {
+        issue(t, c1, v1, any1);  // next group in flight
+        double2 xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[u] = xp[c0[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t tt = t - 4 + u;
+            if (tt < len) {
+                while (tt >= bnd && cur < R - 1) {  // past the end of row cur (rows without entries skipped)
+                    double2 a = acc[cur * 64];
+                    a.x += ar;
+                    a.y += ai;
+                    acc[cur * 64] = a;
+                    ar = 0.0;
+                    ai = 0.0;
+                    ++cur;
+#pragma unroll
+                    for (int q = 0; q < R; ++q)
+                        if (q == cur) bnd = cum[q];
+                }
+                if (REAL) {
+                    ar += v0[u].x * xv[u].x;
+                    ai += v0[u].x * xv[u].y;
+                } else {
+                    ar += v0[u].x * xv[u].x - v0[u].y * xv[u].y;
+                    ai += v0[u].x * xv[u].y + v0[u].y * xv[u].x;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c0[u] = c1[u];
+            v0[u] = v1[u];
+        }
+        any0 = any1;
+    }
+    if (len) {
+        double2 a = acc[cur * 64];
+        a.x += ar;
+        a.y += ai;
+        acc[cur * 64] = a;
+    }
 }
 
 inline PanelView view(const Panels& P, int64_t n) {

@@ -100,6 +100,12 @@ int blqh_spmv_device(const blqh_hamil* h, const void* x_dev, void* y_dev); /* de
  * use; `lanes`, `threads`, `wgs_per_cu` and `rows` are set by the layout and
  * reported by get; BLQH_EINVAL when a row has more than 255 entries in one
  * tile or the matrix has more than 4.2M rows).
+ * kind 5: step-major panels with `lanes` = R rows per lane (2, 4 or 8): lane
+ * l sums rows l, l+64, .., l+64(R-1) of a 64R-row block in one stream per
+ * panel, which evens out the lanes' work per step; `threads` 512 or 1024,
+ * `wgs_per_cu` 1-4, `rows` = blocks per wavefront and round with
+ * R * rows <= 8.  Kind 3 orders each step greedily (lanes that read the same
+ * x cache line share a step), kind 5 keeps each row's entries in order.
  * get: kind = -1 until
  * the autotuner has run (first SpMV or solve).  set: pins a variant (returns
  * BLQH_EINVAL for an unsupported one); kind = -1 re-arms the autotuner. */
