@@ -25,6 +25,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <cstdio>
 #include <complex>
 #include <mutex>
 #include <thread>
@@ -174,6 +176,9 @@ struct BuildArgs {
     int32_t* ovf_count;
     int32_t* ovf_rows;
     int32_t* err;             // fill/count mismatch counter
+    int32_t* stage_col;       // single pass: row i's sorted entries at i * CAP
+    double2* stage_val;
+    uint8_t* stage_ok;        // single pass: 1 when row i was staged (0: overflow row)
 };
 
 template <int CAP>
@@ -202,7 +207,7 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
     extern __shared__ double smem[];
     __shared__ int keys[CAP];
     __shared__ double vre[CAP], vim[CAP];
-    __shared__ unsigned long long sbuf[MODE == 1 ? CAP : 1];
+    __shared__ unsigned long long sbuf[MODE >= 1 ? CAP : 1];
     __shared__ int occ[64];
     __shared__ int s_row, s_ovf;
 
@@ -269,10 +274,11 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
         if (ovf) s_ovf = 1;  // benign race: every writer stores 1
         __syncthreads();
         if (s_ovf) {
-            if (MODE == 0 && lane == 0) {
+            if ((MODE == 0 || MODE == 2) && lane == 0) {
                 A.nnz_out[i] = 0;
                 const int k = atomicAdd(A.ovf_count, 1);
                 A.ovf_rows[k] = (int32_t)i;
+                if (MODE == 2) A.stage_ok[i] = 0;
             }
             if (MODE == 1 && lane == 0) atomicAdd(A.err, 1);
             __syncthreads();
@@ -317,20 +323,55 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
                     }
                     __syncthreads();
                 }
-            const int64_t base = A.rowptr[i];
-            const int64_t len = A.rowptr[i + 1] - base;
-            if (len != E) {
-                if (lane == 0) atomicAdd(A.err, 1);
-            } else {
+            if (MODE == 2) {
+                // single pass: the sorted row goes to its staging slot (E <= CAP)
+                const int64_t base = i * (int64_t)CAP;
                 for (int x = lane; x < E; x += 64) {
                     const unsigned long long u = sbuf[x];
                     const int slot = (int)(u & 0xffffffffu);
-                    A.col[base + x] = (int32_t)(u >> 32);
-                    A.val[base + x] = make_double2(vre[slot], vim[slot]);
+                    A.stage_col[base + x] = (int32_t)(u >> 32);
+                    A.stage_val[base + x] = make_double2(vre[slot], vim[slot]);
+                }
+                if (lane == 0) {
+                    A.nnz_out[i] = E;
+                    A.stage_ok[i] = 1;
+                }
+            } else {
+                const int64_t base = A.rowptr[i];
+                const int64_t len = A.rowptr[i + 1] - base;
+                if (len != E) {
+                    if (lane == 0) atomicAdd(A.err, 1);
+                } else {
+                    for (int x = lane; x < E; x += 64) {
+                        const unsigned long long u = sbuf[x];
+                        const int slot = (int)(u & 0xffffffffu);
+                        A.col[base + x] = (int32_t)(u >> 32);
+                        A.val[base + x] = make_double2(vre[slot], vim[slot]);
+                    }
                 }
             }
         }
         __syncthreads();
+    }
+}
+
+// Single-pass build: copy every staged row to its CSR position (a wavefront
+// per row, entries already sorted); overflow rows are filled by the
+// large-hash pass.
+template <int CAP>
+__global__ void __launch_bounds__(256) k_stage_copy(int64_t n, const int32_t* __restrict__ scol,
+                                                    const double2* __restrict__ sval, const uint8_t* __restrict__ ok,
+                                                    const int64_t* __restrict__ rowptr, int32_t* __restrict__ col,
+                                                    double2* __restrict__ val) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (int64_t)gridDim.x * 4;
+    for (int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += nw) {
+        if (!ok[i]) continue;
+        const int64_t b = rowptr[i], e = rowptr[i + 1], src = i * (int64_t)CAP;
+        for (int64_t x = lane; x < e - b; x += 64) {
+            col[b + x] = scol[src + x];
+            val[b + x] = sval[src + x];
+        }
     }
 }
 
@@ -367,10 +408,22 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil
     BLQH_REQUIRE(p.lx > 0 && p.ly > 0, BLQH_EINVAL, "lx and ly must be positive");
     BLQH_REQUIRE(s.nel <= 64 && s.nel * (s.nel - 1) / 2 < 65536, BLQH_EINVAL, "too many electrons");
     auto t0 = std::chrono::steady_clock::now();
+    // BLQH_BUILD_TIMING=1: per-phase wall times of every build on stderr
+    static const bool timing = std::getenv("BLQH_BUILD_TIMING") != nullptr;
+    double tph[6] = {0, 0, 0, 0, 0, 0};
+    auto tmark = t0;
+    auto phase = [&](int i) {
+        if (!timing) return;
+        BLQH_HIP(hipStreamSynchronize(st));
+        const auto now = std::chrono::steady_clock::now();
+        tph[i] = std::chrono::duration<double, std::milli>(now - tmark).count();
+        tmark = now;
+    };
     H->basis = B;
     H->par = p;
     H->n = B->n;
     HostTables T = host_tables(s, p);
+    phase(0);
     DevBuf<double> dC;
     DevBuf<double2> dFT, dTW;
     DevBuf<HopTerm> dHop;
@@ -396,8 +449,28 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil
     A.ovf_rows = ovf_rows.get();
     A.nnz_out = nnz.get();
     const size_t smem = table_smem(B->geo);
-    // count
-    launch_build<kSmall, 0>(A, B->n, smem, st, ncu);
+    // One pass when the staging slots fit comfortably (each row hashed once,
+    // sorted into a slot of kSmall entries, compacted after the scan); else
+    // count, scan, fill.
+    size_t free_b = 0, total_b = 0;
+    BLQH_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t stage_bytes = (size_t)B->n * kSmall * (sizeof(int32_t) + sizeof(double2)) + (size_t)B->n;
+    const bool single = stage_bytes <= free_b / 4 && std::getenv("BLQH_BUILD_TWO_PASS") == nullptr;
+    DevBuf<int32_t> scol;
+    DevBuf<double2> sval;
+    DevBuf<uint8_t> sok;
+    if (single) {
+        scol.alloc((size_t)B->n * kSmall);
+        sval.alloc((size_t)B->n * kSmall);
+        sok.alloc(std::max<int64_t>(B->n, 1));
+        A.stage_col = scol.get();
+        A.stage_val = sval.get();
+        A.stage_ok = sok.get();
+        launch_build<kSmall, 2>(A, B->n, smem, st, ncu);
+    } else {
+        // count
+        launch_build<kSmall, 0>(A, B->n, smem, st, ncu);
+    }
     int32_t h_ctr[4];
     BLQH_HIP(hipMemcpyAsync(h_ctr, ctr.get(), sizeof(h_ctr), hipMemcpyDeviceToHost, st));
     BLQH_HIP(hipStreamSynchronize(st));
@@ -418,17 +491,26 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil
         BLQH_HIP(hipStreamSynchronize(st));
         BLQH_REQUIRE(h_ctr[1] == 0, BLQH_ERANGE, "a Hamiltonian row has more than 4096 distinct columns");
     }
+    phase(1);
     H->rowptr.alloc(B->n + 1);
     exclusive_scan(nnz.get(), H->rowptr.get(), B->n, st);
     BLQH_HIP(hipMemcpy(&H->nnz, H->rowptr.get() + B->n, 8, hipMemcpyDeviceToHost));
     H->col.alloc(std::max<int64_t>(H->nnz, 1));
     H->val.alloc(std::max<int64_t>(H->nnz, 1));
+    phase(2);
     // fill
     A.rowptr = Ao.rowptr = H->rowptr.get();
     A.col = Ao.col = H->col.get();
     A.val = Ao.val = H->val.get();
     BLQH_HIP(hipMemsetAsync(ctr.get(), 0, 4 * sizeof(int32_t), st));
-    launch_build<kSmall, 1>(A, B->n, smem, st, ncu);  // overflow rows are skipped (counted in err)
+    if (single) {
+        const int gs = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(B->n, 4), 1), 65536);
+        hipLaunchKernelGGL(k_stage_copy<kSmall>, dim3(gs), dim3(256), 0, st, B->n, scol.get(), sval.get(), sok.get(),
+                           H->rowptr.get(), H->col.get(), H->val.get());
+        BLQH_HIP(hipGetLastError());
+    } else {
+        launch_build<kSmall, 1>(A, B->n, smem, st, ncu);  // overflow rows are skipped (counted in err)
+    }
     if (novf > 0) {
         BLQH_HIP(hipMemsetAsync(ctr2.get(), 0, 4 * sizeof(int32_t), st));
         launch_build<kLarge, 1>(Ao, novf, smem, st, ncu);
@@ -438,7 +520,14 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil
     if (novf > 0) BLQH_HIP(hipMemcpyAsync(e2, ctr2.get(), sizeof(e2), hipMemcpyDeviceToHost, st));
     BLQH_HIP(hipStreamSynchronize(st));
     // the small-hash fill pass reports exactly the overflow rows as errors
-    BLQH_REQUIRE(e1[2] == novf && e2[2] == 0, BLQH_EINTERNAL, "count and fill passes disagree");
+    BLQH_REQUIRE((single ? e1[2] == 0 : e1[2] == novf) && e2[2] == 0, BLQH_EINTERNAL,
+                 "count and fill passes disagree");
+    phase(3);
     build_panels(H);
-    H->seconds_build =std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    phase(4);
+    H->seconds_build = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (timing)
+        std::fprintf(stderr, "blqh build n=%lld nnz=%lld: tables %.2f count %.2f scan+alloc %.2f fill %.2f panels %.2f "
+                     "total %.2f ms\n", (long long)B->n, (long long)H->nnz, tph[0], tph[1], tph[2], tph[3], tph[4],
+                     H->seconds_build * 1e3);
 }
