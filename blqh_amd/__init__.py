@@ -77,6 +77,8 @@ _SIGS = {
     "blqh_basis_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "blqh_basis_destroy": (None, [C.c_void_p]),
     "blqh_hamil_build": (C.c_int, [C.c_void_p, C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "blqh_hamil_build_terms": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int32, C.POINTER(C.c_void_p)]),
+    "blqh_hamil_combine": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_void_p)]),
     "blqh_hamil_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "blqh_hamil_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "blqh_hamil_destroy": (None, [C.c_void_p]),
@@ -244,17 +246,35 @@ class Basis:
 class Hamiltonian:
     """Hamiltonian of one sector in CSR on the GPU (lhamil::set_hamil)."""
 
-    def __init__(self, basis, par):
+    def __init__(self, basis, par, terms=3, _handle=None):
+        """terms 3: the Hamiltonian; 1: without the interlayer tunnelling; 2: the
+        tunnelling alone at Delta_SAS = 1 (for combine)."""
         self.basis = basis
         self.par = par
         self._lib = lib()
-        h = C.c_void_p()
-        _check(self._lib.blqh_hamil_build(basis.h, C.byref(par), C.byref(h)))
+        if _handle is None:
+            h = C.c_void_p()
+            if terms == 3:
+                _check(self._lib.blqh_hamil_build(basis.h, C.byref(par), C.byref(h)))
+            else:
+                _check(self._lib.blqh_hamil_build_terms(basis.h, C.byref(par), terms, C.byref(h)))
+        else:
+            h = _handle
         self.h = h
         n = C.c_int64()
         nnz = C.c_int64()
         _check(lib().blqh_hamil_info(self.h, C.byref(n), C.byref(nnz)))
         self.n, self.nnz = n.value, nnz.value
+
+    @classmethod
+    def combine(cls, base, tun, delta_sas):
+        """H(Delta_SAS) = base + Delta_SAS * tun (base without tunnelling, tun a
+        terms=2 build of the same basis and parameters)."""
+        h = C.c_void_p()
+        _check(lib().blqh_hamil_combine(base.h, tun.h, delta_sas, C.byref(h)))
+        par = Params.from_buffer_copy(base.par)
+        par.delta_sas = delta_sas
+        return cls(base.basis, par, _handle=h)
 
     def export(self):
         rowptr = np.empty(self.n + 1, np.int64)

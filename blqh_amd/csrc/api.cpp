@@ -10,7 +10,8 @@
 #include "engine.hpp"
 
 void blqh_basis_build_impl(const blqh_sector& s, blqh_basis* B);
-void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil* H);
+void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil* H, int terms);
+void blqh_hamil_combine_impl(const blqh_hamil* base, const blqh_hamil* tun, double dsas, blqh_hamil* H);
 void blqh_spmv_device_impl(blqh_hamil* H, const void* x, void* y);
 void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_result& r, double* psi, double* alpha,
                        double* beta);
@@ -148,7 +149,37 @@ int blqh_hamil_build(const blqh_basis* b, const blqh_params* p, blqh_hamil** out
     return guarded([&] {
         auto* H = new blqh_hamil;
         try {
-            blqh_hamil_build_impl(b, *p, H);
+            blqh_hamil_build_impl(b, *p, H, 3);
+        } catch (...) {
+            delete H;
+            throw;
+        }
+        *out = H;
+    });
+}
+
+int blqh_hamil_build_terms(const blqh_basis* b, const blqh_params* p, int32_t terms, blqh_hamil** out) {
+    if (!b || !p || !out) return fail(BLQH_EINVAL, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        auto* H = new blqh_hamil;
+        try {
+            blqh_hamil_build_impl(b, *p, H, terms);
+        } catch (...) {
+            delete H;
+            throw;
+        }
+        *out = H;
+    });
+}
+
+int blqh_hamil_combine(const blqh_hamil* base, const blqh_hamil* tun, double delta_sas, blqh_hamil** out) {
+    if (!base || !tun || !out) return fail(BLQH_EINVAL, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        auto* H = new blqh_hamil;
+        try {
+            blqh_hamil_combine_impl(base, tun, delta_sas, H);
         } catch (...) {
             delete H;
             throw;

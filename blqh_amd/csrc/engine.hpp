@@ -40,6 +40,7 @@ struct blqh_hamil {
     int spmv_wgs_per_cu = 0;         // persistent workgroups per CU (panel kernel)
     int spmv_rows = 0;               // rows per lane group in flight (panel kernel)
     double seconds_build = 0;
+    int terms = 3;                   // 1: Coulomb + diagonal, 2: interlayer tunnelling (unit Delta_SAS), 3: both
 };
 
 namespace blqh {

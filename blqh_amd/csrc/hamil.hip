@@ -179,6 +179,7 @@ struct BuildArgs {
     int32_t* stage_col;       // single pass: row i's sorted entries at i * CAP
     double2* stage_val;
     uint8_t* stage_ok;        // single pass: 1 when row i was staged (0: overflow row)
+    int terms;                // bit 0: pair terms and diagonal; tunnelling hops come with the hop list
 };
 
 template <int CAP>
@@ -248,7 +249,7 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
             const uint64_t L = translate(g, L0, ql, signl);
             if (lane < nbits && ((L >> lane) & 1ULL)) occ[popc(L & lowbits(lane))] = lane;
             __syncthreads();
-            const int ncand = npairs * T;
+            const int ncand = (A.terms & 1) ? npairs * T : 0;
             for (int cnd = lane; cnd < ncand; cnd += 64) {
                 const int pr = cnd / T;
                 const int t = cnd - pr * T - (nphi - 1);
@@ -267,7 +268,7 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
             __syncthreads();
         }
         // diagonal
-        if (lane == 0) {
+        if (lane == 0 && (A.terms & 1)) {
             const double dg = diag_term(g, ctx.spinless != 0, L0, A.Ecn, A.d, A.dv, A.dz);
             if (!hash_add<CAP>(keys, vre, vim, (int)i, dg, 0.0)) ovf = true;
         }
@@ -400,8 +401,16 @@ void launch_build(const BuildArgs& A, int64_t rows, size_t smem, hipStream_t st,
 
 }  // namespace
 
-void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil* H) {
+// terms = 3: the full Hamiltonian; 1: without the interlayer tunnelling;
+// 2: the tunnelling alone at Delta_SAS = 1 (p.delta_sas is ignored), the
+// matrix T with H(Delta_SAS) = H(0) + Delta_SAS T (blqh_hamil_combine).
+void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p_in, blqh_hamil* H, int terms) {
     using namespace blqh;
+    BLQH_REQUIRE(terms >= 1 && terms <= 3, BLQH_EINVAL, "terms must be 1, 2 or 3");
+    blqh_params p = p_in;
+    if (terms == 1) p.delta_sas = 0.0;
+    if (terms == 2) p.delta_sas = 1.0;
+    H->terms = terms;
     hipStream_t st = stream();
     const int ncu = runtime().num_cus;
     const blqh_sector& s = B->sec;
@@ -438,6 +447,7 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil
     A.hops = upload(dHop, T.hops, st);
     A.nhop = (int)T.hops.size();
     A.Ecn = T.Ec * s.nel;
+    A.terms = terms;
     A.d = p.d;
     A.dv = p.delta_v;
     A.dz = p.delta_z;
@@ -530,4 +540,102 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p, blqh_hamil
         std::fprintf(stderr, "blqh build n=%lld nnz=%lld: tables %.2f count %.2f scan+alloc %.2f fill %.2f panels %.2f "
                      "total %.2f ms\n", (long long)B->n, (long long)H->nnz, tph[0], tph[1], tph[2], tph[3], tph[4],
                      H->seconds_build * 1e3);
+}
+
+// ---- H(d, Delta_SAS) = H(d, 0) + Delta_SAS T -------------------------------------
+// The interlayer tunnelling moves one electron between the layers, so T's
+// entries never share a (row, column) with the layer-conserving Coulomb and
+// diagonal terms: the sum is a merge of two sorted rows, with the build's
+// drop of entries of modulus <= 1e-10 applied to the scaled T entries.
+namespace {
+
+__global__ void k_combine_count(int64_t n, const int64_t* __restrict__ brp, const int64_t* __restrict__ trp,
+                                const double2* __restrict__ tval, double dsas, int32_t* __restrict__ cnt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int c = (int)(brp[i + 1] - brp[i]);
+        for (int64_t k = trp[i]; k < trp[i + 1]; ++k) c += hypot(dsas * tval[k].x, dsas * tval[k].y) > 1e-10;
+        cnt[i] = c;
+    }
+}
+
+__global__ void k_combine_fill(int64_t n, const int64_t* __restrict__ brp, const int32_t* __restrict__ bcol,
+                               const double2* __restrict__ bval, const int64_t* __restrict__ trp,
+                               const int32_t* __restrict__ tcol, const double2* __restrict__ tval, double dsas,
+                               const int64_t* __restrict__ rp, int32_t* __restrict__ col, double2* __restrict__ val,
+                               int32_t* __restrict__ err) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t a = brp[i], ae = brp[i + 1], b = trp[i], be = trp[i + 1], o = rp[i];
+        const int64_t oe = rp[i + 1];
+        while (a < ae || b < be) {
+            if (b < be) {
+                const double2 t = make_double2(dsas * tval[b].x, dsas * tval[b].y);
+                if (!(hypot(t.x, t.y) > 1e-10)) {
+                    ++b;
+                    continue;
+                }
+                if (a >= ae || tcol[b] < bcol[a]) {
+                    if (o < oe) {
+                        col[o] = tcol[b];
+                        val[o] = t;
+                    }
+                    ++o;
+                    ++b;
+                    continue;
+                }
+                if (tcol[b] == bcol[a]) atomicAdd(err, 1);  // shared entry: not a pure tunnelling term
+            }
+            if (o < oe) {
+                col[o] = bcol[a];
+                val[o] = bval[a];
+            }
+            ++o;
+            ++a;
+        }
+        if (o != oe) atomicAdd(err, 1);
+    }
+}
+
+}  // namespace
+
+void blqh_hamil_combine_impl(const blqh_hamil* base, const blqh_hamil* tun, double dsas, blqh_hamil* H) {
+    using namespace blqh;
+    BLQH_REQUIRE(base->terms == 1 || (base->terms == 3 && base->par.delta_sas == 0.0), BLQH_EINVAL,
+                 "the base must be built without tunnelling (Delta_SAS = 0)");
+    BLQH_REQUIRE(tun->terms == 2, BLQH_EINVAL, "the second matrix must be a tunnelling-only build");
+    BLQH_REQUIRE(base->basis == tun->basis, BLQH_EINVAL, "the two matrices belong to different bases");
+    const blqh_params &a = base->par, &b = tun->par;
+    // T depends on d only through the theta_B phases
+    BLQH_REQUIRE(a.lx == b.lx && a.ly == b.ly && a.nLL == b.nLL && (a.d == b.d || a.theta_b == 0.0) &&
+                     a.theta_b == b.theta_b && a.theta_x == b.theta_x && a.theta_y == b.theta_y,
+                 BLQH_EINVAL, "the two matrices were built for different parameters");
+    BLQH_REQUIRE(std::isfinite(dsas) && dsas >= 0.0, BLQH_EINVAL, "Delta_SAS must be finite and >= 0");
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = stream();
+    const int64_t n = base->n;
+    H->basis = base->basis;
+    H->par = base->par;
+    H->par.delta_sas = dsas;
+    H->n = n;
+    H->terms = 3;
+    DevBuf<int32_t> cnt(std::max<int64_t>(n, 1)), err(1);
+    BLQH_HIP(hipMemsetAsync(err.get(), 0, sizeof(int32_t), st));
+    const int g = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 65536);
+    hipLaunchKernelGGL(k_combine_count, dim3(g), dim3(256), 0, st, n, base->rowptr.get(), tun->rowptr.get(),
+                       tun->val.get(), dsas, cnt.get());
+    H->rowptr.alloc(n + 1);
+    exclusive_scan(cnt.get(), H->rowptr.get(), n, st);
+    BLQH_HIP(hipMemcpyAsync(&H->nnz, H->rowptr.get() + n, 8, hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    H->col.alloc(std::max<int64_t>(H->nnz, 1));
+    H->val.alloc(std::max<int64_t>(H->nnz, 1));
+    hipLaunchKernelGGL(k_combine_fill, dim3(g), dim3(256), 0, st, n, base->rowptr.get(), base->col.get(),
+                       base->val.get(), tun->rowptr.get(), tun->col.get(), tun->val.get(), dsas, H->rowptr.get(),
+                       H->col.get(), H->val.get(), err.get());
+    BLQH_HIP(hipGetLastError());
+    int32_t herr = 0;
+    BLQH_HIP(hipMemcpyAsync(&herr, err.get(), sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    BLQH_REQUIRE(herr == 0, BLQH_EINTERNAL, "tunnelling and Coulomb entries overlap");
+    build_panels(H);
+    H->seconds_build = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }

@@ -74,6 +74,24 @@ void blqh_basis_destroy(blqh_basis* b);
 /* ---- Hamiltonian build into CSR: lhamil::set_hamil / peer_set_hamil
  *      (BLQH_without_spin/lanczos_hamiltonian.cpp:87-385; matrix.h:71-88) ---- */
 int blqh_hamil_build(const blqh_basis* b, const blqh_params* p, blqh_hamil** out);
+/* Parts of the same Hamiltonian for parameter sweeps (no counterpart in the
+ * reference, whose sweeps rebuild H at every point, BLQH_without_spin/main.cpp:76-165).
+ * terms = 3: as blqh_hamil_build; 1: without the interlayer tunnelling
+ * (Delta_SAS taken as 0); 2: the tunnelling alone at Delta_SAS = 1 (the
+ * Delta_SAS of p is ignored; the theta_B phases use p's d).
+ * blqh_hamil_combine(base, tun, Delta_SAS): H(Delta_SAS) = base + Delta_SAS tun
+ * for a base without tunnelling (terms 1, or terms 3 at Delta_SAS = 0) and a
+ * tunnelling-only build of the same basis and parameters (theta_B, theta_x,
+ * theta_y, lx, ly, nLL, and d when theta_B != 0: the tunnelling depends on d
+ * only through the theta_B phases): the tunnelling moves one electron between the layers,
+ * so its entries never meet the layer-conserving ones and the result is a
+ * merge of sorted rows with the build's 1e-10 drop.  It equals
+ * blqh_hamil_build with that Delta_SAS: same rows, columns and Coulomb and
+ * diagonal values; tunnelling values to rounding (Delta_SAS * sum instead of
+ * the sum of Delta_SAS * terms).  The result owns its arrays; base and tun may
+ * be destroyed afterwards. */
+int blqh_hamil_build_terms(const blqh_basis* b, const blqh_params* p, int32_t terms, blqh_hamil** out);
+int blqh_hamil_combine(const blqh_hamil* base, const blqh_hamil* tun, double delta_sas, blqh_hamil** out);
 int blqh_hamil_info(const blqh_hamil* h, int64_t* nrows, int64_t* nnz);
 /* rowptr[nrows+1], col[nnz], val[2*nnz]; any pointer may be NULL. */
 int blqh_hamil_export(const blqh_hamil* h, int64_t* rowptr, int32_t* col, double* val);
