@@ -51,7 +51,9 @@ typedef struct blqh_params {
     double delta_v;     /* bias (spinless: applied when > 0) */
     double delta_z;     /* Zeeman energy (spinful) */
     double theta_b;     /* in-plane field phase (spinful) */
-    double theta_x;     /* twist entering the interlayer table */
+    double theta_x;     /* twist entering the interlayer table; as in the reference, which reads
+                           that table at |s| (lanczos_hamiltonian.cpp:263), H is Hermitian
+                           with theta_x != 0 only in K = 0 sectors */
     double theta_y;     /* twist on interlayer pair hops */
 } blqh_params;
 
