@@ -17,6 +17,7 @@
 // (lanczos_hamiltonian.cpp:624-640), normalised; E_rq = Re<psi|H psi>.
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <cstdio>
@@ -1199,18 +1200,59 @@ void blqh_spmv_device_impl(blqh_hamil* H, const void* x, void* y) {
 }
 
 namespace {
-// Start vector of pass 1 and pass 3: the caller's (normalised here) or the seeded mt19937 one.
-void make_start(int64_t n, uint32_t seed, const double* start, std::vector<double2>& v0) {
-    if (start) {
-        v0.assign((const double2*)start, (const double2*)start + n);
-        double s = 0;
-        for (auto& z : v0) s += z.x * z.x + z.y * z.y;
-        BLQH_REQUIRE(s > 0 && std::isfinite(s), BLQH_EINVAL, "start vector is zero or not finite");
-        s = std::sqrt(s);
-        for (auto& z : v0) { z.x /= s; z.y /= s; }
-    } else {
-        blqh::start_vector(n, seed, v0);
+__global__ void k_div(int64_t n, double2* __restrict__ x, double s) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        double2 v = x[i];
+        v.x /= s;
+        v.y /= s;
+        x[i] = v;
     }
+}
+
+// Pinned host staging for start vectors, reused by every solve (grow-only,
+// held until the process ends).
+std::mutex g_stage_mu;
+double2* g_stage = nullptr;
+int64_t g_stage_n = 0;
+
+// Start vector of pass 1 and pass 3 into device memory dst: the caller's, or
+// the seeded mt19937 one of the reference (draw 2i -> imaginary, 2i+1 -> real
+// part of entry i).  Drawn (or copied) and summed on the host in pinned
+// memory, in the order of start_vector, then divided by the norm on the
+// device: the same IEEE divisions as the host version, without a second
+// host pass, fresh pages or a pageable copy per solve.
+void device_start(int64_t n, uint32_t seed, const double* start, double2* dst, hipStream_t st) {
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    if (g_stage_n < n) {
+        if (g_stage) (void)hipHostFree(g_stage);
+        g_stage = nullptr;
+        g_stage_n = 0;
+        BLQH_HIP(hipHostMalloc((void**)&g_stage, (size_t)std::max<int64_t>(n, 1) * sizeof(double2),
+                               hipHostMallocDefault));
+        g_stage_n = n;
+    }
+    double s = 0;
+    if (start) {
+        const double2* z = (const double2*)start;
+        for (int64_t i = 0; i < n; ++i) {
+            g_stage[i] = z[i];
+            s += z[i].x * z[i].x + z[i].y * z[i].y;
+        }
+        BLQH_REQUIRE(s > 0 && std::isfinite(s), BLQH_EINVAL, "start vector is zero or not finite");
+    } else {
+        std::mt19937 rng(seed);
+        for (int64_t i = 0; i < n; ++i) {
+            const double im = rng() * 1.0 / rng.max() - 0.5;
+            const double re = rng() * 1.0 / rng.max() - 0.5;
+            g_stage[i] = make_double2(re, im);
+            s += re * re + im * im;
+        }
+    }
+    s = std::sqrt(s);
+    BLQH_HIP(hipMemcpyAsync(dst, g_stage, n * sizeof(double2), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_div, dim3(vec_grid(n)), dim3(kThreads), 0, st, n, dst, s);
+    BLQH_HIP(hipGetLastError());
+    BLQH_HIP(hipStreamSynchronize(st));  // the staging buffer is free again
 }
 }  // namespace
 
@@ -1244,10 +1286,10 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
     BLQH_HIP(hipMemcpyAsync(sc.beta, &one, 8, hipMemcpyHostToDevice, st));
     BLQH_HIP(hipMemcpyAsync(sc.invb, &one, 8, hipMemcpyHostToDevice, st));
 
-    std::vector<double2> v0;
-    make_start(n, o.seed, o.start, v0);
     auto slot = [&](int j) -> double2* { return Wb.get() + (size_t)(store ? j : j % 3) * n; };
-    BLQH_HIP(hipMemcpyAsync(slot(0), v0.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
+    DevBuf<double2> v0(store ? 0 : n);  // kept for the pass-3 replay
+    device_start(n, o.seed, o.start, slot(0), st);
+    if (!store) BLQH_HIP(hipMemcpyAsync(v0.get(), slot(0), n * sizeof(double2), hipMemcpyDeviceToDevice, st));
 
     const int gv = vec_grid(n);
     Profile& prof = profile();
@@ -1350,7 +1392,7 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
         // replay with the same kernels: alpha/beta are regenerated bit for bit
         BLQH_HIP(hipMemcpyAsync(coef.get(), c.data(), m * 8, hipMemcpyHostToDevice, st));
         BLQH_HIP(hipMemsetAsync(psi.get(), 0, n * sizeof(double2), st));
-        BLQH_HIP(hipMemcpyAsync(slot(0), v0.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
+        BLQH_HIP(hipMemcpyAsync(slot(0), v0.get(), n * sizeof(double2), hipMemcpyDeviceToDevice, st));
         for (int j = 0; j + 1 < m; ++j) step(j, psi.get(), coef.get());
         // last vector's weight needs invb[m-1], which the replay just recomputed
         std::vector<double> hb2(m);
@@ -1417,10 +1459,9 @@ void blqh_lanczos_reconstruct_impl(blqh_hamil* H, int m, uint32_t seed, const do
     const double one = 1.0;
     BLQH_HIP(hipMemcpyAsync(sc.beta, &one, 8, hipMemcpyHostToDevice, st));
     BLQH_HIP(hipMemcpyAsync(sc.invb, &one, 8, hipMemcpyHostToDevice, st));
-    std::vector<double2> v0;
-    make_start(n, seed, start, v0);
+
     auto slot = [&](int j) -> double2* { return Wb.get() + (size_t)(j % 3) * n; };
-    BLQH_HIP(hipMemcpyAsync(slot(0), v0.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
+    device_start(n, seed, start, slot(0), st);
     BLQH_HIP(hipMemcpyAsync(coef.get(), coefs, (size_t)ns * m * 8, hipMemcpyHostToDevice, st));
     BLQH_HIP(hipMemsetAsync(psi.get(), 0, (size_t)ns * n * sizeof(double2), st));
     const int gv = vec_grid(n);
