@@ -1026,6 +1026,7 @@ void tune(blqh_hamil* H) {
     BLQH_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     Plan bp = cands.front();
+    std::vector<std::pair<Plan, float>> timed;
     const int reps = H->nnz > 50000000 ? 3 : 10;
     const bool log = std::getenv("BLQH_TUNE_LOG") != nullptr;  // print the timing of every variant
     for (const Plan& pl : cands) {
@@ -1039,8 +1040,19 @@ void tune(blqh_hamil* H) {
         BLQH_HIP(hipEventElapsedTime(&ms, e0, e1));
         if (log) std::fprintf(stderr, "blqh spmv tune: kind %d lanes %d threads %d wgs/cu %d rows %d: %.3f ms\n",
                               pl.kind, pl.lpr, pl.threads, pl.wpc, pl.rows, ms / reps);
+        timed.push_back({pl, ms});
         if (ms < best) { best = ms; bp = pl; }
     }
+    // Near-ties (within 2 %) of the winning panel kind go to the largest
+    // workgroup, then the fewest workgroups per CU: the per-panel barrier keeps
+    // a whole workgroup on one panel, so bigger workgroups hold a smaller x
+    // working set per CU (config 2: 1024 x 1 moves 13.5 GB per launch where
+    // 512 x 2 moves 23 GB, in the same time).
+    if (bp.kind >= 2)
+        for (const auto& c : timed)
+            if (c.first.kind == bp.kind && c.second <= best * 1.02f &&
+                (c.first.threads > bp.threads || (c.first.threads == bp.threads && c.first.wpc < bp.wpc)))
+                bp = c.first;
     BLQH_HIP(hipGetLastError());
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
