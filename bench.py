@@ -180,6 +180,11 @@ def run_sweep(args):
     solved = solves = 0
     t_solve = t_build = 0.0
     iters, e_by_point = [], {}
+    # continuation starts: every sector starts from its ground state at the
+    # previous point (conj(psi): psi = sum c_l conj(v_l) is the eigenvector of
+    # conj(H)); checked against the cold batched solve of the same point
+    cont_psi = None
+    cont = {"solved": 0, "seconds": 0.0, "iters": [], "max_rel_dE0": 0.0, "mismatch": 0}
     for k, (d, ds) in enumerate(order):
         hs, tb = build(d, ds)
         warm = k < nwarm
@@ -198,6 +203,25 @@ def run_sweep(args):
         log(f"{'warmup' if warm else 'point'} d={d} DSAS={ds}: build {tb:.2f} s, batched solve {dt:.3f} s, "
             f"{ok}/{len(res)} converged, iterations {min(r['iterations'] for r in res)}.."
             f"{max(r['iterations'] for r in res)}")
+        if args.continuation:
+            t = time.perf_counter()
+            starts = [np.conj(p) for p in cont_psi] if cont_psi is not None else None
+            wres = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=args.check_every,
+                                 starts=starts, want_psi=True, start_mix=args.start_mix if starts else 0.0)
+            dtw = time.perf_counter() - t
+            cont_psi = [r["psi"] for r in wres]
+            wok = [r["converged"] == 1 and r["dE"] < 1e-10 for r in wres]
+            rel = [abs(a["E0"] - b["E0"]) / abs(b["E0"]) for a, b in zip(wres, res)]
+            bad = sum(int(x > 1e-10) for x, w in zip(rel, wok) if w)
+            log(f"  continuation: batched solve {dtw:.3f} s, {sum(wok)}/{len(wres)} converged, iterations "
+                f"{min(r['iterations'] for r in wres)}..{max(r['iterations'] for r in wres)}, "
+                f"max |E0 - cold E0|/|E0| = {max(rel):.1e}")
+            if not warm and starts is not None:
+                cont["solved"] += sum(int(w and x <= 1e-10) for w, x in zip(wok, rel))
+                cont["seconds"] += dtw
+                cont["iters"] += [r["iterations"] for r in wres]
+                cont["max_rel_dE0"] = max(cont["max_rel_dE0"], max(rel))
+                cont["mismatch"] += bad
         if warm and plain is None:
             # plain single-problem solves of the same Hamiltonians (each autotuned by a first solve)
             for i, H in enumerate(hs):
@@ -258,6 +282,12 @@ def run_sweep(args):
             "batched_over_plain": value / plain["rate"] if plain and plain["rate"] else None,
             "plain": plain,
             "end_to_end_rate_incl_build": solved / (t_solve + t_build) if t_solve + t_build > 0 else None,
+            "continuation_rate": cont["solved"] / cont["seconds"] if cont["seconds"] > 0 else None,
+            "continuation_iterations_mean": float(np.mean(cont["iters"])) if cont["iters"] else None,
+            "continuation_max_rel_dE0_vs_cold": cont["max_rel_dE0"] if cont["iters"] else None,
+            "continuation_mismatches": cont["mismatch"] if cont["iters"] else None,
+            "continuation_vs_plain": (cont["solved"] / cont["seconds"]) / plain["rate"]
+            if cont["seconds"] > 0 and plain else None,
             "build_s_per_point": t_build / max(1, args.steps),  # build + teardown of the point's Hamiltonians
             "build_scheme": "H(d,0) once per d (every third point), H(d,DSAS) = H(d,0) + DSAS*T by merge, "
                             "T once per sector (timed in the first DSAS>0 point)",
@@ -430,6 +460,10 @@ def main():
     ap.add_argument("--check-every", type=int, default=5)
     ap.add_argument("--lam", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--start-mix", type=float, default=0.1,
+                    help="weight of the seeded random unit vector added to warm starts (guards level crossings)")
+    ap.add_argument("--continuation", type=int, default=0,
+                    help="sweeps: also solve every point from the previous point's ground states (1) or not (0)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (hardware counter calibration)")
     args = ap.parse_args()

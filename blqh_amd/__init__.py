@@ -43,7 +43,7 @@ class Params(C.Structure):
 class LanczosOpts(C.Structure):
     _fields_ = [("lam", C.c_int32), ("seed", C.c_uint32), ("tol", C.c_double), ("check_every", C.c_int32),
                 ("store_basis", C.c_int32), ("start", C.c_void_p), ("res_tol", C.c_double),
-                ("skip_states", C.c_int32)]
+                ("skip_states", C.c_int32), ("start_mix", C.c_double)]
 
 
 class LanczosResult(C.Structure):
@@ -299,11 +299,13 @@ class Hamiltonian:
         return y
 
     def lanczos(self, lam=200, seed=0, tol=1e-10, res_tol=1e-7, check_every=1, store_basis=-1, start=None,
-                want_psi=True, skip_states=False):
+                want_psi=True, skip_states=False, start_mix=0.0):
         """Ground state.  Stops when the lowest Ritz value moves by < tol and the
         Ritz residual is < res_tol (tol <= 0: exactly lam steps).  skip_states:
-        passes 1 and 2 only (alpha, beta, E0; no psi, E_rq is NaN)."""
-        opts = LanczosOpts(lam, seed, tol, check_every, store_basis, None, res_tol, 1 if skip_states else 0)
+        passes 1 and 2 only (alpha, beta, E0; no psi, E_rq is NaN).  start_mix:
+        weight of the seeded random unit vector added to a given start."""
+        opts = LanczosOpts(lam, seed, tol, check_every, store_basis, None, res_tol, 1 if skip_states else 0,
+                           start_mix)
         keep = None
         if start is not None:
             keep = np.ascontiguousarray(start, np.complex128)
@@ -414,11 +416,13 @@ class Batch:
     def lanes(self, v):
         _check(self._lib.blqh_batch_set_lanes(self.h, v))
 
-    def lanczos(self, lam=200, seeds=0, tol=1e-10, res_tol=1e-7, check_every=5, starts=None, want_psi=True):
+    def lanczos(self, lam=200, seeds=0, tol=1e-10, res_tol=1e-7, check_every=5, starts=None, want_psi=True,
+                start_mix=0.0):
         """Ground states of every problem; `seeds` is one seed or one per problem,
-        `starts` None or a list of start vectors (None entries: seeded)."""
+        `starts` None or a list of start vectors (None entries: seeded),
+        `start_mix` the weight of the seeded unit vector added to each start."""
         P = len(self.hams)
-        opts = LanczosOpts(lam, 0, tol, check_every, 1, None, res_tol)
+        opts = LanczosOpts(lam, 0, tol, check_every, 1, None, res_tol, 0, start_mix)
         sd = np.broadcast_to(np.asarray(seeds, np.uint32), (P,)).copy()
         keep = []
         st = None

@@ -554,6 +554,7 @@ void blqh_batch_set_lanes_impl(blqh_batch* b, int lanes) {
 
 void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const uint32_t* seeds,
                              const double* const* starts, blqh_lanczos_result* res, double* const* psi_out) {
+    BLQH_REQUIRE(o.start_mix >= 0 && std::isfinite(o.start_mix), BLQH_EINVAL, "start_mix must be finite and >= 0");
     auto t0 = std::chrono::steady_clock::now();
     // BLQH_BATCH_TIMING=1 prints the host-side phases of every solve to stderr
     const bool phase_log = std::getenv("BLQH_BATCH_TIMING") != nullptr;
@@ -630,6 +631,17 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
                     double s = 0;
                     for (auto& z : v0[p]) s += z.x * z.x + z.y * z.y;
                     if (!(s > 0) || !std::isfinite(s)) { v0[p].clear(); continue; }
+                    if (o.start_mix > 0) {  // start/|start| + start_mix * seeded unit vector
+                        std::vector<double2> r;
+                        start_vector(n, seeds ? seeds[p] : o.seed, r);
+                        const double zs = 1.0 / std::sqrt(s);
+                        s = 0;
+                        for (int64_t i = 0; i < n; ++i) {
+                            double2& z = v0[p][i];
+                            z = make_double2(z.x * zs + o.start_mix * r[i].x, z.y * zs + o.start_mix * r[i].y);
+                            s += z.x * z.x + z.y * z.y;
+                        }
+                    }
                     s = std::sqrt(s);
                     for (auto& z : v0[p]) { z.x /= s; z.y /= s; }
                 } else {

@@ -1233,7 +1233,8 @@ int64_t g_stage_n = 0;
 // memory, in the order of start_vector, then divided by the norm on the
 // device: the same IEEE divisions as the host version, without a second
 // host pass, fresh pages or a pageable copy per solve.
-void device_start(int64_t n, uint32_t seed, const double* start, double2* dst, hipStream_t st) {
+void device_start(int64_t n, uint32_t seed, const double* start, double2* dst, hipStream_t st,
+                  double mix = 0.0) {
     std::lock_guard<std::mutex> lk(g_stage_mu);
     if (g_stage_n < n) {
         if (g_stage) (void)hipHostFree(g_stage);
@@ -1244,7 +1245,19 @@ void device_start(int64_t n, uint32_t seed, const double* start, double2* dst, h
         g_stage_n = n;
     }
     double s = 0;
-    if (start) {
+    if (start && mix > 0) {
+        std::vector<double2> r;
+        blqh::start_vector(n, seed, r);  // unit vector
+        const double2* z = (const double2*)start;
+        double zs = 0;
+        for (int64_t i = 0; i < n; ++i) zs += z[i].x * z[i].x + z[i].y * z[i].y;
+        BLQH_REQUIRE(zs > 0 && std::isfinite(zs), BLQH_EINVAL, "start vector is zero or not finite");
+        zs = 1.0 / std::sqrt(zs);
+        for (int64_t i = 0; i < n; ++i) {
+            g_stage[i] = make_double2(z[i].x * zs + mix * r[i].x, z[i].y * zs + mix * r[i].y);
+            s += g_stage[i].x * g_stage[i].x + g_stage[i].y * g_stage[i].y;
+        }
+    } else if (start) {
         const double2* z = (const double2*)start;
         for (int64_t i = 0; i < n; ++i) {
             g_stage[i] = z[i];
@@ -1300,7 +1313,8 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
 
     auto slot = [&](int j) -> double2* { return Wb.get() + (size_t)(store ? j : j % 3) * n; };
     DevBuf<double2> v0(store ? 0 : n);  // kept for the pass-3 replay
-    device_start(n, o.seed, o.start, slot(0), st);
+    BLQH_REQUIRE(o.start_mix >= 0 && std::isfinite(o.start_mix), BLQH_EINVAL, "start_mix must be finite and >= 0");
+    device_start(n, o.seed, o.start, slot(0), st, o.start_mix);
     if (!store) BLQH_HIP(hipMemcpyAsync(v0.get(), slot(0), n * sizeof(double2), hipMemcpyDeviceToDevice, st));
 
     const int gv = vec_grid(n);

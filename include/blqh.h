@@ -148,6 +148,12 @@ typedef struct blqh_lanczos_opts {
     int32_t skip_states;  /* 1: stop after passes 1 and 2 (alpha, beta, E0; psi untouched, E_rq = NaN),
                              for callers that reconstruct states with blqh_lanczos_reconstruct;
                              0: reconstruct the ground state.  Ignored by blqh_batch_lanczos. */
+    double start_mix;     /* with a caller's start vector: start/|start| + start_mix * r, r the seeded
+                             mt19937 unit vector, then normalised (0: the start as given).  A warm start
+                             orthogonal to the ground state (the previous parameter point's ground state
+                             across a level crossing between symmetry classes of the sector) otherwise
+                             converges to an excited level; with start_mix = 0.1 the ground-state
+                             component starts near 0.1/sqrt(N), far above the stopping tolerance. */
 } blqh_lanczos_opts;
 
 typedef struct blqh_lanczos_result {
@@ -171,7 +177,8 @@ int blqh_lanczos(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_r
  * (BLQH_without_spin/lanczos_hamiltonian.cpp:610-644; the spinful one,
  * BLQH_with_spin/lanczos_hamiltonian.cpp:1143-1191, builds psir_0 and psir_1
  * in one replay).  The Lanczos vectors v_0..v_{m-1} are regenerated from the
- * start vector (seeded mt19937 when start is NULL, else start, normalised),
+ * start vector (seeded mt19937 when start is NULL, else start, normalised; a
+ * pass 1 run with start_mix > 0 replays from the mixed vector, so pass that),
  * and for each of the ns coefficient vectors coefs[s*m .. s*m+m-1] the state
  * psi_s = sum_l coefs[s*m+l] conj(v_l) is formed, normalised and written to
  * psi[s*2*nrows ..] (complex, 2*nrows doubles each; psi may be NULL), with
