@@ -475,7 +475,8 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE={world}); "
-              "sectors and parameter points shard across GPUs without communication, see DESIGN.md",
+              "sectors and parameter points would shard across GPUs without communication, one large "
+              "sector by row blocks with an all-gather of x per Lanczos step; neither is built",
               file=sys.stderr)
         sys.exit(2)
 

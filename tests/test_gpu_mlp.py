@@ -78,7 +78,7 @@ def test_training_lowers_the_loss(trained):
 def test_warm_start_saves_iterations_same_energy(trained):
     """At d between the training points: the network's start vector needs fewer
     Lanczos steps than the seeded random one, and both reach the same E0
-    (the plain solve is the judge)."""
+    (the plain solve is the reference point)."""
     nphi, dsas, bases, net, info = trained
     plain_it, warm_it = [], []
     for s, B in bases.items():
