@@ -7,7 +7,7 @@ Delta_SAS = 0.02, gamma = 1, Lanczos cap 200, stop at |dE| < 1e-10
 tridiagonal pass 2, reconstruction and Rayleigh energy) with the Hamiltonian
 resident in HBM; a step counts only if it reached |dE| < 1e-10.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c2u|c1]
+    python bench.py [--steps K] [--warmup W] [--workload nphi16_all|nphi16_u8|nphi8_readme|sweep_nphi12|sweep_spin_nphi8|warm_nphi14]
 
 Prints one JSON line.  The roofline object is for the dominant kernel (the
 fused Lanczos SpMV), timed with HIP events on the library stream during the
@@ -31,28 +31,28 @@ HBM_PEAK_GBS = 8000.0
 # passes, scaled by the same counters on a plain 1 GiB copy (profiles/);
 # reported only when the live run uses the profiled workload and variant
 MEASURED_TRAFFIC = {
-    ("c2", (3, 64, 1024, 1, 8)): (13.47e9, "profiles/k_jag_c2_counters.md"),
-    ("c2", (3, 64, 512, 2, 8)): (23.02e9, "profiles/k_jag_c2_counters.md"),
+    ("nphi16_all", (3, 64, 1024, 1, 8)): (13.47e9, "profiles/k_jag_nphi16_counters.md"),
+    ("nphi16_all", (3, 64, 512, 2, 8)): (23.02e9, "profiles/k_jag_nphi16_counters.md"),
 }
 # fused Lanczos K1 kernel of each SpMV variant (lanczos.hip)
 KERNEL_NAMES = {0: "k_lz1", 1: "k_pspmv<LZ=true>", 2: "k_sps<LZ=true>", 3: "k_jag<LZ=true>", 4: "k_t4<LZ=true>"}
 
 WORKLOADS = {
     # name: (nspecies, nphi, nel, nel_up, J, K, d, delta_sas, label)
-    "c2": (2, 16, 16, -1, 0, 0, 1.0, 0.02, "spinless bilayer Nphi=16 Ne=16 all densities (J,K)=(0,0) d=1.0 DSAS=0.02"),
-    "c2u": (2, 16, 16, 8, 0, 0, 1.0, 0.02, "spinless bilayer Nphi=16 Ne=16 nel_up=8 (J,K)=(0,0) d=1.0 DSAS=0.02"),
-    "c1": (2, 8, 8, 4, 4, 4, 3.0, 0.0, "spinless bilayer Nphi=8 Ne=8 nel_up=4 (J,K)=(4,4) d=3.0"),
+    "nphi16_all": (2, 16, 16, -1, 0, 0, 1.0, 0.02, "spinless bilayer Nphi=16 Ne=16 all densities (J,K)=(0,0) d=1.0 DSAS=0.02"),
+    "nphi16_u8": (2, 16, 16, 8, 0, 0, 1.0, 0.02, "spinless bilayer Nphi=16 Ne=16 nel_up=8 (J,K)=(0,0) d=1.0 DSAS=0.02"),
+    "nphi8_readme": (2, 8, 8, 4, 4, 4, 3.0, 0.0, "spinless bilayer Nphi=8 Ne=8 nel_up=4 (J,K)=(4,4) d=3.0"),
 }
 
 
 # Sweeps of all K-sectors x parameter points, solved by the batched engine.
 SWEEPS = {
-    "c3": dict(S=2, nphi=12, nel=12, nu=-1, nsu=-1, dv=0.0, dz=0.0,
+    "sweep_nphi12": dict(S=2, nphi=12, nel=12, nu=-1, nsu=-1, dv=0.0, dz=0.0,
                sectors=[(J, K) for J in range(12) for K in range(12)],
                points=[(round(0.2 * i, 1), ds) for i in range(16) for ds in (0.0, 0.05, 0.1)],
                label="spinless bilayer Nphi=12 Ne=12 all densities, all 144 (J,K) sectors, "
                      "d in {0,0.2,..,3.0} x DSAS in {0,0.05,0.1}"),
-    "c5": dict(S=4, nphi=8, nel=8, nu=4, nsu=4, dv=0.02, dz=0.01,
+    "sweep_spin_nphi8": dict(S=4, nphi=8, nel=8, nu=4, nsu=4, dv=0.02, dz=0.01,
                sectors=[(J, K) for J in range(8) for K in range(8)],
                points=[(d, 0.05) for d in (0.5, 1.0, 1.5, 2.0)],
                label="spinful bilayer Nphi=8 Ne=8 nel_up=4 nel_su=4, all 64 (J,K) sectors, "
@@ -320,7 +320,7 @@ def run_sweep(args):
 
 # Learned warm start: train on ground states at coarse d, solve at other d.
 WARM = {
-    "c4": dict(nphi=14, nel=14, nu=-1, sectors=[(0, 0), (7, 7)], dsas=0.05,
+    "warm_nphi14": dict(nphi=14, nel=14, nu=-1, sectors=[(0, 0), (7, 7)], dsas=0.05,
                train_d=[0.5 * i for i in range(7)],
                eval_d=[round(0.05 * i, 2) for i in range(61)],
                label="spinless bilayer Nphi=14 Ne=14 all densities, sectors (0,0) and (7,7), DSAS=0.05, "
@@ -455,7 +455,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + sorted(SWEEPS) + sorted(WARM))
+    ap.add_argument("--workload", default="nphi16_all", choices=sorted(WORKLOADS) + sorted(SWEEPS) + sorted(WARM))
     ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--check-every", type=int, default=5)
     ap.add_argument("--lam", type=int, default=200)

@@ -1046,7 +1046,7 @@ void tune(blqh_hamil* H) {
     // Near-ties (within 2 %) of the winning panel kind go to the largest
     // workgroup, then the fewest workgroups per CU: the per-panel barrier keeps
     // a whole workgroup on one panel, so bigger workgroups hold a smaller x
-    // working set per CU (config 2: 1024 x 1 moves 13.5 GB per launch where
+    // working set per CU (2.35M-state N_phi = 16 sector: 1024 x 1 moves 13.5 GB per launch where
     // 512 x 2 moves 23 GB, in the same time).
     if (bp.kind >= 2)
         for (const auto& c : timed)

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def c2():
+def sector16():
     print("building the 2,348,125-state sector", flush=True)
     B = blqh_amd.Basis(16, 16, -1, 0, 0)
     H = blqh_amd.Hamiltonian(B, blqh_amd.params(16, d=1.0, delta_sas=0.02))
@@ -22,15 +22,15 @@ def c2():
     return B, H
 
 
-def test_dimensions(c2):
-    B, H = c2
+def test_dimensions(sector16):
+    B, H = sector16
     assert H.n == 2348125
     # every row holds its diagonal; a few hundred entries per row on average
     assert 300 * H.n < H.nnz < 400 * H.n
 
 
-def test_spmv_hermitian_and_linear(c2):
-    B, H = c2
+def test_spmv_hermitian_and_linear(sector16):
+    B, H = sector16
     rng = np.random.default_rng(7)
     x = rng.normal(size=H.n) + 1j * rng.normal(size=H.n)
     y = rng.normal(size=H.n) + 1j * rng.normal(size=H.n)
@@ -42,8 +42,8 @@ def test_spmv_hermitian_and_linear(c2):
     assert np.linalg.norm(z - ref) <= 1e-13 * np.linalg.norm(ref)
 
 
-def test_ground_state_from_two_seeds(c2):
-    B, H = c2
+def test_ground_state_from_two_seeds(sector16):
+    B, H = sector16
     r1 = H.lanczos(lam=200, seed=1, tol=1e-10)
     r2 = H.lanczos(lam=200, seed=2, tol=1e-10)
     print({k: r1[k] for k in ("E0", "iterations", "residual")}, flush=True)

@@ -160,12 +160,12 @@ def test_dense_cross_check_matches_numpy():
     assert ep == pytest.approx(np.real(np.vdot(Hd @ v, v)), rel=1e-12)
 
 
-def test_config3_sector_against_oracle(oracle_mod):
+def test_nphi12_all_density_sector_against_oracle(oracle_mod):
     """One all-density N_phi=12 sector (18,772 states, the batched workload's
     sector size): basis and CSR exact, energy vs the oracle's Lanczos."""
     O = oracle_mod
     case, kw = (2, 12, 12, -1, -1, 5, 3), dict(d=1.0, delta_sas=0.05)
-    print("config-3 sector", flush=True)
+    print("N_phi = 12 all-density sector", flush=True)
     OB = O.Basis(O.sector(*case))
     B, H = _build(case, kw)
     np.testing.assert_array_equal(B.id, OB.id)
