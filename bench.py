@@ -35,7 +35,8 @@ MEASURED_TRAFFIC = {
     ("nphi16_all", (3, 64, 512, 2, 8)): (23.02e9, "profiles/k_jag_nphi16_counters.md"),
 }
 # fused Lanczos K1 kernel of each SpMV variant (lanczos.hip)
-KERNEL_NAMES = {0: "k_lz1", 1: "k_pspmv<LZ=true>", 2: "k_sps<LZ=true>", 3: "k_jag<LZ=true>", 4: "k_t4<LZ=true>"}
+KERNEL_NAMES = {0: "k_lz1", 1: "k_pspmv<LZ=true>", 2: "k_sps<LZ=true>", 3: "k_jag<LZ=true>", 4: "k_t4<LZ=true>",
+                5: "k_jagr<LZ=true>"}
 
 WORKLOADS = {
     # name: (nspecies, nphi, nel, nel_up, J, K, d, delta_sas, label)
