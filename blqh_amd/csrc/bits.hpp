@@ -142,11 +142,12 @@ template <class T>
 BLQH_HD int64_t rank_word(const RankSpec& s, const T* cnt, uint64_t w) {
     int k = s.nel, r = s.Jt, a = s.nA, b = s.nB;
     int64_t rk = 0;
-    for (int p = s.nbits - 1; p >= 0 && k > 0; --p) {
+    int pm = (s.nbits - 1) % s.nphi;  // p mod nphi, stepped down with p
+    for (int p = s.nbits - 1; p >= 0 && k > 0; --p, pm = pm == 0 ? s.nphi - 1 : pm - 1) {
         if (!((w >> p) & 1ULL)) continue;
         rk += (int64_t)cnt[rs_index(s, p, k, r, a, b)];
         --k;
-        r -= p % s.nphi;
+        r -= pm;
         if (r < 0) r += s.R;
         if (s.R == 1) r = 0;
         if ((s.maskA >> p) & 1ULL) --a;

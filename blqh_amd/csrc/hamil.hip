@@ -250,14 +250,22 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
             if (lane < nbits && ((L >> lane) & 1ULL)) occ[popc(L & lowbits(lane))] = lane;
             __syncthreads();
             const int ncand = (A.terms & 1) ? npairs * T : 0;
+            // candidate cnd = pr * T + tt, advanced by 64 without division
+            const int d0 = 64 / T, d1 = 64 % T;
+            int pr = lane / T, tt = lane % T;
             for (int cnd = lane; cnd < ncand; cnd += 64) {
-                const int pr = cnd / T;
-                const int t = cnd - pr * T - (nphi - 1);
+                const int t = tt - (nphi - 1);
                 const unsigned short pw = sPair[pr];
                 int j;
                 double re, im;
                 if (pair_term(ctx, L, signl, Dl, ql, occ[pw & 0xff], occ[pw >> 8], t, j, re, im))
                     if (!hash_add<CAP>(keys, vre, vim, j, re, im)) ovf = true;
+                pr += d0;
+                tt += d1;
+                if (tt >= T) {
+                    tt -= T;
+                    ++pr;
+                }
             }
             for (int h = lane; h < A.nhop; h += 64) {
                 int j;

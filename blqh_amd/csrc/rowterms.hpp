@@ -81,18 +81,25 @@ BLQH_HD bool locate(const RowCtx& c, uint64_t R0, int ql, int& j, int& qr, int& 
 BLQH_HD bool pair_term(const RowCtx& c, uint64_t L, int signl, int Dl, int ql, int n, int m, int t, int& j, double& re,
                        double& im) {
     const int nphi = c.g.nphi;
-    const int a = n / nphi, b = m / nphi;
-    const int nt = a * nphi + (n - a * nphi + t + nphi) % nphi;
-    const int mt = b * nphi + (m - b * nphi - t + nphi) % nphi;
+    // blocks (layer / species) of n and m and the shifted orbitals, wrapped
+    // within the block (|t| < nphi): compares instead of integer division
+    const int a = n >= nphi ? (n >= 2 * nphi ? (n >= 3 * nphi ? 3 : 2) : 1) : 0;
+    const int b = m >= nphi ? (m >= 2 * nphi ? (m >= 3 * nphi ? 3 : 2) : 1) : 0;
+    const int n0 = n - a * nphi, m0 = m - b * nphi;
+    int xn = n0 + t, xm = m0 - t;
+    xn += xn < 0 ? nphi : (xn >= nphi ? -nphi : 0);
+    xm += xm < 0 ? nphi : (xm >= nphi ? -nphi : 0);
+    const int nt = a * nphi + xn;
+    const int mt = b * nphi + xm;
     const uint64_t rest = L ^ ((1ULL << n) | (1ULL << m));
     const uint64_t tgt = (1ULL << nt) | (1ULL << mt);
     if (nt == mt || (rest & tgt)) return false;
     int qr, Dr, sr;
     if (!locate(c, rest | tgt, ql, j, qr, Dr, sr)) return false;
     const int sign = sign2(L, n, m, nt, mt) * signl * sr;
-    const int half = c.g.S / 2;
-    const int alpha = (a / half) != (b / half) ? 1 : 0;
-    int dm = (mt % nphi) - (n % nphi);
+    const int sh = c.g.S == 4 ? 1 : 0;  // layer of block a: a / (S / 2)
+    const int alpha = (a >> sh) != (b >> sh) ? 1 : 0;
+    int dm = xm - n0;
     dm = dm < 0 ? -dm : dm;
     const int at = t < 0 ? -t : t;
     const double Cv = c.C[alpha * nphi * nphi + dm * nphi + at];
