@@ -93,3 +93,41 @@ def test_low_lying_spectrum_from_fixed_lambda_lanczos():
     assert lev[0] == pytest.approx(w[0], rel=1e-12)
     for e in ref:
         assert np.min(np.abs(lev - e)) < 1e-8 * abs(e), e
+
+
+def test_tridiag_vectors_match_lapack_and_reject_bad_ranges():
+    a, b = _tri(30, 5)
+    T = np.diag(a) + np.diag(b[1:30], 1) + np.diag(b[1:30], -1)
+    w, V = np.linalg.eigh(T)
+    vecs = blqh_amd.tridiag_vectors(a, b, 2, 4)
+    for k in range(4):
+        v = vecs[k]
+        assert v[0] >= 0
+        assert abs(abs(v @ V[:, 2 + k]) - 1) < 1e-10
+        assert np.linalg.norm(T @ v - w[2 + k] * v) < 1e-10
+    for k0, cnt in ((-1, 1), (29, 2), (0, 0), (0, 31)):
+        with pytest.raises(blqh_amd.BlqhError):
+            blqh_amd.tridiag_vectors(a, b, k0, cnt)
+
+
+def test_spectrum_distinct_edge_cases():
+    e = np.array([-1.0, -1.0, -0.5])
+    assert len(blqh_amd.spectrum_distinct(e, kmax=0)) == 0
+    np.testing.assert_array_equal(blqh_amd.spectrum_distinct(e[:1], kmax=5), [-1.0])
+    np.testing.assert_array_equal(blqh_amd.spectrum_distinct(e, kmax=5, gap=0.6), [-1.0])
+
+
+def test_record_single_step_and_malformed_files(tmp_path):
+    p = tmp_path / "one.txt"
+    blqh_amd.record_save(p, 1, 3, eig=[-2.5], beta=[1.0, 0.0], alpha=[-2.5], psi0=[1.0], psin0=[1.0])
+    r = blqh_amd.record_load(p)
+    assert r["lambda"] == 1 and r["nhilbert"] == 1 and r["seed"] == 3
+    np.testing.assert_array_equal(r["norm"], [1.0, 0.0])
+    text = p.read_text()
+    for bad in ("", "nHilbert:= x\n", text.replace("[ -2.5 ,", "[ abc ,"), text.split("norm")[0]):
+        q = tmp_path / "bad.txt"
+        q.write_text(bad)
+        with pytest.raises(blqh_amd.BlqhError):
+            blqh_amd.record_load(q)
+    with pytest.raises(blqh_amd.BlqhError):
+        blqh_amd.record_load(tmp_path / "missing.txt")
