@@ -61,6 +61,8 @@ struct BState {
     const BProb* probs;
     const int32_t* item_prob;  // work item -> problem
     int64_t nitems;
+    const int32_t* xitems;     // k_b1j work lists: items grouped per XCD (real, complex, all members)
+    const int32_t* xseg;       // [3][9] list offsets: kind (real, complex, all) x XCD
     const int32_t* active;     // [P]
     int32_t* jbase;            // step of the first kernel in a replay (advanced by k_bnext)
     int cap;                   // lambda
@@ -183,21 +185,27 @@ __global__ void __launch_bounds__(kBT) k_b1(BState S, int jrel, const double2* _
 
 // Same contract as k_b1 over panels in step-major order (jag_row_sum): wave w
 // of the workgroup owns the item's w-th 64-row slice, one lane per row.
-// REAL: launched once for the problems with real panel values and once for
-// the complex ones (each skips the other kind), so each instantiation holds
-// only its own registers.
-template <bool LZ, bool REAL>
+// KIND 0 / 1 walk only the members with real / complex panel values, KIND 2
+// walks all members and branches per work item (one launch per step for a
+// mixed batch, so that the members of both kinds run side by side).
+template <bool LZ, int KIND>
 __global__ void __launch_bounds__(kBT) k_b1j(BState S, int jrel, const double2* __restrict__ xs,
                                              double2* __restrict__ ys, double* __restrict__ part) {
     __shared__ double sh[kBT / 64];
     const int j = LZ ? *S.jbase + jrel : 0;
     if (LZ && j >= S.cap) return;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int64_t it = blockIdx.x; it < S.nitems; it += gridDim.x) {
+    // Workgroups go to the XCDs round robin (workgroup b to XCD b mod 8, grid
+    // a multiple of 8); each XCD walks its own list of whole problems, so the
+    // x vectors it gathers from are those of the few problems its 32 CUs are
+    // working on and stay in its L2.
+    const int32_t* seg = S.xseg + KIND * 9 + (blockIdx.x & 7);
+    const int32_t kend = seg[1];
+    for (int32_t k = seg[0] + (int32_t)(blockIdx.x >> 3); k < kend; k += (int32_t)(gridDim.x >> 3)) {
+        const int64_t it = S.xitems[k];
         const int p = S.item_prob[it];
         if (!S.active[p]) continue;  // uniform over the workgroup
         const BProb P = S.probs[p];
-        if (P.real != (REAL ? 1 : 0)) continue;
         const double2* x;
         double2* y;
         const double2* Wp = nullptr;
@@ -225,7 +233,10 @@ __global__ void __launch_bounds__(kBT) k_b1j(BState S, int jrel, const double2* 
                 const uint32_t e = myrow < n ? rp[myrow + 1] : rp[rend];
                 const int64_t run = P.pbase[pan] + (uint32_t)__builtin_amdgcn_readfirstlane((int)s);  // rp[row0]
                 const double2* xp = x + (int64_t)pan * kPanelWidth;
-                jag_row_sum<REAL>(P.pcol + run, P.pvr + run, P.pvc + run, xp, e - s, ar, ai);
+                if (KIND == 2 ? P.real != 0 : KIND == 0)
+                    jag_row_sum<true>(P.pcol + run, P.pvr + run, P.pvc + run, xp, e - s, ar, ai);
+                else
+                    jag_row_sum<false>(P.pcol + run, P.pvr + run, P.pvc + run, xp, e - s, ar, ai);
             }
             if (myrow < n) {
                 double ur = ar, ui = ai;
@@ -364,6 +375,8 @@ struct blqh_batch {
     std::vector<int64_t> item0;  // [P + 1]
     int64_t rows = 0, nitems = 0, nnz = 0;
     DevBuf<int32_t> item_prob;
+    DevBuf<int32_t> xitems, xseg;    // per-XCD work lists of k_b1j (rebuilt by refresh_probs)
+    std::vector<int32_t> h_xitems, h_xseg;
     int lpr = 0;                 // lanes per row of k_b1 (autotuned)
     bool has_real = false, has_complex = false;  // value kinds among the members
     int grid = 0;
@@ -385,6 +398,72 @@ struct blqh_batch {
 
 namespace {
 
+// Work lists of k_b1j, per value kind.  A member whose x fits one panel
+// (at most 65,536 rows, 1 MiB) is cut into chunks of consecutive items:
+// whole members when there are many, so that an XCD gathers from the x
+// vectors of the few members it is working on, which its L2 then holds; at
+// least 64 chunks in all, so that a few members still spread over every XCD.
+// The chunks are dealt largest first, each to the XCD with the fewest
+// nonzeros so far.  The items of a larger member are dealt one by one, item i
+// to XCD i mod 8, which makes all XCDs walk the member together (measured
+// faster for 204k-state members than chunks).  A list holds its items in
+// item order.
+void build_xcd_lists(blqh_batch* b) {
+    const int P = (int)b->hs.size();
+    std::vector<int32_t> items, seg(27, 0);
+    items.reserve(2 * b->nitems);
+    struct Chunk { int64_t i0, i1; double w; };
+    for (int kind = 0; kind < 3; ++kind) {
+        auto mine = [&](int p) { return kind == 2 || (b->hs[p]->pan.real ? 0 : 1) == kind; };
+        auto per_item = [&](int p) { return (double)b->hs[p]->nnz / (double)(b->item0[p + 1] - b->item0[p]); };
+        std::vector<double> load(8, 0.0);
+        std::vector<std::vector<Chunk>> own(8);
+        int64_t small_items = 0;
+        for (int p = 0; p < P; ++p) {
+            if (!mine(p)) continue;
+            if (b->hs[p]->pan.npan == 1) {
+                small_items += b->item0[p + 1] - b->item0[p];
+                continue;
+            }
+            for (int64_t it = b->item0[p]; it < b->item0[p + 1]; ++it) {
+                own[it & 7].push_back({it, it + 1, per_item(p)});
+                load[it & 7] += per_item(p);
+            }
+        }
+        const int64_t cmax = std::max<int64_t>(1, ceil_div(small_items, 64));
+        std::vector<Chunk> ch;
+        for (int p = 0; p < P; ++p) {
+            if (!mine(p) || b->hs[p]->pan.npan != 1) continue;
+            for (int64_t i0 = b->item0[p]; i0 < b->item0[p + 1]; i0 += cmax) {
+                const int64_t i1 = std::min(i0 + cmax, b->item0[p + 1]);
+                ch.push_back({i0, i1, per_item(p) * (double)(i1 - i0)});
+            }
+        }
+        std::stable_sort(ch.begin(), ch.end(), [](const Chunk& a, const Chunk& c) { return a.w > c.w; });
+        for (const Chunk& c : ch) {
+            const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[x] += c.w;
+            own[x].push_back(c);
+        }
+        for (int x = 0; x < 8; ++x) {
+            std::sort(own[x].begin(), own[x].end(), [](const Chunk& a, const Chunk& c) { return a.i0 < c.i0; });
+            seg[kind * 9 + x] = (int32_t)items.size();
+            for (const Chunk& c : own[x])
+                for (int64_t it = c.i0; it < c.i1; ++it) items.push_back((int32_t)it);
+        }
+        seg[kind * 9 + 8] = (int32_t)items.size();
+    }
+    if (items == b->h_xitems && seg == b->h_xseg) return;
+    b->h_xitems.swap(items);
+    b->h_xseg.swap(seg);
+    // ordered after every kernel queued so far; the host vectors are not
+    // touched again before the copies have completed
+    hipStream_t st = stream();
+    BLQH_HIP(hipMemcpyAsync(b->xitems.get(), b->h_xitems.data(), b->h_xitems.size() * 4, hipMemcpyHostToDevice, st));
+    BLQH_HIP(hipMemcpyAsync(b->xseg.get(), b->h_xseg.data(), 27 * 4, hipMemcpyHostToDevice, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+}
+
 // Row-major panel layout and current panel buffers of every member.
 void refresh_probs(blqh_batch* b) {
     b->has_real = b->has_complex = false;
@@ -392,6 +471,7 @@ void refresh_probs(blqh_batch* b) {
     // k_b1j reads step-major runs, k_b1 row-major ones: one relayout launch for all members
     // plain step order: a rebind re-lays out every member at each parameter point
     set_panel_layouts(b->hs.data(), (int)b->hs.size(), b->lpr == 64, b->relayout.get(), b->relayout.bytes(), false);
+    build_xcd_lists(b);
     for (size_t p = 0; p < b->hs.size(); ++p) {
         blqh_hamil* H = b->hs[p];
         BProb& d = b->h_probs[p];
@@ -409,8 +489,12 @@ void launch_b1(const blqh_batch* b, const BState& S, int jrel, const double2* xs
     const dim3 g(b->grid), t(kBT);
     switch (b->lpr) {
         case 64:
-            if (b->has_real) hipLaunchKernelGGL((k_b1j<LZ, true>), g, t, 0, st, S, jrel, xs, ys, part);
-            if (b->has_complex) hipLaunchKernelGGL((k_b1j<LZ, false>), g, t, 0, st, S, jrel, xs, ys, part);
+            if (b->has_real && b->has_complex) {  // measured faster than one launch per kind
+                hipLaunchKernelGGL((k_b1j<LZ, 2>), g, t, 0, st, S, jrel, xs, ys, part);
+                break;
+            }
+            if (b->has_real) hipLaunchKernelGGL((k_b1j<LZ, 0>), g, t, 0, st, S, jrel, xs, ys, part);
+            if (b->has_complex) hipLaunchKernelGGL((k_b1j<LZ, 1>), g, t, 0, st, S, jrel, xs, ys, part);
             break;
         case 4: hipLaunchKernelGGL((k_b1<4, LZ>), g, t, 0, st, S, jrel, xs, ys, part); break;
         case 8: hipLaunchKernelGGL((k_b1<8, LZ>), g, t, 0, st, S, jrel, xs, ys, part); break;
@@ -425,6 +509,8 @@ BState make_state(const blqh_batch* b, Pools& pl, int cap) {
     S.probs = pl.probs.get();
     S.item_prob = b->item_prob.get();
     S.nitems = b->nitems;
+    S.xitems = b->xitems.get();
+    S.xseg = b->xseg.get();
     S.active = pl.active.get();
     S.jbase = pl.jbase.get();
     S.cap = cap;
@@ -507,7 +593,10 @@ blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count, int lanes) 
         BLQH_REQUIRE(b->nitems < (int64_t)1 << 31, BLQH_ERANGE, "batch too large");
         b->item_prob.alloc(ip.size());
         BLQH_HIP(hipMemcpy(b->item_prob.get(), ip.data(), ip.size() * 4, hipMemcpyHostToDevice));
-        b->grid = (int)std::max<int64_t>(1, std::min<int64_t>(b->nitems, (int64_t)runtime().num_cus * 8));
+        // a multiple of 8 (one share per XCD in k_b1j)
+        b->grid = (int)(8 * ceil_div(std::min<int64_t>(b->nitems, (int64_t)runtime().num_cus * 8), 8));
+        b->xitems.alloc(2 * b->nitems);
+        b->xseg.alloc(27);
         b->relayout.alloc(panel_layout_scratch_bytes(b->hs.data(), count));
         if (lanes) {
             b->lpr = lanes;

@@ -169,3 +169,24 @@ def test_rebind_to_next_parameter_point(hams, batch):
             batch.rebind(new[::-1])
     finally:
         batch.rebind(old_members)
+
+
+def test_many_members_one_sector_each():
+    """All 100 (J, K) sectors of the N_phi = 10 all-density system in one
+    batch (1,846-1,855 states each): enough members that each is worked on
+    whole, real and complex ones mixed.  Every member's ground state equals the
+    single solver's; a second batch of the same members gives the same bits."""
+    hs = []
+    for J in range(10):
+        for K in range(10):
+            B = blqh_amd.Basis(10, 10, -1, J, K)
+            hs.append(blqh_amd.Hamiltonian(B, blqh_amd.params(10, d=1.0, delta_sas=0.05)))
+    print(f"batch of {len(hs)} sectors", flush=True)
+    seeds = [3 + i for i in range(len(hs))]
+    r = blqh_amd.Batch(hs, lanes=64).lanczos(lam=200, seeds=seeds, want_psi=False)
+    r2 = blqh_amd.Batch(hs, lanes=64).lanczos(lam=200, seeds=seeds, want_psi=False)
+    for i, H in enumerate(hs):
+        s = H.lanczos(lam=200, seed=seeds[i])
+        assert r[i]["converged"] == 1 and s["converged"] == 1, i
+        assert r[i]["E0"] == pytest.approx(s["E0"], rel=1e-10, abs=0), i
+        assert r2[i]["E0"] == r[i]["E0"] and r2[i]["iterations"] == r[i]["iterations"], i
