@@ -2,6 +2,7 @@
 // stages of the blqh library.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <cstdint>
 #include <vector>
 
@@ -23,6 +24,7 @@ struct blqh_basis {
     std::vector<uint64_t> h_id;
     std::vector<int16_t> h_period;
     double seconds_host = 0, seconds_device = 0;
+    mutable std::atomic<int32_t> max_row_nnz{0};  // largest row kept by a build with the pair terms (0: none yet)
 };
 
 struct blqh_hamil {

@@ -176,6 +176,7 @@ struct BuildArgs {
     int32_t* ovf_count;
     int32_t* ovf_rows;
     int32_t* err;             // fill/count mismatch counter
+    int32_t* row_max;         // largest row (entries kept) seen by the count / single pass
     int32_t* stage_col;       // single pass: row i's sorted entries at i * CAP
     double2* stage_val;
     uint8_t* stage_ok;        // single pass: 1 when row i was staged (0: overflow row)
@@ -298,7 +299,10 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
             for (int x = lane; x < CAP; x += 64)
                 if (keys[x] >= 0 && hypot(vre[x], vim[x]) > 1e-10) ++c;
             for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-            if (lane == 0) A.nnz_out[i] = c;
+            if (lane == 0) {
+                A.nnz_out[i] = c;
+                atomicMax(A.row_max, c);
+            }
         } else {
             // compact (key, slot) pairs: lane owns a contiguous chunk of slots
             constexpr int CH = CAP / 64;
@@ -344,6 +348,7 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
                 if (lane == 0) {
                     A.nnz_out[i] = E;
                     A.stage_ok[i] = 1;
+                    atomicMax(A.row_max, E);
                 }
             } else {
                 const int64_t base = A.rowptr[i];
@@ -384,7 +389,11 @@ __global__ void __launch_bounds__(256) k_stage_copy(int64_t n, const int32_t* __
     }
 }
 
-constexpr int kSmall = 512, kLarge = 4096;
+// Row hash capacities: 256 or 512 slots for the first pass (256 when an
+// earlier build over the same basis kept at most kSmallRows entries in every
+// row: half the LDS, so 50 % more wavefronts per CU), 4096 for the rows that
+// overflow it.
+constexpr int kSmall = 512, kSmaller = 256, kLarge = 4096, kSmallRows = 192;
 
 size_t table_smem(const Geo& g) {
     const int nphi = g.nphi, T = 2 * nphi - 1, np = g.nel * (g.nel - 1) / 2;
@@ -405,6 +414,14 @@ void launch_build(const BuildArgs& A, int64_t rows, size_t smem, hipStream_t st,
     BLQH_HIP(hipMemsetAsync(A.row_counter, 0, sizeof(int32_t), st));
     hipLaunchKernelGGL((k_build<CAP, MODE>), dim3((unsigned)blocks), dim3(64), smem, st, A);
     BLQH_HIP(hipGetLastError());
+}
+
+template <int MODE>
+void launch_first(int cap, const BuildArgs& A, int64_t rows, size_t smem, hipStream_t st, int ncu) {
+    if (cap == kSmaller)
+        launch_build<kSmaller, MODE>(A, rows, smem, st, ncu);
+    else
+        launch_build<kSmall, MODE>(A, rows, smem, st, ncu);
 }
 
 }  // namespace
@@ -464,35 +481,40 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p_in, blqh_ha
     A.row_counter = ctr.get();
     A.ovf_count = ctr.get() + 1;
     A.err = ctr.get() + 2;
+    A.row_max = ctr.get() + 3;
+    const int seen = B->max_row_nnz.load();
+    int cap = seen > 0 && seen <= kSmallRows ? kSmaller : kSmall;
+    if (const char* e = std::getenv("BLQH_BUILD_CAP")) cap = std::atoi(e) == kSmaller ? kSmaller : kSmall;
     A.ovf_rows = ovf_rows.get();
     A.nnz_out = nnz.get();
     const size_t smem = table_smem(B->geo);
     // One pass when the staging slots fit comfortably (each row hashed once,
-    // sorted into a slot of kSmall entries, compacted after the scan); else
+    // sorted into a slot of `cap` entries, compacted after the scan); else
     // count, scan, fill.
     size_t free_b = 0, total_b = 0;
     BLQH_HIP(hipMemGetInfo(&free_b, &total_b));
-    const size_t stage_bytes = (size_t)B->n * kSmall * (sizeof(int32_t) + sizeof(double2)) + (size_t)B->n;
+    const size_t stage_bytes = (size_t)B->n * cap * (sizeof(int32_t) + sizeof(double2)) + (size_t)B->n;
     const bool single = stage_bytes <= free_b / 4 && std::getenv("BLQH_BUILD_TWO_PASS") == nullptr;
     DevBuf<int32_t> scol;
     DevBuf<double2> sval;
     DevBuf<uint8_t> sok;
     if (single) {
-        scol.alloc((size_t)B->n * kSmall);
-        sval.alloc((size_t)B->n * kSmall);
+        scol.alloc((size_t)B->n * cap);
+        sval.alloc((size_t)B->n * cap);
         sok.alloc(std::max<int64_t>(B->n, 1));
         A.stage_col = scol.get();
         A.stage_val = sval.get();
         A.stage_ok = sok.get();
-        launch_build<kSmall, 2>(A, B->n, smem, st, ncu);
+        launch_first<2>(cap, A, B->n, smem, st, ncu);
     } else {
         // count
-        launch_build<kSmall, 0>(A, B->n, smem, st, ncu);
+        launch_first<0>(cap, A, B->n, smem, st, ncu);
     }
     int32_t h_ctr[4];
     BLQH_HIP(hipMemcpyAsync(h_ctr, ctr.get(), sizeof(h_ctr), hipMemcpyDeviceToHost, st));
     BLQH_HIP(hipStreamSynchronize(st));
     const int32_t novf = h_ctr[1];
+    int32_t row_max = h_ctr[3];
     BuildArgs Ao = A;
     DevBuf<int32_t> ovf2(std::max(novf, 1)), ctr2(4);
     if (novf > 0) {
@@ -503,11 +525,19 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p_in, blqh_ha
         Ao.row_counter = ctr2.get();
         Ao.ovf_count = ctr2.get() + 1;
         Ao.err = ctr2.get() + 2;
+        Ao.row_max = ctr2.get() + 3;
         Ao.ovf_rows = ovf2.get();
         launch_build<kLarge, 0>(Ao, novf, smem, st, ncu);
         BLQH_HIP(hipMemcpyAsync(h_ctr, ctr2.get(), sizeof(h_ctr), hipMemcpyDeviceToHost, st));
         BLQH_HIP(hipStreamSynchronize(st));
         BLQH_REQUIRE(h_ctr[1] == 0, BLQH_ERANGE, "a Hamiltonian row has more than 4096 distinct columns");
+        row_max = std::max(row_max, h_ctr[3]);
+    }
+    // rows of a build with the pair terms size the hash of later builds over this basis
+    if (terms & 1) {
+        int32_t cur = B->max_row_nnz.load();
+        while (row_max > cur && !B->max_row_nnz.compare_exchange_weak(cur, row_max)) {
+        }
     }
     phase(1);
     H->rowptr.alloc(B->n + 1);
@@ -523,11 +553,15 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p_in, blqh_ha
     BLQH_HIP(hipMemsetAsync(ctr.get(), 0, 4 * sizeof(int32_t), st));
     if (single) {
         const int gs = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(B->n, 4), 1), 65536);
-        hipLaunchKernelGGL(k_stage_copy<kSmall>, dim3(gs), dim3(256), 0, st, B->n, scol.get(), sval.get(), sok.get(),
-                           H->rowptr.get(), H->col.get(), H->val.get());
+        if (cap == kSmaller)
+            hipLaunchKernelGGL(k_stage_copy<kSmaller>, dim3(gs), dim3(256), 0, st, B->n, scol.get(), sval.get(),
+                               sok.get(), H->rowptr.get(), H->col.get(), H->val.get());
+        else
+            hipLaunchKernelGGL(k_stage_copy<kSmall>, dim3(gs), dim3(256), 0, st, B->n, scol.get(), sval.get(),
+                               sok.get(), H->rowptr.get(), H->col.get(), H->val.get());
         BLQH_HIP(hipGetLastError());
     } else {
-        launch_build<kSmall, 1>(A, B->n, smem, st, ncu);  // overflow rows are skipped (counted in err)
+        launch_first<1>(cap, A, B->n, smem, st, ncu);  // overflow rows are skipped (counted in err)
     }
     if (novf > 0) {
         BLQH_HIP(hipMemsetAsync(ctr2.get(), 0, 4 * sizeof(int32_t), st));
