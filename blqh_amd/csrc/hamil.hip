@@ -118,10 +118,10 @@ HostTables host_tables(const blqh_sector& s, const blqh_params& p) {
     HostTables H;
     const int nphi = s.nphi;
     coulomb_tables(s, p, H.C, H.Ec);
-    H.FT.resize(nphi * nphi);
-    for (int a = 0; a < nphi; ++a)
-        for (int b = 0; b < nphi; ++b)
-            H.FT[a * nphi + b] = make_double2(std::cos(2.0 * M_PI * (a - b) * s.K / nphi), std::sin(2.0 * M_PI * (a - b) * s.K / nphi));
+    // the reference's FT[a][b] depends on a - b only (same argument, same bits)
+    H.FT.resize(2 * nphi - 1);
+    for (int dq = -nphi + 1; dq < nphi; ++dq)
+        H.FT[dq + nphi - 1] = make_double2(std::cos(2.0 * M_PI * dq * s.K / nphi), std::sin(2.0 * M_PI * dq * s.K / nphi));
     H.tw.resize(2 * nphi - 1);
     for (int t = -nphi + 1; t < nphi; ++t) {
         const double ang = s.nspecies == 2 ? 2 * p.theta_y * t / nphi : p.theta_y * t / nphi;
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
     extern __shared__ double smem[];
     __shared__ int keys[CAP];
     __shared__ double vre[CAP], vim[CAP];
-    __shared__ unsigned long long sbuf[MODE >= 1 ? CAP : 1];
+    __shared__ unsigned long long sbuf[MODE >= 1 && CAP > 512 ? CAP : 1];
     __shared__ int occ[64];
     __shared__ int s_row, s_ovf;
 
@@ -219,12 +219,12 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
     const int T = 2 * nphi - 1;
     // stage the tables in LDS
     double* sC = smem;                                   // 2 nphi^2
-    double2* sFT = (double2*)(sC + 2 * nphi * nphi);     // nphi^2
-    double2* sTW = sFT + nphi * nphi;                    // T
+    double2* sFT = (double2*)(sC + 2 * nphi * nphi);     // T
+    double2* sTW = sFT + T;                              // T
     unsigned short* sPair = (unsigned short*)(sTW + T);  // npairs
     const int npairs = nel * (nel - 1) / 2;
     for (int x = lane; x < 2 * nphi * nphi; x += 64) sC[x] = A.ctx.C[x];
-    for (int x = lane; x < nphi * nphi; x += 64) sFT[x] = A.ctx.FT[x];
+    for (int x = lane; x < T; x += 64) sFT[x] = A.ctx.FT[x];
     for (int x = lane; x < T; x += 64) sTW[x] = A.ctx.tw[x];
     for (int x = lane; x < npairs; x += 64) sPair[x] = pair_decode(nel, x);
     RowCtx ctx = A.ctx;
@@ -303,8 +303,89 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
                 A.nnz_out[i] = c;
                 atomicMax(A.row_max, c);
             }
+        } else if constexpr (CAP <= 512) {
+            // compact the kept (column, value) entries to the front of the
+            // hash arrays (lane owns a contiguous chunk of slots, read into
+            // registers first), then sort them by column in place
+            constexpr int CH = CAP / 64;
+            int kk[CH];
+            double rr[CH], ii[CH];
+            int c = 0;
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int x = lane * CH + u;
+                rr[u] = vre[x];
+                ii[u] = vim[x];
+                kk[u] = keys[x] >= 0 && hypot(rr[u], ii[u]) > 1e-10 ? keys[x] : -1;
+                c += kk[u] >= 0;
+            }
+            int pre = c;  // inclusive scan over lanes
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(pre, o, 64);
+                if (lane >= o) pre += y;
+            }
+            const int E = __shfl(pre, 63, 64);
+            __syncthreads();  // every slot read before the front is overwritten
+            int w = pre - c;
+#pragma unroll
+            for (int u = 0; u < CH; ++u)
+                if (kk[u] >= 0) {
+                    keys[w] = kk[u];
+                    vre[w] = rr[u];
+                    vim[w] = ii[u];
+                    ++w;
+                }
+            int P = 1;
+            while (P < E) P <<= 1;
+            for (int x = E + lane; x < P; x += 64) keys[x] = 0x7fffffff;  // columns are < 2^31 - 1
+            __syncthreads();
+            // bitonic sort of keys[0, P) (distinct columns), values moved along
+            for (int k = 2; k <= P; k <<= 1)
+                for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                    for (int x = lane; x < P; x += 64) {
+                        const int y = x ^ jj;
+                        if (y > x) {
+                            const int u = keys[x], v = keys[y];
+                            const bool up = (x & k) == 0;
+                            if ((u > v) == up) {
+                                keys[x] = v;
+                                keys[y] = u;
+                                const double a = vre[x], b = vim[x];
+                                vre[x] = vre[y];
+                                vim[x] = vim[y];
+                                vre[y] = a;
+                                vim[y] = b;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            if (MODE == 2) {
+                // single pass: the sorted row goes to its staging slot (E <= CAP)
+                const int64_t base = i * (int64_t)CAP;
+                for (int x = lane; x < E; x += 64) {
+                    A.stage_col[base + x] = keys[x];
+                    A.stage_val[base + x] = make_double2(vre[x], vim[x]);
+                }
+                if (lane == 0) {
+                    A.nnz_out[i] = E;
+                    A.stage_ok[i] = 1;
+                    atomicMax(A.row_max, E);
+                }
+            } else {
+                const int64_t base = A.rowptr[i];
+                const int64_t len = A.rowptr[i + 1] - base;
+                if (len != E) {
+                    if (lane == 0) atomicAdd(A.err, 1);
+                } else {
+                    for (int x = lane; x < E; x += 64) {
+                        A.col[base + x] = keys[x];
+                        A.val[base + x] = make_double2(vre[x], vim[x]);
+                    }
+                }
+            }
         } else {
-            // compact (key, slot) pairs: lane owns a contiguous chunk of slots
+            // large hash: compact (key, slot) pairs: lane owns a contiguous chunk of slots
             constexpr int CH = CAP / 64;
             int c = 0;
             for (int x = lane * CH; x < lane * CH + CH; ++x)
@@ -397,7 +478,7 @@ constexpr int kSmall = 512, kSmaller = 256, kLarge = 4096, kSmallRows = 192;
 
 size_t table_smem(const Geo& g) {
     const int nphi = g.nphi, T = 2 * nphi - 1, np = g.nel * (g.nel - 1) / 2;
-    return sizeof(double) * 2 * nphi * nphi + sizeof(double2) * (nphi * nphi + T) + sizeof(unsigned short) * np + 16;
+    return sizeof(double) * 2 * nphi * nphi + sizeof(double2) * 2 * T + sizeof(unsigned short) * np + 16;
 }
 
 template <class T>
@@ -410,7 +491,7 @@ T* upload(DevBuf<T>& buf, const std::vector<T>& v, hipStream_t st) {
 template <int CAP, int MODE>
 void launch_build(const BuildArgs& A, int64_t rows, size_t smem, hipStream_t st, int ncu) {
     if (rows <= 0) return;
-    const int64_t blocks = std::min<int64_t>(rows, (int64_t)ncu * 16);
+    const int64_t blocks = std::min<int64_t>(rows, (int64_t)ncu * 32);  // rows are claimed by a counter
     BLQH_HIP(hipMemsetAsync(A.row_counter, 0, sizeof(int32_t), st));
     hipLaunchKernelGGL((k_build<CAP, MODE>), dim3((unsigned)blocks), dim3(64), smem, st, A);
     BLQH_HIP(hipGetLastError());

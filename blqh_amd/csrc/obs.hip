@@ -167,10 +167,9 @@ void blqh_observables_impl(const blqh_basis* B, const double* psi_host, blqh_obs
     BLQH_HIP(hipMemcpyAsync(psi.get(), psi_host, n * sizeof(double2), hipMemcpyHostToDevice, st));
     // FT phases of the sector
     const int f = g.nphi, K = B->sec.K;
-    std::vector<double2> ft((size_t)f * f);
-    for (int a = 0; a < f; ++a)
-        for (int b = 0; b < f; ++b)
-            ft[a * f + b] = make_double2(std::cos(2.0 * M_PI * (a - b) * K / f), std::sin(2.0 * M_PI * (a - b) * K / f));
+    std::vector<double2> ft(2 * (size_t)f - 1);  // indexed by ql - qr + f - 1
+    for (int dq = -f + 1; dq < f; ++dq)
+        ft[dq + f - 1] = make_double2(std::cos(2.0 * M_PI * dq * K / f), std::sin(2.0 * M_PI * dq * K / f));
     DevBuf<double2> dft(ft.size());
     BLQH_HIP(hipMemcpyAsync(dft.get(), ft.data(), ft.size() * sizeof(double2), hipMemcpyHostToDevice, st));
     std::vector<HopTerm> hops = sx_hops(g);

@@ -26,7 +26,7 @@ struct RowCtx {
     const int32_t* lut;
     const uint32_t* cnt;
     const double* C;     // [2][nphi][nphi] Coulomb table V[alpha][|dm|][|t|]
-    const double2* FT;   // [nphi][nphi] exp(2 pi i (ql - qr) K / nphi)
+    const double2* FT;   // [2 nphi - 1]: FT[ql - qr + nphi - 1] = exp(2 pi i (ql - qr) K / nphi)
     const double2* tw;   // [2 nphi - 1] twist phase per t
 };
 
@@ -103,7 +103,7 @@ BLQH_HD bool pair_term(const RowCtx& c, uint64_t L, int signl, int Dl, int ql, i
     dm = dm < 0 ? -dm : dm;
     const int at = t < 0 ? -t : t;
     const double Cv = c.C[alpha * nphi * nphi + dm * nphi + at];
-    double2 ft = c.FT[ql * nphi + qr];
+    double2 ft = c.FT[ql - qr + nphi - 1];
     const bool twisted = c.twist_sel == 1 ? (a != b) : (c.twist_sel == 2 && ((a == 0 && b == 3) || (a == 1 && b == 2)));
     if (twisted) ft = cmul(ft, c.tw[t + nphi - 1]);
     const double sq = sqrt((double)(Dl * Dr));
@@ -134,7 +134,7 @@ BLQH_HD bool hop_term(const RowCtx& c, uint64_t L, int signl, int Dl, int ql, co
     const int sign = sign1(L, h.lo, h.hi) * signl * sr;
     const int nphi = c.g.nphi;
     const double cs = h.coef * sign;
-    const double2 f = c.FT[ql * nphi + qr];
+    const double2 f = c.FT[ql - qr + nphi - 1];
     double2 v = make_double2(cs * f.x, cs * f.y);
     if (h.hasq) v = cmul(v, make_double2(h.qre, h.qim));
     const double sq = sqrt((double)(Dl * Dr));
