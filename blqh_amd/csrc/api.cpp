@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <list>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -103,6 +104,86 @@ Runtime& runtime() {
     return g_rt;
 }
 hipStream_t stream() { return runtime().stream; }
+
+namespace {
+struct BlockCache {
+    std::mutex mu;
+    std::list<std::pair<size_t, void*>> blocks;  // oldest first
+    size_t bytes = 0, cap = 0;
+};
+BlockCache& block_cache() {
+    static BlockCache* c = new BlockCache;  // never destroyed: no hipFree after the runtime shuts down
+    return *c;
+}
+void cache_drop_all(BlockCache& c) {  // caller holds c.mu
+    for (auto& b : c.blocks) (void)hipFree(b.second);
+    c.blocks.clear();
+    c.bytes = 0;
+}
+}  // namespace
+
+void* dev_alloc(size_t bytes) {
+    BlockCache& c = block_cache();
+    if (bytes >= kCacheMin) {
+        void* hit = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(c.mu);
+            for (auto it = c.blocks.begin(); it != c.blocks.end(); ++it)
+                if (it->first == bytes) {
+                    hit = it->second;
+                    c.bytes -= bytes;
+                    c.blocks.erase(it);
+                    break;
+                }
+        }
+        if (hit) {
+            BLQH_HIP(hipStreamSynchronize(stream()));
+            return hit;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> lk(c.mu);
+            cache_drop_all(c);
+        }
+        e = hipMalloc(&p, bytes);
+    }
+    if (e != hipSuccess)
+        throw Error(BLQH_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    return p;
+}
+
+void dev_free(void* p, size_t bytes) noexcept {
+    if (!p) return;
+    BlockCache& c = block_cache();
+    if (bytes >= kCacheMin && g_rt_ok) {
+        std::lock_guard<std::mutex> lk(c.mu);
+        if (c.cap == 0) {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) == hipSuccess) c.cap = tot / 4;
+        }
+        if (bytes <= c.cap) {
+            while (c.bytes + bytes > c.cap && !c.blocks.empty()) {
+                (void)hipFree(c.blocks.front().second);
+                c.bytes -= c.blocks.front().first;
+                c.blocks.pop_front();
+            }
+            c.blocks.emplace_back(bytes, p);
+            c.bytes += bytes;
+            return;
+        }
+    }
+    (void)hipFree(p);
+}
+
+size_t dev_cached_bytes() {
+    BlockCache& c = block_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    return c.bytes;
+}
 }  // namespace blqh
 
 extern "C" {

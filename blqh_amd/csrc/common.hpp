@@ -29,6 +29,19 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::blqh::Error((code), (msg)); \
     } while (0)
 
+// Device memory of DevBuf.  Blocks of kCacheMin bytes or more are kept after
+// release and handed out again for the same size: sweeps free and
+// re-allocate the same matrix sizes at every parameter point, and hipMalloc /
+// hipFree of large blocks (page mapping; hipFree also waits for the device)
+// cost more than the kernels that fill them.  The cache holds at most a
+// quarter of device memory (oldest blocks go first) and is emptied when an
+// allocation fails.  A reused block is handed out after the library stream
+// has drained, so no kernel queued before its release still reads it.
+constexpr size_t kCacheMin = size_t(1) << 20;
+void* dev_alloc(size_t bytes);                 // throws Error(BLQH_ENOMEM)
+void dev_free(void* p, size_t bytes) noexcept;
+size_t dev_cached_bytes();                     // free for the library: counted as free memory
+
 // Owning device allocation.
 template <class T>
 struct DevBuf {
@@ -47,16 +60,11 @@ struct DevBuf {
     void alloc(size_t count) {
         release();
         if (count == 0) return;
-        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            throw Error(BLQH_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " +
-                                         hipGetErrorString(e));
-        }
+        p = static_cast<T*>(dev_alloc(count * sizeof(T)));
         n = count;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) dev_free(p, n * sizeof(T));
         p = nullptr;
         n = 0;
     }

@@ -574,6 +574,7 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p_in, blqh_ha
     // count, scan, fill.
     size_t free_b = 0, total_b = 0;
     BLQH_HIP(hipMemGetInfo(&free_b, &total_b));
+    free_b += dev_cached_bytes();  // cached blocks are released if an allocation needs them
     const size_t stage_bytes = (size_t)B->n * cap * (sizeof(int32_t) + sizeof(double2)) + (size_t)B->n;
     const bool single = stage_bytes <= free_b / 4 && std::getenv("BLQH_BUILD_TWO_PASS") == nullptr;
     DevBuf<int32_t> scol;

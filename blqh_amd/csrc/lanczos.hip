@@ -1296,6 +1296,7 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
     // Krylov storage: keep all vectors when they fit comfortably
     size_t free_b = 0, total_b = 0;
     BLQH_HIP(hipMemGetInfo(&free_b, &total_b));
+    free_b += dev_cached_bytes();  // cached blocks are released if an allocation needs them
     const size_t need = (size_t)(cap + 1) * n * sizeof(double2);
     const bool skip = o.skip_states != 0;
     bool store = !skip && (o.store_basis > 0 || (o.store_basis < 0 && need < free_b / 2));
