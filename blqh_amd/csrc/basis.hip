@@ -55,6 +55,7 @@ RankSpec make_rank_spec(const blqh_sector& s, bool with_spin_constraint) {
 // entries bounded by the sector size, which is checked to fit.
 std::vector<uint32_t> make_rank_table(const RankSpec& s, int64_t* total) {
     const int64_t sz = (int64_t)(s.nbits + 1) * s.sP;
+    BLQH_REQUIRE(sz < (int64_t)INT32_MAX, BLQH_ERANGE, "rank table too large");  // rank_word indexes in 32 bits
     std::vector<int64_t> c(sz, 0);
     auto sat_add = [](int64_t x, int64_t y) { int64_t z = x + y; return z < x ? INT64_MAX : z; };
     c[rs_index(s, 0, 0, 0, 0, 0)] = 1;

@@ -137,21 +137,34 @@ BLQH_HD bool in_sector(const RankSpec& s, uint64_t w) {
     return true;
 }
 
-// rank of a sector word (precondition: in_sector(w)).  cnt is the table.
+// Highest set bit of x != 0.
+BLQH_HD int top_bit(uint64_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return 63 - __clzll((long long)x);
+#else
+    return 63 - __builtin_clzll(x);
+#endif
+}
+
+// rank of a sector word (precondition: in_sector(w)).  cnt is the table
+// (fewer than 2^31 entries).  The walk visits the set bits from the top, so
+// every word of the sector takes exactly nel steps (no lane of a wavefront
+// waits on another's empty orbitals); the table index is 32-bit.
 template <class T>
 BLQH_HD int64_t rank_word(const RankSpec& s, const T* cnt, uint64_t w) {
     int k = s.nel, r = s.Jt, a = s.nA, b = s.nB;
+    const int sP = (int)s.sP, sK = (int)s.sK, sR = (int)s.sR, sA = (int)s.sA, nphi = s.nphi;
     int64_t rk = 0;
-    int pm = (s.nbits - 1) % s.nphi;  // p mod nphi, stepped down with p
-    for (int p = s.nbits - 1; p >= 0 && k > 0; --p, pm = pm == 0 ? s.nphi - 1 : pm - 1) {
-        if (!((w >> p) & 1ULL)) continue;
-        rk += (int64_t)cnt[rs_index(s, p, k, r, a, b)];
-        --k;
+    for (; k > 0 && w; --k) {
+        const int p = top_bit(w);
+        w ^= 1ULL << p;
+        rk += (int64_t)cnt[p * sP + k * sK + r * sR + a * sA + b];
+        const int pm = p - (p >= nphi ? nphi : 0) - (p >= 2 * nphi ? nphi : 0) - (p >= 3 * nphi ? nphi : 0);
         r -= pm;
         if (r < 0) r += s.R;
         if (s.R == 1) r = 0;
-        if ((s.maskA >> p) & 1ULL) --a;
-        if ((s.maskB >> p) & 1ULL) --b;
+        a -= (int)((s.maskA >> p) & 1ULL);
+        b -= (int)((s.maskB >> p) & 1ULL);
     }
     return rk;
 }
