@@ -32,6 +32,23 @@ struct RowCtx {
 
 BLQH_HD double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
+// s mod d for 0 <= s < 2^20, d >= 1.  On the device the quotient comes from
+// the FP32 reciprocal (off by at most one for such s, corrected below)
+// instead of an integer division sequence.
+BLQH_HD int mod_small(int s, int d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const int q = (int)((float)s * __builtin_amdgcn_rcpf((float)d));
+    int r = s - q * d;
+    if (r < 0)
+        r += d;
+    else if (r >= d)
+        r -= d;
+    return r;
+#else
+    return s % d;
+#endif
+}
+
 // Survivor test of the reference's orbit reduction (basis.cpp:82-99): word c
 // at position pos(c) of the map's iteration order survives when no other
 // orbit member in the sector comes later.
@@ -70,7 +87,7 @@ BLQH_HD bool locate(const RowCtx& c, uint64_t R0, int ql, int& j, int& qr, int& 
         return true;
     }
     Dr = c.period[j];
-    qr = (ql + (e & 31)) % Dr;
+    qr = mod_small(ql + (e & 31), Dr);
     (void)inv_translate(c.g, R0, qr, sr);
     return true;
 }
