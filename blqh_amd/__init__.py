@@ -121,6 +121,7 @@ _SIGS = {
     "blqh_device_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64]),
     "blqh_device_free": (C.c_int, [C.c_void_p]),
     "blqh_synchronize": (C.c_int, []),
+    "blqh_release_cached": (C.c_int, [C.POINTER(C.c_int64)]),
     "blqh_profile_enable": (C.c_int, [C.c_int]),
     "blqh_profile_read": (C.c_int, [C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "blqh_profile_reset": (C.c_int, []),
@@ -140,6 +141,13 @@ def copy_calibration(nbytes, reps=5):
     finally:
         lib().blqh_device_free(a)
         lib().blqh_device_free(b)
+
+
+def release_cached():
+    """Return the device blocks the library keeps for reuse; bytes released."""
+    b = C.c_int64()
+    _check(lib().blqh_release_cached(C.byref(b)))
+    return b.value
 
 
 def profile_enable(on=True):

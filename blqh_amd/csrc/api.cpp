@@ -184,6 +184,14 @@ size_t dev_cached_bytes() {
     std::lock_guard<std::mutex> lk(c.mu);
     return c.bytes;
 }
+
+size_t dev_release_cached() {
+    BlockCache& c = block_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    const size_t b = c.bytes;
+    cache_drop_all(c);
+    return b;
+}
 }  // namespace blqh
 
 extern "C" {
@@ -511,6 +519,14 @@ int blqh_device_free(void* ptr) {
 
 int blqh_synchronize(void) {
     return guarded([&] { BLQH_HIP(hipStreamSynchronize(blqh::stream())); });
+}
+
+int blqh_release_cached(int64_t* bytes) {
+    return guarded([&] {
+        if (g_rt_ok) BLQH_HIP(hipStreamSynchronize(blqh::stream()));
+        const size_t b = blqh::dev_release_cached();
+        if (bytes) *bytes = (int64_t)b;
+    });
 }
 
 int blqh_profile_enable(int on) {

@@ -41,6 +41,7 @@ constexpr size_t kCacheMin = size_t(1) << 20;
 void* dev_alloc(size_t bytes);                 // throws Error(BLQH_ENOMEM)
 void dev_free(void* p, size_t bytes) noexcept;
 size_t dev_cached_bytes();                     // free for the library: counted as free memory
+size_t dev_release_cached();                   // hipFree every cached block; returns the bytes released
 
 // Owning device allocation.
 template <class T>

@@ -334,6 +334,14 @@ int blqh_device_copy(const void* src_dev, void* dst_dev, int64_t bytes);
 int blqh_device_alloc(void** ptr, int64_t bytes);
 int blqh_device_free(void* ptr);
 int blqh_synchronize(void);
+/* Release the device blocks the library keeps for reuse.  Engine memory
+ * management, no reference counterpart: blocks of 1 MiB or more freed by the
+ * library (matrices, panels, Krylov blocks of destroyed handles) are kept,
+ * up to a quarter of device memory, and reused for allocations of the same
+ * size; they are also released by the library itself when an allocation
+ * fails.  Callers sharing the device with other allocators call this to
+ * return the memory.  *bytes (may be NULL) receives the amount released. */
+int blqh_release_cached(int64_t* bytes);
 int blqh_profile_enable(int on);
 int blqh_profile_read(int64_t* launches, double* kernel_ms, double* bytes);
 int blqh_profile_reset(void);
