@@ -682,40 +682,50 @@ __global__ void k_combine_count(int64_t n, const int64_t* __restrict__ brp, cons
     }
 }
 
-__global__ void k_combine_fill(int64_t n, const int64_t* __restrict__ brp, const int32_t* __restrict__ bcol,
-                               const double2* __restrict__ bval, const int64_t* __restrict__ trp,
-                               const int32_t* __restrict__ tcol, const double2* __restrict__ tval, double dsas,
-                               const int64_t* __restrict__ rp, int32_t* __restrict__ col, double2* __restrict__ val,
-                               int32_t* __restrict__ err) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t a = brp[i], ae = brp[i + 1], b = trp[i], be = trp[i + 1], o = rp[i];
-        const int64_t oe = rp[i + 1];
-        while (a < ae || b < be) {
-            if (b < be) {
-                const double2 t = make_double2(dsas * tval[b].x, dsas * tval[b].y);
-                if (!(hypot(t.x, t.y) > 1e-10)) {
-                    ++b;
-                    continue;
-                }
-                if (a >= ae || tcol[b] < bcol[a]) {
-                    if (o < oe) {
-                        col[o] = tcol[b];
-                        val[o] = t;
-                    }
-                    ++o;
-                    ++b;
-                    continue;
-                }
-                if (tcol[b] == bcol[a]) atomicAdd(err, 1);  // shared entry: not a pure tunnelling term
-            }
-            if (o < oe) {
-                col[o] = bcol[a];
-                val[o] = bval[a];
-            }
-            ++o;
-            ++a;
+// One wavefront per row.  The output row is the sorted union of the two
+// disjoint rows, so a base entry moves right by the kept T entries of
+// smaller column and a kept T entry lands after the kept T entries before it
+// and the base entries of smaller column (binary search); every entry is
+// placed independently.  A column present in both rows is reported in err.
+__global__ void __launch_bounds__(256) k_combine_fill(int64_t n, const int64_t* __restrict__ brp,
+                                                      const int32_t* __restrict__ bcol,
+                                                      const double2* __restrict__ bval,
+                                                      const int64_t* __restrict__ trp,
+                                                      const int32_t* __restrict__ tcol,
+                                                      const double2* __restrict__ tval, double dsas,
+                                                      const int64_t* __restrict__ rp, int32_t* __restrict__ col,
+                                                      double2* __restrict__ val, int32_t* __restrict__ err) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += nw) {
+        const int64_t a0 = brp[i], na = brp[i + 1] - a0, b0 = trp[i], nb = trp[i + 1] - b0, o0 = rp[i];
+        auto kept = [&](int64_t k) { return hypot(dsas * tval[k].x, dsas * tval[k].y) > 1e-10; };
+        int64_t placed = 0;  // entries this lane wrote (the row's total is checked below)
+        for (int64_t k = lane; k < na; k += 64) {
+            const int32_t c = bcol[a0 + k];
+            int64_t shift = 0;
+            for (int64_t q = 0; q < nb; ++q) shift += (tcol[b0 + q] < c && kept(b0 + q)) ? 1 : 0;
+            col[o0 + k + shift] = c;
+            val[o0 + k + shift] = bval[a0 + k];
+            ++placed;
         }
-        if (o != oe) atomicAdd(err, 1);
+        for (int64_t q = lane; q < nb; q += 64) {
+            if (!kept(b0 + q)) continue;
+            int64_t before = 0;
+            for (int64_t r = 0; r < q; ++r) before += kept(b0 + r) ? 1 : 0;
+            const int32_t c = tcol[b0 + q];
+            int64_t lo = 0, hi = na;  // base entries with column < c
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (bcol[a0 + mid] < c) lo = mid + 1; else hi = mid;
+            }
+            if (lo < na && bcol[a0 + lo] == c) atomicAdd(err, 1);  // shared entry: not a pure tunnelling term
+            col[o0 + before + lo] = c;
+            val[o0 + before + lo] = make_double2(dsas * tval[b0 + q].x, dsas * tval[b0 + q].y);
+            ++placed;
+        }
+        for (int o = 32; o > 0; o >>= 1) placed += __shfl_xor(placed, o, 64);
+        if (lane == 0 && placed != rp[i + 1] - o0) atomicAdd(err, 1);
     }
 }
 
@@ -752,7 +762,8 @@ void blqh_hamil_combine_impl(const blqh_hamil* base, const blqh_hamil* tun, doub
     BLQH_HIP(hipStreamSynchronize(st));
     H->col.alloc(std::max<int64_t>(H->nnz, 1));
     H->val.alloc(std::max<int64_t>(H->nnz, 1));
-    hipLaunchKernelGGL(k_combine_fill, dim3(g), dim3(256), 0, st, n, base->rowptr.get(), base->col.get(),
+    const int gw = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 4), 1), 65536);  // four rows per block
+    hipLaunchKernelGGL(k_combine_fill, dim3(gw), dim3(256), 0, st, n, base->rowptr.get(), base->col.get(),
                        base->val.get(), tun->rowptr.get(), tun->col.get(), tun->val.get(), dsas, H->rowptr.get(),
                        H->col.get(), H->val.get(), err.get());
     BLQH_HIP(hipGetLastError());
