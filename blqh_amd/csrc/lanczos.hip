@@ -1089,21 +1089,71 @@ void autotune(blqh_hamil* H) {
 // ---- host: start vector and tridiagonal eigensolver ------------------------------
 namespace blqh {
 
-// std::mt19937(seed); entry i takes its imaginary part from draw 2i and its
-// real part from draw 2i+1 (the evaluation order g++ gives to the two rng()
-// calls in the reference's complex<double>(re, im) constructor), each
-// draw/max - 0.5; normalised by the running sum of |z|^2.
-void start_vector(int64_t n, uint32_t seed, std::vector<double2>& v) {
-    std::mt19937 rng(seed);
-    v.resize(n);
+namespace {
+// The std::mt19937 sequence (Matsumoto & Nishimura's MT19937: 624-word
+// state, the standard's default seeding and tempering) produced a whole
+// state at a time: the twist and the tempering are plain loops over the
+// block, several times faster than one call per draw; same words.
+struct Mt19937Blocks {
+    uint32_t mt[624];
+    explicit Mt19937Blocks(uint32_t seed) {
+        mt[0] = seed;
+        for (uint32_t i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + i;
+    }
+    // the next 624 draws
+    void next(uint32_t* out) {
+        constexpr uint32_t kUp = 0x80000000u, kLo = 0x7fffffffu, kA = 0x9908b0dfu;
+        int i = 0;
+        for (; i < 227; ++i) {
+            const uint32_t y = (mt[i] & kUp) | (mt[i + 1] & kLo);
+            mt[i] = mt[i + 397] ^ (y >> 1) ^ ((y & 1u) ? kA : 0u);
+        }
+        for (; i < 623; ++i) {
+            const uint32_t y = (mt[i] & kUp) | (mt[i + 1] & kLo);
+            mt[i] = mt[i - 227] ^ (y >> 1) ^ ((y & 1u) ? kA : 0u);
+        }
+        const uint32_t y = (mt[623] & kUp) | (mt[0] & kLo);
+        mt[623] = mt[396] ^ (y >> 1) ^ ((y & 1u) ? kA : 0u);
+        for (int k = 0; k < 624; ++k) {
+            uint32_t z = mt[k];
+            z ^= z >> 11;
+            z ^= (z << 7) & 0x9d2c5680u;
+            z ^= (z << 15) & 0xefc60000u;
+            z ^= z >> 18;
+            out[k] = z;
+        }
+    }
+};
+}  // namespace
+
+// The reference's start vector before normalisation: entry i takes its
+// imaginary part from draw 2i and its real part from draw 2i+1 of
+// std::mt19937(seed) (the evaluation order g++ gives to the two rng() calls
+// in the reference's complex<double>(re, im) constructor), each
+// draw/max - 0.5.  Returns the running sum of |z_i|^2 in entry order.
+double start_entries(int64_t n, uint32_t seed, double2* v) {
+    Mt19937Blocks rng(seed);
+    uint32_t raw[624];
+    int used = 624;
     double nrm = 0;
     for (int64_t i = 0; i < n; ++i) {
-        const double im = rng() * 1.0 / rng.max() - 0.5;
-        const double re = rng() * 1.0 / rng.max() - 0.5;
+        if (used == 624) {  // 624 is even: an entry's two draws share a block
+            rng.next(raw);
+            used = 0;
+        }
+        const double im = raw[used] * 1.0 / 4294967295u - 0.5;
+        const double re = raw[used + 1] * 1.0 / 4294967295u - 0.5;
+        used += 2;
         v[i] = make_double2(re, im);
         nrm += re * re + im * im;
     }
-    nrm = std::sqrt(nrm);
+    return nrm;
+}
+
+// start_entries normalised by the square root of their running sum.
+void start_vector(int64_t n, uint32_t seed, std::vector<double2>& v) {
+    v.resize(n);
+    const double nrm = std::sqrt(start_entries(n, seed, v.data()));
     for (int64_t i = 0; i < n; ++i) { v[i].x /= nrm; v[i].y /= nrm; }
 }
 
@@ -1265,13 +1315,7 @@ void device_start(int64_t n, uint32_t seed, const double* start, double2* dst, h
         }
         BLQH_REQUIRE(s > 0 && std::isfinite(s), BLQH_EINVAL, "start vector is zero or not finite");
     } else {
-        std::mt19937 rng(seed);
-        for (int64_t i = 0; i < n; ++i) {
-            const double im = rng() * 1.0 / rng.max() - 0.5;
-            const double re = rng() * 1.0 / rng.max() - 0.5;
-            g_stage[i] = make_double2(re, im);
-            s += re * re + im * im;
-        }
+        s = blqh::start_entries(n, seed, g_stage);
     }
     s = std::sqrt(s);
     BLQH_HIP(hipMemcpyAsync(dst, g_stage, n * sizeof(double2), hipMemcpyHostToDevice, st));

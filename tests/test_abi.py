@@ -27,7 +27,8 @@ def test_library_exports_declared_symbols():
 
 
 def test_start_vector_matches_oracle(oracle_mod):
-    for n, seed in ((1, 0), (79, 1), (1000, 4242)):
+    # 100003 entries: 200006 draws, 321 state blocks of the generator
+    for n, seed in ((1, 0), (79, 1), (1000, 4242), (100003, 5)):
         np.testing.assert_array_equal(blqh_amd.start_vector(n, seed), oracle_mod.start_vector(n, seed))
 
 
