@@ -41,6 +41,7 @@ using namespace blqh;
 
 constexpr int kItemRows = 256;  // rows per work item
 constexpr int kBT = 256;        // threads per workgroup
+constexpr int kBTj = 512;       // threads per workgroup of k_b1j: two wavefronts per 64-row slice
 
 // device descriptor of one problem
 struct BProb {
@@ -74,13 +75,14 @@ struct BState {
     double* part2;             // [nitems]
 };
 
+template <int NT = kBT>
 __device__ __forceinline__ double block_sum(double v, double* sh) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
     double t = 0;
-    for (int w = 0; w < kBT / 64; ++w) t += sh[w];
+    for (int w = 0; w < NT / 64; ++w) t += sh[w];
     return t;
 }
 
@@ -183,18 +185,22 @@ __global__ void __launch_bounds__(kBT) k_b1(BState S, int jrel, const double2* _
     }
 }
 
-// Same contract as k_b1 over panels in step-major order (jag_row_sum): wave w
-// of the workgroup owns the item's w-th 64-row slice, one lane per row.
-// KIND 0 / 1 walk only the members with real / complex panel values, KIND 2
-// walks all members and branches per work item (one launch per step for a
-// mixed batch, so that the members of both kinds run side by side).
+// Same contract as k_b1 over panels in step-major order: two wavefronts per
+// 64-row slice of the item (waves 2s and 2s+1 own slice s, one lane per row),
+// each summing alternate step groups (jag_row_sum_half); the odd wave's
+// partial sums go through LDS to the even one, which adds them and runs the
+// K1 epilogue.  KIND 0 / 1 walk only the members with real / complex panel
+// values, KIND 2 walks all members and branches per work item (one launch per
+// step for a mixed batch, so that the members of both kinds run side by side).
 template <bool LZ, int KIND>
-__global__ void __launch_bounds__(kBT) k_b1j(BState S, int jrel, const double2* __restrict__ xs,
-                                             double2* __restrict__ ys, double* __restrict__ part) {
-    __shared__ double sh[kBT / 64];
+__global__ void __launch_bounds__(kBTj) k_b1j(BState S, int jrel, const double2* __restrict__ xs,
+                                              double2* __restrict__ ys, double* __restrict__ part) {
+    __shared__ double sh[kBTj / 64];
+    __shared__ double2 odd[kBTj / 128][64];  // partial row sums of the odd waves
     const int j = LZ ? *S.jbase + jrel : 0;
     if (LZ && j >= S.cap) return;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int sl = wv >> 1, par = wv & 1;
     // Workgroups go to the XCDs round robin (workgroup b to XCD b mod 8, grid
     // a multiple of 8); each XCD walks its own list of whole problems, so the
     // x vectors it gathers from are those of the few problems its 32 CUs are
@@ -221,12 +227,11 @@ __global__ void __launch_bounds__(kBT) k_b1j(BState S, int jrel, const double2* 
             x = xs + P.voff;
             y = ys + P.voff;
         }
-        const int64_t row0 = (it - P.item0) * kItemRows + wv * 64;
-        double acc = 0.0;
-        if (row0 < P.n) {  // uniform over the wavefront
-            const int64_t n = P.n, myrow = row0 + lane;
+        const int64_t n = P.n;
+        const int64_t row0 = (it - P.item0) * kItemRows + sl * 64, myrow = row0 + lane;
+        double ar = 0.0, ai = 0.0;
+        if (row0 < n) {  // uniform over the wavefront
             const int64_t rend = row0 + 64 < n ? row0 + 64 : n;
-            double ar = 0.0, ai = 0.0;
             for (int pan = 0; pan < P.npan; ++pan) {
                 const uint32_t* rp = P.prow + (int64_t)pan * (n + 1);
                 const uint32_t s = myrow < n ? rp[myrow] : rp[rend];
@@ -234,27 +239,33 @@ __global__ void __launch_bounds__(kBT) k_b1j(BState S, int jrel, const double2* 
                 const int64_t run = P.pbase[pan] + (uint32_t)__builtin_amdgcn_readfirstlane((int)s);  // rp[row0]
                 const double2* xp = x + (int64_t)pan * kPanelWidth;
                 if (KIND == 2 ? P.real != 0 : KIND == 0)
-                    jag_row_sum<true>(P.pcol + run, P.pvr + run, P.pvc + run, xp, e - s, ar, ai);
+                    jag_row_sum_half<true>(P.pcol + run, P.pvr + run, P.pvc + run, xp, e - s, par, ar, ai);
                 else
-                    jag_row_sum<false>(P.pcol + run, P.pvr + run, P.pvc + run, xp, e - s, ar, ai);
-            }
-            if (myrow < n) {
-                double ur = ar, ui = ai;
-                if (LZ) {
-                    ur = ib * ar;
-                    ui = ib * ai;
-                    if (j > 0) {
-                        const double2 q = Wp[myrow];
-                        ur -= cp * q.x;
-                        ui -= cp * q.y;
-                    }
-                }
-                y[myrow] = make_double2(ur, ui);
-                const double2 w = x[myrow];
-                acc = w.x * ur + w.y * ui;
+                    jag_row_sum_half<false>(P.pcol + run, P.pvr + run, P.pvc + run, xp, e - s, par, ar, ai);
             }
         }
-        const double t = block_sum(acc, sh);
+        if (par) odd[sl][lane] = make_double2(ar, ai);
+        __syncthreads();
+        double acc = 0.0;
+        if (!par && myrow < n) {
+            const double2 o = odd[sl][lane];
+            ar += o.x;
+            ai += o.y;
+            double ur = ar, ui = ai;
+            if (LZ) {
+                ur = ib * ar;
+                ui = ib * ai;
+                if (j > 0) {
+                    const double2 q = Wp[myrow];
+                    ur -= cp * q.x;
+                    ui -= cp * q.y;
+                }
+            }
+            y[myrow] = make_double2(ur, ui);
+            const double2 w = x[myrow];
+            acc = w.x * ur + w.y * ui;
+        }
+        const double t = block_sum<kBTj>(acc, sh);
         if (threadIdx.x == 0) part[it] = t;
     }
 }
@@ -486,15 +497,15 @@ void refresh_probs(blqh_batch* b) {
 template <bool LZ>
 void launch_b1(const blqh_batch* b, const BState& S, int jrel, const double2* xs, double2* ys, double* part,
                hipStream_t st) {
-    const dim3 g(b->grid), t(kBT);
+    const dim3 g(b->grid), t(kBT), tj(kBTj);
     switch (b->lpr) {
         case 64:
             if (b->has_real && b->has_complex) {  // measured faster than one launch per kind
-                hipLaunchKernelGGL((k_b1j<LZ, 2>), g, t, 0, st, S, jrel, xs, ys, part);
+                hipLaunchKernelGGL((k_b1j<LZ, 2>), g, tj, 0, st, S, jrel, xs, ys, part);
                 break;
             }
-            if (b->has_real) hipLaunchKernelGGL((k_b1j<LZ, 0>), g, t, 0, st, S, jrel, xs, ys, part);
-            if (b->has_complex) hipLaunchKernelGGL((k_b1j<LZ, 1>), g, t, 0, st, S, jrel, xs, ys, part);
+            if (b->has_real) hipLaunchKernelGGL((k_b1j<LZ, 0>), g, tj, 0, st, S, jrel, xs, ys, part);
+            if (b->has_complex) hipLaunchKernelGGL((k_b1j<LZ, 1>), g, tj, 0, st, S, jrel, xs, ys, part);
             break;
         case 4: hipLaunchKernelGGL((k_b1<4, LZ>), g, t, 0, st, S, jrel, xs, ys, part); break;
         case 8: hipLaunchKernelGGL((k_b1<8, LZ>), g, t, 0, st, S, jrel, xs, ys, part); break;

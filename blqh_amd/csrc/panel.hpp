@@ -157,6 +157,72 @@ __device__ __forceinline__ uint32_t jag_row_sum(const uint16_t* __restrict__ cr,
     return off;  // entries of the whole run (wave-uniform)
 }
 
+// jag_row_sum split over two wavefronts: the wave of parity `par` sums the
+// step groups g (four steps each) with g mod 2 == par and only counts the
+// entries of the others, so each lane's chain of dependent loads is half as
+// long (a step's cost when few rows are live is that chain).  The row sum is
+// the two waves' partial sums added by the caller, in parity order.
+template <bool REAL>
+__device__ __forceinline__ void jag_row_sum_half(const uint16_t* __restrict__ cr, const double* __restrict__ vr,
+                                                 const double2* __restrict__ vc, const double2* __restrict__ xp,
+                                                 uint32_t len, int par, double& ar, double& ai) {
+    typedef double d2v_t __attribute__((ext_vector_type(2)));
+    uint32_t off = 0;
+    auto issue = [&](uint32_t t, int* c, double2* v, bool& any) {
+        uint64_t m[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) m[u] = __ballot(t + u < len);
+        any = m[0] != 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t q = off + lanes_below(m[u]);
+            off += (uint32_t)__popcll(m[u]);
+            const bool act = t + u < len;
+            c[u] = act ? (int)__builtin_nontemporal_load(&cr[q]) : 0;
+            if (REAL) {
+                v[u] = make_double2(act ? __builtin_nontemporal_load(&vr[q]) : 0.0, 0.0);
+            } else if (act) {
+                const d2v_t w = __builtin_nontemporal_load(reinterpret_cast<const d2v_t*>(&vc[q]));
+                v[u] = make_double2(w.x, w.y);
+            } else {
+                v[u] = make_double2(0.0, 0.0);
+            }
+        }
+    };
+    auto skip = [&](uint32_t t) {  // the other wave's group: advance the offset only
+#pragma unroll
+        for (int u = 0; u < 4; ++u) off += (uint32_t)__popcll(__ballot(t + u < len));
+    };
+    int c0[4], c1[4];
+    double2 v0[4], v1[4];
+    bool any0, any1;
+    if (par) skip(0);
+    issue(4u * par, c0, v0, any0);
+    for (uint32_t t = 4u * par + 8u; any0; t += 8u) {
+        skip(t - 4u);
+        issue(t, c1, v1, any1);  // next own group in flight
+        double2 xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[u] = xp[c0[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (REAL) {
+                ar += v0[u].x * xv[u].x;
+                ai += v0[u].x * xv[u].y;
+            } else {
+                ar += v0[u].x * xv[u].x - v0[u].y * xv[u].y;
+                ai += v0[u].x * xv[u].y + v0[u].y * xv[u].x;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c0[u] = c1[u];
+            v0[u] = v1[u];
+        }
+        any0 = any1;
+    }
+}
+
 // The same over a lane stream that covers R rows one after the other (SpMV
 // kind 5, panel runs with R rows per lane): cum[q] = the lane's entries in
 // rows 0..q of its stream.  The running sum is added to acc[q * 64] (this
