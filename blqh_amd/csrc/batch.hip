@@ -41,7 +41,8 @@ using namespace blqh;
 
 constexpr int kItemRows = 256;  // rows per work item
 constexpr int kBT = 256;        // threads per workgroup
-constexpr int kBTj = 512;       // threads per workgroup of k_b1j: two wavefronts per 64-row slice
+constexpr int kSplit = 2;       // k_b1j: wavefronts per 64-row slice (measured: 2 beats 1 and 4)
+constexpr int kBTj = kSplit * 256;  // threads per workgroup of k_b1j (an item's four slices)
 
 // device descriptor of one problem
 struct BProb {
@@ -185,22 +186,22 @@ __global__ void __launch_bounds__(kBT) k_b1(BState S, int jrel, const double2* _
     }
 }
 
-// Same contract as k_b1 over panels in step-major order: two wavefronts per
-// 64-row slice of the item (waves 2s and 2s+1 own slice s, one lane per row),
-// each summing alternate step groups (jag_row_sum_half); the odd wave's
-// partial sums go through LDS to the even one, which adds them and runs the
-// K1 epilogue.  KIND 0 / 1 walk only the members with real / complex panel
+// Same contract as k_b1 over panels in step-major order: kSplit wavefronts
+// per 64-row slice of the item (waves kSplit*s .. kSplit*s + kSplit-1 own
+// slice s, one lane per row), each summing every kSplit-th step group
+// (jag_row_sum_part); the partial sums of waves 1.. go through LDS to wave 0
+// of the slice, which adds them in wave order and runs the K1 epilogue.  KIND 0 / 1 walk only the members with real / complex panel
 // values, KIND 2 walks all members and branches per work item (one launch per
 // step for a mixed batch, so that the members of both kinds run side by side).
 template <bool LZ, int KIND>
 __global__ void __launch_bounds__(kBTj) k_b1j(BState S, int jrel, const double2* __restrict__ xs,
                                               double2* __restrict__ ys, double* __restrict__ part) {
     __shared__ double sh[kBTj / 64];
-    __shared__ double2 odd[kBTj / 128][64];  // partial row sums of the odd waves
+    __shared__ double2 red[kItemRows / 64][kSplit - 1][64];  // partial row sums of waves 1.. of each slice
     const int j = LZ ? *S.jbase + jrel : 0;
     if (LZ && j >= S.cap) return;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int sl = wv >> 1, par = wv & 1;
+    const int sl = wv / kSplit, par = wv % kSplit;
     // Workgroups go to the XCDs round robin (workgroup b to XCD b mod 8, grid
     // a multiple of 8); each XCD walks its own list of whole problems, so the
     // x vectors it gathers from are those of the few problems its 32 CUs are
@@ -239,18 +240,21 @@ __global__ void __launch_bounds__(kBTj) k_b1j(BState S, int jrel, const double2*
                 const int64_t run = P.pbase[pan] + (uint32_t)__builtin_amdgcn_readfirstlane((int)s);  // rp[row0]
                 const double2* xp = x + (int64_t)pan * kPanelWidth;
                 if (KIND == 2 ? P.real != 0 : KIND == 0)
-                    jag_row_sum_half<true>(P.pcol + run, P.pvr + run, P.pvc + run, xp, e - s, par, ar, ai);
+                    jag_row_sum_part<true, kSplit>(P.pcol + run, P.pvr + run, P.pvc + run, xp, e - s, par, ar, ai);
                 else
-                    jag_row_sum_half<false>(P.pcol + run, P.pvr + run, P.pvc + run, xp, e - s, par, ar, ai);
+                    jag_row_sum_part<false, kSplit>(P.pcol + run, P.pvr + run, P.pvc + run, xp, e - s, par, ar, ai);
             }
         }
-        if (par) odd[sl][lane] = make_double2(ar, ai);
+        if (par) red[sl][par - 1][lane] = make_double2(ar, ai);
         __syncthreads();
         double acc = 0.0;
         if (!par && myrow < n) {
-            const double2 o = odd[sl][lane];
-            ar += o.x;
-            ai += o.y;
+#pragma unroll
+            for (int q = 0; q < kSplit - 1; ++q) {
+                const double2 o = red[sl][q][lane];
+                ar += o.x;
+                ai += o.y;
+            }
             double ur = ar, ui = ai;
             if (LZ) {
                 ur = ib * ar;

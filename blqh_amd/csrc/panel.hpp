@@ -157,13 +157,13 @@ __device__ __forceinline__ uint32_t jag_row_sum(const uint16_t* __restrict__ cr,
     return off;  // entries of the whole run (wave-uniform)
 }
 
-// jag_row_sum split over two wavefronts: the wave of parity `par` sums the
-// step groups g (four steps each) with g mod 2 == par and only counts the
-// entries of the others, so each lane's chain of dependent loads is half as
-// long (a step's cost when few rows are live is that chain).  The row sum is
-// the two waves' partial sums added by the caller, in parity order.
-template <bool REAL>
-__device__ __forceinline__ void jag_row_sum_half(const uint16_t* __restrict__ cr, const double* __restrict__ vr,
+// jag_row_sum split over NP wavefronts: wave `par` (0 <= par < NP) sums the
+// step groups g (four steps each) with g mod NP == par and only counts the
+// entries of the others, so each lane's chain of dependent loads is 1/NP as
+// long.  The row sum is the NP waves' partial sums added by the caller in
+// wave order.
+template <bool REAL, int NP>
+__device__ __forceinline__ void jag_row_sum_part(const uint16_t* __restrict__ cr, const double* __restrict__ vr,
                                                  const double2* __restrict__ vc, const double2* __restrict__ xp,
                                                  uint32_t len, int par, double& ar, double& ai) {
     typedef double d2v_t __attribute__((ext_vector_type(2)));
@@ -189,17 +189,18 @@ __device__ __forceinline__ void jag_row_sum_half(const uint16_t* __restrict__ cr
             }
         }
     };
-    auto skip = [&](uint32_t t) {  // the other wave's group: advance the offset only
+    auto skip = [&](uint32_t t) {  // another wave's group: advance the offset only
 #pragma unroll
         for (int u = 0; u < 4; ++u) off += (uint32_t)__popcll(__ballot(t + u < len));
     };
     int c0[4], c1[4];
     double2 v0[4], v1[4];
     bool any0, any1;
-    if (par) skip(0);
+    for (int g = 0; g < par; ++g) skip(4u * g);
     issue(4u * par, c0, v0, any0);
-    for (uint32_t t = 4u * par + 8u; any0; t += 8u) {
-        skip(t - 4u);
+    for (uint32_t t = 4u * par + 4u * NP; any0; t += 4u * NP) {
+#pragma unroll
+        for (int g = NP - 1; g > 0; --g) skip(t - 4u * g);
         issue(t, c1, v1, any1);  // next own group in flight
         double2 xv[4];
 #pragma unroll
