@@ -302,7 +302,9 @@ def run_sweep(args):
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_b1<LZ=true> (batched fused SpMV + Lanczos K1)",
+            "kernel": ("k_b1j<LZ=true> (batched fused SpMV + Lanczos K1, step-major panels, two wavefronts per "
+                       "64-row slice)" if batch.lanes == 64 else
+                       f"k_b1<{batch.lanes}, LZ=true> (batched fused SpMV + Lanczos K1, row-major panels)"),
             "avg_launch_ms": avg_ms,
             "launches": launches,
             "algorithmic_bytes_per_launch": kbytes / max(launches, 1),
