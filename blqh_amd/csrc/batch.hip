@@ -29,6 +29,7 @@
 #include <thread>
 
 #include "engine.hpp"
+#include "xcdplan.hpp"
 
 namespace blqh {
 void start_vector(int64_t n, uint32_t seed, std::vector<double2>& v);
@@ -413,61 +414,14 @@ struct blqh_batch {
 
 namespace {
 
-// Work lists of k_b1j, per value kind.  A member whose x fits one panel
-// (at most 65,536 rows, 1 MiB) is cut into chunks of consecutive items:
-// whole members when there are many, so that an XCD gathers from the x
-// vectors of the few members it is working on, which its L2 then holds; at
-// least 64 chunks in all, so that a few members still spread over every XCD.
-// The chunks are dealt largest first, each to the XCD with the fewest
-// nonzeros so far.  The items of a larger member are dealt one by one, item i
-// to XCD i mod 8, which makes all XCDs walk the member together (measured
-// faster for 204k-state members than chunks).  A list holds its items in
-// item order.
+// Work lists of k_b1j (xcd_lists), uploaded when they change.
 void build_xcd_lists(blqh_batch* b) {
     const int P = (int)b->hs.size();
-    std::vector<int32_t> items, seg(27, 0);
-    items.reserve(2 * b->nitems);
-    struct Chunk { int64_t i0, i1; double w; };
-    for (int kind = 0; kind < 3; ++kind) {
-        auto mine = [&](int p) { return kind == 2 || (b->hs[p]->pan.real ? 0 : 1) == kind; };
-        auto per_item = [&](int p) { return (double)b->hs[p]->nnz / (double)(b->item0[p + 1] - b->item0[p]); };
-        std::vector<double> load(8, 0.0);
-        std::vector<std::vector<Chunk>> own(8);
-        int64_t small_items = 0;
-        for (int p = 0; p < P; ++p) {
-            if (!mine(p)) continue;
-            if (b->hs[p]->pan.npan == 1) {
-                small_items += b->item0[p + 1] - b->item0[p];
-                continue;
-            }
-            for (int64_t it = b->item0[p]; it < b->item0[p + 1]; ++it) {
-                own[it & 7].push_back({it, it + 1, per_item(p)});
-                load[it & 7] += per_item(p);
-            }
-        }
-        const int64_t cmax = std::max<int64_t>(1, ceil_div(small_items, 64));
-        std::vector<Chunk> ch;
-        for (int p = 0; p < P; ++p) {
-            if (!mine(p) || b->hs[p]->pan.npan != 1) continue;
-            for (int64_t i0 = b->item0[p]; i0 < b->item0[p + 1]; i0 += cmax) {
-                const int64_t i1 = std::min(i0 + cmax, b->item0[p + 1]);
-                ch.push_back({i0, i1, per_item(p) * (double)(i1 - i0)});
-            }
-        }
-        std::stable_sort(ch.begin(), ch.end(), [](const Chunk& a, const Chunk& c) { return a.w > c.w; });
-        for (const Chunk& c : ch) {
-            const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-            load[x] += c.w;
-            own[x].push_back(c);
-        }
-        for (int x = 0; x < 8; ++x) {
-            std::sort(own[x].begin(), own[x].end(), [](const Chunk& a, const Chunk& c) { return a.i0 < c.i0; });
-            seg[kind * 9 + x] = (int32_t)items.size();
-            for (const Chunk& c : own[x])
-                for (int64_t it = c.i0; it < c.i1; ++it) items.push_back((int32_t)it);
-        }
-        seg[kind * 9 + 8] = (int32_t)items.size();
-    }
+    std::vector<XcdMember> mem(P);
+    for (int p = 0; p < P; ++p)
+        mem[p] = {b->item0[p], b->item0[p + 1], b->hs[p]->nnz, b->hs[p]->pan.npan, b->hs[p]->pan.real ? 1 : 0};
+    std::vector<int32_t> items, seg;
+    xcd_lists(mem, items, seg);
     if (items == b->h_xitems && seg == b->h_xseg) return;
     b->h_xitems.swap(items);
     b->h_xseg.swap(seg);

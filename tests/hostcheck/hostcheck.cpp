@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../blqh_amd/csrc/engine.hpp"
+#include "../../blqh_amd/csrc/xcdplan.hpp"
 
 using namespace blqh;
 
@@ -103,6 +104,18 @@ uint64_t hc_inv_translate(int S, int nphi, int nel, uint64_t c, int k, int* sign
     return inv_translate(Geo{S, nphi, nel}, c, k, *sign);
 }
 int hc_sign2(uint64_t c, int n, int m, int nt, int mt) { return sign2(c, n, m, nt, mt); }
+
+// Per-XCD work lists of the batched K1 for P members (member p owns items
+// item0[p] .. item0[p+1]); items receives 2 * item0[P] entries, seg 27.
+void hc_xcd_lists(int P, const int64_t* item0, const int64_t* nnz, const int32_t* npan, const int32_t* real,
+                  int32_t* items, int32_t* seg) {
+    std::vector<XcdMember> mem(P);
+    for (int p = 0; p < P; ++p) mem[p] = {item0[p], item0[p + 1], nnz[p], npan[p], real[p]};
+    std::vector<int32_t> it, sg;
+    xcd_lists(mem, it, sg);
+    std::memcpy(items, it.data(), it.size() * 4);
+    std::memcpy(seg, sg.data(), sg.size() * 4);
+}
 int hc_sign1(uint64_t c, int n, int nt) { return sign1(c, n, nt); }
 
 // Full Hamiltonian through the mirror.  Returns nnz (or -nnz when cap is too small).

@@ -22,6 +22,8 @@ def lib():
                                 C.c_void_p, C.c_int64]
         _L.hc_sector_words.restype = C.c_int64
         _L.hc_sector_words.argtypes = [C.POINTER(blqh_amd.Sector), C.c_int, C.c_void_p, C.c_int64]
+        _L.hc_xcd_lists.restype = None
+        _L.hc_xcd_lists.argtypes = [C.c_int] + [C.c_void_p] * 6
     return _L
 
 
@@ -53,3 +55,15 @@ def sector_words(sec, with_spin=True):
     m = lib().hc_sector_words(C.byref(sec), 1 if with_spin else 0, _p(out), cap)
     assert m >= 0, "rank/unrank inconsistency"
     return out[:m].copy()
+
+
+def xcd_lists(item0, nnz, npan, real):
+    """The batched K1's per-XCD work lists: (items, seg) as the library builds them."""
+    item0 = np.ascontiguousarray(item0, np.int64)
+    nnz = np.ascontiguousarray(nnz, np.int64)
+    npan = np.ascontiguousarray(npan, np.int32)
+    real = np.ascontiguousarray(real, np.int32)
+    items = np.empty(2 * int(item0[-1]), np.int32)
+    seg = np.empty(27, np.int32)
+    lib().hc_xcd_lists(len(nnz), _p(item0), _p(nnz), _p(npan), _p(real), _p(items), _p(seg))
+    return items, seg

@@ -61,3 +61,53 @@ def test_rank_unrank_enumerates_sector(oracle_mod, case):
         up = np.array([bin(int(w) & m).count("1") + bin((int(w) >> (2 * nphi)) & m).count("1") for w in ref])
         ref = ref[up == case[4]]
     np.testing.assert_array_equal(words, ref)
+
+
+def _members(rng, P, real_frac, multi_frac, max_items):
+    nit = rng.integers(1, max_items + 1, size=P)
+    item0 = np.concatenate([[0], np.cumsum(nit)])
+    nnz = nit * rng.integers(50, 400, size=P)
+    npan = np.where(rng.random(P) < multi_frac, rng.integers(2, 5, size=P), 1)
+    real = (rng.random(P) < real_frac).astype(np.int32)
+    return item0, nnz, npan, real
+
+
+@pytest.mark.parametrize("P,real_frac,multi_frac,max_items", [
+    (1, 0.0, 0.0, 74), (2, 1.0, 1.0, 800), (6, 0.5, 0.0, 30), (144, 0.17, 0.0, 74),
+    (64, 0.1, 0.0, 63), (40, 0.5, 0.3, 200), (300, 0.5, 0.05, 9)])
+def test_xcd_work_lists(P, real_frac, multi_frac, max_items):
+    """The batched K1's per-XCD lists: every item of a kind exactly once,
+    ascending within an XCD, several-panel members dealt item i -> XCD i mod 8,
+    one-panel members whole when there are many and spread when there are
+    few, chunk loads balanced to within the largest chunk."""
+    rng = np.random.default_rng(P)
+    item0, nnz, npan, real = _members(rng, P, real_frac, multi_frac, max_items)
+    items, seg = hostmirror.xcd_lists(item0, nnz, npan, real)
+    nitems = int(item0[-1])
+    owner = np.repeat(np.arange(P), np.diff(item0))
+    assert seg[0] == 0 and seg[26] == 2 * nitems and seg[8] == seg[9] and seg[17] == seg[18] == nitems
+    for kind in range(3):
+        mine = np.ones(P, bool) if kind == 2 else (real == (1 if kind == 0 else 0))
+        got = np.concatenate([items[seg[9 * kind + x]:seg[9 * kind + x + 1]] for x in range(8)])
+        np.testing.assert_array_equal(np.sort(got), np.flatnonzero(mine[owner]))
+        small = mine & (npan == 1)
+        cmax = max(1, -(-int(np.diff(item0)[small].sum()) // 64))
+        load = np.zeros(8)
+        for x in range(8):
+            lst = items[seg[9 * kind + x]:seg[9 * kind + x + 1]]
+            assert np.all(np.diff(lst) > 0)
+            multi = lst[npan[owner[lst]] > 1]
+            assert np.all(multi % 8 == x)
+            load[x] = (nnz[owner[lst]] / np.diff(item0)[owner[lst]]).sum()
+        if small.any() and not (mine & (npan > 1)).any():
+            # greedy largest-first dealing: loads differ by at most one chunk
+            wmax = max((nnz[p] / (item0[p + 1] - item0[p])) * min(cmax, item0[p + 1] - item0[p])
+                       for p in np.flatnonzero(small))
+            assert load.max() - load.min() <= wmax * (1 + 1e-12)
+        whole = small & (np.diff(item0) <= cmax)
+        for p in np.flatnonzero(whole):
+            xs = {x for x in range(8) if np.isin(np.arange(item0[p], item0[p + 1]),
+                                                 items[seg[9 * kind + x]:seg[9 * kind + x + 1]]).any()}
+            assert len(xs) == 1, (kind, p)
+        if small.sum() <= 2 and np.diff(item0)[small].sum() >= 64:
+            assert all(seg[9 * kind + x + 1] > seg[9 * kind + x] for x in range(8))
