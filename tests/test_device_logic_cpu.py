@@ -111,3 +111,22 @@ def test_xcd_work_lists(P, real_frac, multi_frac, max_items):
             assert len(xs) == 1, (kind, p)
         if small.sum() <= 2 and np.diff(item0)[small].sum() >= 64:
             assert all(seg[9 * kind + x + 1] > seg[9 * kind + x] for x in range(8))
+
+
+def test_small_modulo_by_fp32_reciprocal():
+    """The build's hop-target shift on the device: s mod d from the FP32
+    reciprocal of d (the hardware's approximate reciprocal is within 1 ulp;
+    checked here for every reciprocal within 2 ulp of 1/d), truncated
+    quotient, one correction step either way (mod_small in rowterms.hpp).
+    Exact for 0 <= s < 2^16, 1 <= d <= 64 (the build uses s < 64)."""
+    s = np.arange(1 << 16, dtype=np.int64)
+    for d in range(1, 65):
+        r0 = np.float32(1.0) / np.float32(d)
+        up1 = np.nextafter(r0, np.float32(2))
+        dn1 = np.nextafter(r0, np.float32(0))
+        cands = [r0, up1, np.nextafter(up1, np.float32(2)), dn1, np.nextafter(dn1, np.float32(0))]
+        for rc in cands:
+            q = (s.astype(np.float32) * np.float32(rc)).astype(np.int64)
+            r = s - q * d
+            r = np.where(r < 0, r + d, np.where(r >= d, r - d, r))
+            np.testing.assert_array_equal(r, s % d)
