@@ -32,7 +32,7 @@ struct RowCtx {
 
 BLQH_HD double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
-// s mod d for 0 <= s < 2^20, d >= 1.  On the device the quotient comes from
+// s mod d for 0 <= s < 2^16, d >= 1.  On the device the quotient comes from
 // the FP32 reciprocal (off by at most one for such s, corrected below)
 // instead of an integer division sequence.
 BLQH_HD int mod_small(int s, int d) {
