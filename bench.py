@@ -11,8 +11,12 @@ resident in HBM; a step counts only if it reached |dE| < 1e-10.
 
 Prints one JSON line.  The roofline object is for the dominant kernel (the
 fused Lanczos SpMV), timed with HIP events on the library stream during the
-timed solves.  cpu_baseline times the oracle (the CPU port of the reference's
-fixed-200-step three-pass Lanczos) on a sample of the same matrix's rows.
+timed solves.  cpu_baseline times the oracle's CSR SpMV (the reference's row
+loop) over the whole matrix on 16 host threads and charges the reference's
+fixed 200-step three-pass Lanczos (400 SpMVs) per ground state.  The default
+run also times the sweep workloads (all 64 spinful N_phi = 8 sectors and all
+144 N_phi = 12 sectors, --sweep-points parameter points each): batched
+solves against plain single-problem solves, under "sweeps".
 """
 import argparse
 import json
@@ -30,10 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the fused SpMV from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # passes, scaled by the same counters on a plain 1 GiB copy (profiles/);
 # reported only when the live run uses the profiled workload and variant
-MEASURED_TRAFFIC = {
-    ("nphi16_all", (3, 64, 1024, 1, 8)): (13.47e9, "profiles/k_jag_nphi16_counters.md"),
-    ("nphi16_all", (3, 64, 512, 2, 8)): (23.02e9, "profiles/k_jag_nphi16_counters.md"),
-}
+MEASURED_TRAFFIC = {}
 # fused Lanczos K1 kernel of each SpMV variant (lanczos.hip)
 KERNEL_NAMES = {0: "k_lz1", 1: "k_pspmv<LZ=true>", 2: "k_sps<LZ=true>", 3: "k_jag<LZ=true>", 4: "k_t4<LZ=true>",
                 5: "k_jagr<LZ=true>"}
@@ -65,48 +66,51 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(basis_ids, periods, wl, n, nnz, iters_ref=200, budget_s=20.0):
-    """Oracle (CPU port of the reference) on a row sample of the same sector."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_full(H, iters_ref=200, threads=16, reps=3):
+    """The oracle's CSR row loop (the reference's SpMV, Mat::operator*) over the
+    whole matrix exported from the GPU build: one full, unsampled SpMV, median
+    of `reps`, times the 2*iters_ref SpMVs of the reference's fixed-length
+    three-pass Lanczos (pass 1, pass-3 replay, energy)."""
     import oracle as O
-    S, nphi, nel, nu, J, K, d, dsas, _ = wl
-    sec = O.sector(S=S, nphi=nphi, nel=nel, nel_up=nu, J=J, K=K)
     t0 = time.time()
-    OB = O.Basis(sec, adopt=(basis_ids, periods))
-    t_index = time.time() - t0
-    par = O.params(nphi, d=d, dsas=dsas)
+    rp, col, val = H.export()
+    t_export = time.time() - t0
+    n = H.n
     rng = np.random.default_rng(0)
-    rows = np.sort(rng.choice(n, size=min(n, 256), replace=False))
-    threads = os.cpu_count() or 1
-    t0 = time.time()
-    rp, col, val = O.hamil_rows(OB, par, rows, nthreads=threads)
-    t_rows = time.time() - t0
-    sub = O.lib().orc_csr_wrap(len(rows), rp.ctypes.data, col.ctypes.data,
-                               np.ascontiguousarray(val).view(np.float64).ctypes.data)
     x = (rng.normal(size=n) + 1j * rng.normal(size=n)).astype(np.complex128)
-    y = np.empty(len(rows), np.complex128)
-    reps, t_spmv = 0, 0.0
-    while t_spmv < 2.0 and reps < 20000:
-        t0 = time.time()
-        for _ in range(50):
-            O.lib().orc_spmv(sub, x.ctypes.data, y.ctypes.data)
-        t_spmv += time.time() - t0
-        reps += 50
-    O.lib().orc_hamil_free(sub)
-    nnz_s = len(col)
-    t_full_spmv = (t_spmv / reps) * (nnz / nnz_s)
-    # the reference runs exactly 200 steps: 200 pass-1 + 199 pass-3 SpMVs + 1 energy SpMV
-    t_solve = (2 * iters_ref) * t_full_spmv
-    t_build = t_rows * n / len(rows)
+    y = np.empty(n, np.complex128)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.lib().orc_spmv32(n, rp.ctypes.data, col.ctypes.data, val.ctypes.data, x.ctypes.data, y.ctypes.data,
+                           threads)
+        ts.append(time.perf_counter() - t0)
+    t_spmv = float(np.median(ts))
+    del rp, col, val
+    t_solve = 2 * iters_ref * t_spmv
     return {
         "value": 1.0 / t_solve,
         "unit": "K-sector ground states/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle SpMV over {len(rows)} sampled rows ({nnz_s} nnz) of the same CSR, "
-                   f"{reps} reps with gathers into the full vector, scaled by nnz to one full SpMV "
-                   f"({t_full_spmv * 1e3:.1f} ms) x 400 SpMVs (fixed 200-step three-pass Lanczos); "
-                   f"solve only, H resident; the oracle's row build extrapolates to {t_build:.0f} s for "
-                   f"the full matrix ({t_rows:.1f} s for the sample, index {t_index:.1f} s)"),
+        "cpu": cpu_model(),
+        "sample": (f"oracle CSR SpMV over the full {n}-row, {H.nnz}-nnz matrix (not sampled), median of {reps}: "
+                   f"{t_spmv:.3f} s ({[round(t, 3) for t in ts]}); x {2 * iters_ref} SpMVs of the reference's "
+                   f"fixed {iters_ref}-step three-pass Lanczos = {t_solve:.0f} s per ground state; vector updates "
+                   f"not counted; H resident (host copy exported in {t_export:.1f} s)"),
+        "spmv_s": t_spmv,
+        "threads": threads,
     }
 
 
@@ -318,35 +322,50 @@ def run_sweep(args):
             out["cpu_baseline"] = sweep_cpu_baseline(sw, iters_ref=args.lam)
         except Exception as e:  # report, never hide, a failed baseline
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 # Learned warm start: train on ground states at coarse d, solve at other d.
 WARM = {
-    "warm_nphi14": dict(nphi=14, nel=14, nu=-1, sectors=[(0, 0), (7, 7)], dsas=0.05,
+    "warm_nphi14": dict(S=2, nphi=14, nel=14, nu=-1, nsu=-1, dv=0.0, dz=0.0, sectors=[(0, 0), (7, 7)], dsas=0.05,
                train_d=[0.5 * i for i in range(7)],
                eval_d=[round(0.05 * i, 2) for i in range(61)],
                label="spinless bilayer Nphi=14 Ne=14 all densities, sectors (0,0) and (7,7), DSAS=0.05, "
                      "network trained at d in {0,0.5,..,3.0}, solves at d in {0,0.05,..,3.0}"),
+    # the network's sign output fits real sectors (K = 0, nphi/2): complex
+    # sectors keep the seeded start
+    "warm_spin_nphi8": dict(S=4, nphi=8, nel=8, nu=4, nsu=4, dv=0.02, dz=0.01,
+               sectors=[(J, K) for J in range(8) for K in range(8)], dsas=0.05,
+               train_d=[0.25, 0.75, 1.25, 1.75, 2.25],
+               eval_d=[0.5, 1.0, 1.5, 2.0],
+               label="spinful bilayer Nphi=8 Ne=8 nel_up=4 nel_su=4, all 64 (J,K) sectors, DZ=0.01 DSAS=0.05 "
+                     "DV=0.02, network (32 occupation bits) trained on the 16 real sectors at d in "
+                     "{0.25,0.75,..,2.25}, solves at d in {0.5,1.0,1.5,2.0}; complex sectors start seeded"),
 }
 
 
 def run_warm(args):
     import blqh_amd
     wc = WARM[args.workload]
-    nphi, nel, dsas = wc["nphi"], wc["nel"], wc["dsas"]
+    S, nphi, nel, dsas = wc["S"], wc["nphi"], wc["nel"], wc["dsas"]
     log(f"workload {args.workload}: {wc['label']}")
-    bases = {s: blqh_amd.Basis(nphi, nel, wc["nu"], *s) for s in wc["sectors"]}
-    log("bases " + ", ".join(f"{s}: {B.n}" for s, B in bases.items()))
+    bases = {s: blqh_amd.Basis(nphi, nel, wc["nu"], *s, nspecies=S, nel_su=wc["nsu"]) for s in wc["sectors"]}
+    log(f"{len(bases)} bases, N {min(B.n for B in bases.values())}..{max(B.n for B in bases.values())}")
     chk = args.check_every
 
-    def ham(B, d):
-        return blqh_amd.Hamiltonian(B, blqh_amd.params(nphi, d=d, delta_sas=dsas))
+    def real_sector(s):
+        return s[1] in (0, nphi // 2)
 
-    # 1. training data: plain solves at the coarse grid
+    def ham(B, d):
+        return blqh_amd.Hamiltonian(B, blqh_amd.params(nphi, d=d, delta_sas=dsas, delta_v=wc["dv"],
+                                                       delta_z=wc["dz"]))
+
+    # 1. training data: plain solves at the coarse grid (real sectors)
     t0 = time.time()
     sets = []
     for s, B in bases.items():
+        if not real_sector(s):
+            continue
         for d in wc["train_d"]:
             H = ham(B, d)
             r = H.lanczos(lam=args.lam, seed=1, tol=1e-10, check_every=chk)
@@ -355,7 +374,7 @@ def run_warm(args):
     t_data = time.time() - t0
     log(f"training data: {len(sets)} ground states in {t_data:.1f} s")
     # 2. train on the GPU
-    net = blqh_amd.Mlp(2 * nphi, seed=7)
+    net = blqh_amd.Mlp(S * nphi, seed=7)
     info = net.train(sets, steps=args.train_steps, batch=8192, lr=3e-3, lr_final=1e-4, seed=11)
     log(f"trained: loss {info['loss_first']:.3f} -> {info['loss_last']:.3f} in {info['seconds']:.2f} s "
         f"({info['steps']} steps, {info['samples']} states)")
@@ -378,9 +397,12 @@ def run_warm(args):
     # warm: network start (inference included), one problem at a time
     t = time.perf_counter()
     warm = []
-    for H, (s, d) in zip(hs, keys):
-        v = net.start_vector(bases[s], d, dsas)
-        warm.append(H.lanczos(lam=args.lam, start=v, tol=1e-10, check_every=chk, want_psi=False))
+    for i, (H, (s, d)) in enumerate(zip(hs, keys)):
+        if real_sector(s):
+            v = net.start_vector(bases[s], d, dsas)
+            warm.append(H.lanczos(lam=args.lam, start=v, tol=1e-10, check_every=chk, want_psi=False))
+        else:
+            warm.append(H.lanczos(lam=args.lam, seed=1, tol=1e-10, check_every=chk, want_psi=False))
     t_warm = time.perf_counter() - t
     # batched plain and batched + warm
     batch = blqh_amd.Batch(hs)
@@ -388,7 +410,7 @@ def run_warm(args):
     bplain = batch.lanczos(lam=args.lam, seeds=1, tol=1e-10, check_every=chk, want_psi=False)
     t_bplain = time.perf_counter() - t
     t = time.perf_counter()
-    starts = [net.start_vector(bases[s], d, dsas) for s, d in keys]
+    starts = [net.start_vector(bases[s], d, dsas) if real_sector(s) else None for s, d in keys]
     bwarm = batch.lanczos(lam=args.lam, seeds=1, starts=starts, tol=1e-10, check_every=chk, want_psi=False)
     t_bwarm = time.perf_counter() - t
 
@@ -443,9 +465,12 @@ def run_warm(args):
             "batched_warm_over_plain": rates["batched_warm"] / rates["plain"],
             "warm_over_plain": rates["warm"] / rates["plain"],
             "iterations_mean": it,
+            "iterations_mean_warm_sectors": {
+                name: float(np.mean([r["iterations"] for r, (sct, _) in zip(rs, keys) if real_sector(sct)]))
+                for name, rs in (("plain", plain), ("warm", warm), ("batched", bplain), ("batched_warm", bwarm))},
             "max_rel_E0_diff_vs_plain_both_converged": dev,
             "not_converged_at_cap": unconverged,
-            "training": {"ground_states": len(wc["sectors"]) * len(wc["train_d"]), "data_s": round(t_data, 2),
+            "training": {"ground_states": sum(int(real_sector(x)) for x in wc["sectors"]) * len(wc["train_d"]), "data_s": round(t_data, 2),
                          "train_s": round(info["seconds"], 3), "steps": info["steps"], "batch": 8192,
                          "loss_first": info["loss_first"], "loss_last": info["loss_last"]},
         },
@@ -463,6 +488,8 @@ def main():
     ap.add_argument("--check-every", type=int, default=5)
     ap.add_argument("--lam", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sweep-points", type=int, default=2,
+                    help="default workload: also time the sweep workloads at this many parameter points (0: skip)")
     ap.add_argument("--start-mix", type=float, default=0.1,
                     help="weight of the seeded random unit vector added to warm starts (guards level crossings)")
     ap.add_argument("--continuation", type=int, default=0,
@@ -484,7 +511,7 @@ def main():
         sys.exit(2)
 
     if args.workload in SWEEPS:
-        run_sweep(args)
+        print(json.dumps(run_sweep(args)), flush=True)
         return
     if args.workload in WARM:
         run_warm(args)
@@ -566,12 +593,38 @@ def main():
             "algorithmic_bytes_per_launch": kbytes / max(launches, 1),
         },
     }
+    # end to end: the Hamiltonian build plus one solve (the per-J basis tables are one-time)
+    out["config"]["end_to_end_rate_incl_build"] = 1.0 / (t_build + elapsed / args.steps)
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle port on a row sample)")
+        log("cpu baseline (oracle CSR SpMV over the full matrix)")
         try:
-            out["cpu_baseline"] = cpu_baseline(B.id, B.period, wl, H.n, H.nnz, iters_ref=args.lam)
+            out["cpu_baseline"] = cpu_baseline_full(H, iters_ref=args.lam)
         except Exception as e:  # report, never hide, a failed baseline
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    del H, B
+    blqh_amd.release_cached()
+    if args.workload == "nphi16_all" and args.sweep_points > 0:
+        # the headline's second half in the same run: all K-sectors of the
+        # sweep workloads, batched vs plain single-problem solves
+        out["sweeps"] = {}
+        for name in ("sweep_spin_nphi8", "sweep_nphi12"):
+            sa = argparse.Namespace(**vars(args))
+            sa.workload, sa.steps, sa.warmup, sa.continuation = name, args.sweep_points, 1, 0
+            sa.no_cpu_baseline = args.no_cpu_baseline
+            r = run_sweep(sa)
+            c = r["config"]
+            out["sweeps"][name] = {
+                "workload": c["workload"], "sectors": c["sectors"], "points_timed": c["points_timed"],
+                "solves_timed": c["solves_timed"], "converged": c["converged"],
+                "batched_rate": c["batched_rate"], "plain_rate": c["plain_rate"],
+                "batched_over_plain": c["batched_over_plain"], "iterations_mean": c["iterations_mean"],
+                "end_to_end_rate_incl_build": c["end_to_end_rate_incl_build"],
+                "max_rel_dE0_batched_vs_plain": c["plain"]["max_rel_dE0_vs_batched"] if c["plain"] else None,
+                "roofline": {k: r["roofline"][k] for k in ("achieved", "peak", "unit", "frac", "kernel",
+                                                           "avg_launch_ms", "algorithmic_bytes_per_launch")},
+                "cpu_baseline": r.get("cpu_baseline"),
+            }
+            blqh_amd.release_cached()
     print(json.dumps(out), flush=True)
 
 

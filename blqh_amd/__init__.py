@@ -301,7 +301,9 @@ class Hamiltonian:
         _check(lib().blqh_hamil_set_spmv(self.h, kind, lanes, threads, wgs_per_cu, rows))
 
     def spmv(self, x):
-        x = np.ascontiguousarray(x, np.complex128)
+        x = np.ascontiguousarray(x, np.complex128).reshape(-1)
+        if x.size != self.n:
+            raise ValueError(f"x has {x.size} entries, the Hamiltonian {self.n} rows")
         y = np.empty_like(x)
         _check(lib().blqh_spmv(self.h, _p(x), _p(y)))
         return y
@@ -312,17 +314,24 @@ class Hamiltonian:
         Ritz residual is < res_tol (tol <= 0: exactly lam steps).  skip_states:
         passes 1 and 2 only (alpha, beta, E0; no psi, E_rq is NaN).  start_mix:
         weight of the seeded random unit vector added to a given start."""
+        return self._lanczos(lib().blqh_lanczos, lam, seed, tol, res_tol, check_every, store_basis, start,
+                             want_psi, skip_states, start_mix)
+
+    def _lanczos(self, fn, lam, seed, tol, res_tol, check_every, store_basis, start, want_psi, skip_states,
+                 start_mix):
         opts = LanczosOpts(lam, seed, tol, check_every, store_basis, None, res_tol, 1 if skip_states else 0,
                            start_mix)
         keep = None
         if start is not None:
-            keep = np.ascontiguousarray(start, np.complex128)
+            keep = np.ascontiguousarray(start, np.complex128).reshape(-1)
+            if keep.size != self.n:
+                raise ValueError(f"start vector has {keep.size} entries, the Hamiltonian {self.n} rows")
             opts.start = keep.ctypes.data
         res = LanczosResult()
         psi = np.empty(self.n, np.complex128) if want_psi else None
         a = np.zeros(lam)
         b = np.zeros(lam + 1)
-        _check(lib().blqh_lanczos(self.h, C.byref(opts), C.byref(res), _p(psi) if want_psi else None, _p(a), _p(b)))
+        _check(fn(self.h, C.byref(opts), C.byref(res), _p(psi) if want_psi else None, _p(a), _p(b)))
         out = {k: getattr(res, k) for k, _ in LanczosResult._fields_}
         out.update(psi=psi, alpha=a[:res.iterations], beta=b[:res.iterations + 1])
         return out
@@ -335,12 +344,17 @@ class Hamiltonian:
         given ones (beta[0] = 1 as returned by lanczos())."""
         c = np.ascontiguousarray(np.atleast_2d(coefs), np.float64)
         ns, m = c.shape
-        a = None if alpha is None else np.ascontiguousarray(alpha, np.float64)
-        b = None if beta is None else np.ascontiguousarray(beta, np.float64)
-        assert a is None or len(a) >= m - 1
-        assert b is None or len(b) >= m
-        keep = None if start is None else np.ascontiguousarray(start, np.complex128)
-        assert keep is None or len(keep) == self.n
+        a = None if alpha is None else np.ascontiguousarray(alpha, np.float64).reshape(-1)
+        b = None if beta is None else np.ascontiguousarray(beta, np.float64).reshape(-1)
+        if a is not None and len(a) < m - 1:
+            raise ValueError(f"alpha has {len(a)} entries, {m - 1} needed")
+        if b is not None and len(b) < m:
+            raise ValueError(f"beta has {len(b)} entries, {m} needed")
+        if not 1 <= ns <= 16:
+            raise ValueError(f"{ns} coefficient vectors (1 to 16 allowed)")
+        keep = None if start is None else np.ascontiguousarray(start, np.complex128).reshape(-1)
+        if keep is not None and keep.size != self.n:
+            raise ValueError(f"start vector has {keep.size} entries, the Hamiltonian {self.n} rows")
         psi = np.empty((ns, self.n), np.complex128)
         erq = np.empty(ns)
         dev = C.c_double()
@@ -431,14 +445,22 @@ class Batch:
         `start_mix` the weight of the seeded unit vector added to each start."""
         P = len(self.hams)
         opts = LanczosOpts(lam, 0, tol, check_every, 1, None, res_tol, 0, start_mix)
-        sd = np.broadcast_to(np.asarray(seeds, np.uint32), (P,)).copy()
+        sd = np.asarray(seeds, np.uint32)
+        if sd.ndim > 1 or (sd.ndim == 1 and sd.size != P):
+            raise ValueError(f"{sd.size} seeds for {P} problems")
+        sd = np.broadcast_to(sd, (P,)).copy()
         keep = []
         st = None
         if starts is not None:
+            starts = list(starts)
+            if len(starts) != P:
+                raise ValueError(f"{len(starts)} start vectors for {P} problems")
             st = (C.c_void_p * P)()
             for i, v in enumerate(starts):
                 if v is not None:
-                    a = np.ascontiguousarray(v, np.complex128)
+                    a = np.ascontiguousarray(v, np.complex128).reshape(-1)
+                    if a.size != self.hams[i].n:
+                        raise ValueError(f"start vector {i} has {a.size} entries, problem {i} {self.hams[i].n} rows")
                     keep.append(a)
                     st[i] = a.ctypes.data
         res = (LanczosResult * P)()
@@ -484,14 +506,17 @@ class Mlp:
 
     @params.setter
     def params(self, v):
-        v = np.ascontiguousarray(v, np.float32)
-        assert v.size == self.nparams
+        v = np.ascontiguousarray(v, np.float32).reshape(-1)
+        if v.size != self.nparams:
+            raise ValueError(f"{v.size} parameters, the network has {self.nparams}")
         _check(self._lib.blqh_mlp_set_params(self.h, _p(v)))
 
     def loss_grad(self, words, d, dsas, ya, ys):
         """Mean loss and gradient over explicit states (all arrays of one length)."""
-        w = np.ascontiguousarray(words, np.uint64)
-        f = [np.ascontiguousarray(a, np.float32) for a in (d, dsas, ya, ys)]
+        w = np.ascontiguousarray(words, np.uint64).reshape(-1)
+        f = [np.ascontiguousarray(a, np.float32).reshape(-1) for a in (d, dsas, ya, ys)]
+        if any(a.size != w.size for a in f):
+            raise ValueError("words, d, dsas, ya and ys must have one length")
         loss = C.c_double()
         g = np.empty(self.nparams, np.float32)
         _check(self._lib.blqh_mlp_loss_grad(self.h, len(w), _p(w), _p(f[0]), _p(f[1]), _p(f[2]), _p(f[3]),
@@ -501,9 +526,12 @@ class Mlp:
     def train(self, sets, steps=2000, batch=4096, lr=3e-3, lr_final=3e-4, seed=1):
         """sets: list of (Basis, d, dsas, psi).  Returns a dict of the run."""
         n = len(sets)
+        keep = [np.ascontiguousarray(p, np.complex128).reshape(-1) for _, _, _, p in sets]
+        for k, (b, _, _, _) in enumerate(sets):
+            if keep[k].size != b.n:
+                raise ValueError(f"state {k} has {keep[k].size} entries, its basis {b.n} states")
         bases = (C.c_void_p * n)(*[b.h.value for b, _, _, _ in sets])
         par = np.ascontiguousarray([[d, s] for _, d, s, _ in sets], np.float64)
-        keep = [np.ascontiguousarray(p, np.complex128) for _, _, _, p in sets]
         psi = (C.c_void_p * n)(*[k.ctypes.data for k in keep])
         o = MlpTrainOpts(steps, batch, lr, lr_final, seed)
         r = MlpTrainResult()
@@ -527,7 +555,9 @@ class Mlp:
 
 def observables(basis, psi):
     """Ground-state observables of psi in `basis` (dict; occupation is [2][nphi])."""
-    v = np.ascontiguousarray(psi, np.complex128)
+    v = np.ascontiguousarray(psi, np.complex128).reshape(-1)
+    if v.size != basis.n:
+        raise ValueError(f"state has {v.size} entries, the basis {basis.n} states")
     o = Observables()
     _check(lib().blqh_compute_observables(basis.h, _p(v), C.byref(o)))
     nphi = basis.sector.nphi
@@ -570,7 +600,8 @@ def spectrum_distinct(eig, kmax=7, gap=1e-5):
 def record_save(path, nhilbert, seed, eig, beta, alpha, psi0, psin0):
     m = len(alpha)
     arrs = [np.ascontiguousarray(v, np.float64) for v in (eig, beta, alpha, psi0, psin0)]
-    assert len(arrs[1]) == m + 1 and all(len(v) == m for i, v in enumerate(arrs) if i != 1)
+    if len(arrs[1]) != m + 1 or any(len(v) != m for i, v in enumerate(arrs) if i != 1):
+        raise ValueError("record arrays: beta needs len(alpha) + 1 entries, the others len(alpha)")
     _check(lib().blqh_record_save(os.fsencode(path), nhilbert, m, seed, *[_p(v) for v in arrs]))
 
 

@@ -561,7 +561,7 @@ blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count, int lanes) 
         }
         BLQH_REQUIRE(b->nitems < (int64_t)1 << 31, BLQH_ERANGE, "batch too large");
         b->item_prob.alloc(ip.size());
-        BLQH_HIP(hipMemcpy(b->item_prob.get(), ip.data(), ip.size() * 4, hipMemcpyHostToDevice));
+        copy_sync(b->item_prob.get(), ip.data(), ip.size() * 4, hipMemcpyHostToDevice);
         // a multiple of 8 (one share per XCD in k_b1j)
         b->grid = (int)(8 * ceil_div(std::min<int64_t>(b->nitems, (int64_t)runtime().num_cus * 8), 8));
         b->xitems.alloc(2 * b->nitems);
@@ -705,6 +705,12 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
                 } else {
                     start_vector(n, seeds ? seeds[p] : o.seed, v0[p]);
                 }
+                const blqh_hamil* Hp = b->hs[p];
+                if (Hp->ord.on) {  // the member's SpMV formats are in its internal order
+                    std::vector<double2> w(n);
+                    for (int64_t r = 0; r < n; ++r) w[r] = v0[p][Hp->ord.h_iperm[r]];
+                    v0[p].swap(w);
+                }
             }
         };
         const int nt = std::max(1, std::min<int>(P, (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
@@ -772,7 +778,7 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     }
     hipGraphExec_t gexec = b->gexec;
     const int zero = 0;
-    BLQH_HIP(hipMemcpy(pl.jbase.get(), &zero, 4, hipMemcpyHostToDevice));
+    copy_sync(pl.jbase.get(), &zero, 4, hipMemcpyHostToDevice);
 
     // pinned staging: alpha/beta read back per replay, active flags (two slots,
     // so a slot is rewritten only after its copy has run)
@@ -909,9 +915,14 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     BLQH_HIP(hipGetLastError());
     BLQH_HIP(hipMemcpyAsync(hp2.data(), pl.part2.get(), b->nitems * 8, hipMemcpyDeviceToHost, st));
     for (int p = 0; p < P; ++p)
-        if (psi_out && psi_out[p])
-            BLQH_HIP(hipMemcpyAsync(psi_out[p], pl.psi.get() + hp[p].voff, hp[p].n * sizeof(double2),
-                                    hipMemcpyDeviceToHost, st));
+        if (psi_out && psi_out[p]) {
+            const double2* src = pl.psi.get() + hp[p].voff;
+            if (b->hs[p]->ord.on) {  // back to the reference order, through u (free after the last SpMV)
+                order_to_reference(b->hs[p], src, pl.u.get() + hp[p].voff, st);
+                src = pl.u.get() + hp[p].voff;
+            }
+            BLQH_HIP(hipMemcpyAsync(psi_out[p], src, hp[p].n * sizeof(double2), hipMemcpyDeviceToHost, st));
+        }
     BLQH_HIP(hipStreamSynchronize(st));
     if (timed) {
         for (int k = 0; k < ntimed; ++k) {

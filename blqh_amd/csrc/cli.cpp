@@ -174,7 +174,10 @@ int main(int argc, char** argv) {
 #endif
     const double ly = std::sqrt(r.nphi * 2.0 * M_PI / r.gamma);
     const double lx = ly * r.gamma;
-    double tol = 1e-10;
+    // The reference runs exactly lambda Lanczos steps (no convergence test),
+    // so by default the drop-in does too (its printed amplitudes then carry
+    // the reference's digits); BLQH_LANCZOS_TOL > 0 stops at |dE| < tol instead.
+    double tol = 0.0;
     if (const char* e = std::getenv("BLQH_LANCZOS_TOL")) tol = std::atof(e);
 
     blqh_sector sec{S, r.nphi, r.nel, r.nel_up, S == 4 ? r.nel_su : -1, r.J, r.kx};

@@ -84,4 +84,12 @@ hipStream_t stream();
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Blocking copy ordered on the library stream: queued behind every kernel
+// already on it, returns when the copy is done.  (A plain hipMemcpy runs on
+// the null stream, which the non-blocking library stream does not wait for.)
+inline void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    BLQH_HIP(hipMemcpyAsync(dst, src, bytes, kind, stream()));
+    BLQH_HIP(hipStreamSynchronize(stream()));
+}
+
 }  // namespace blqh

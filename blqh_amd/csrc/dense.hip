@@ -35,8 +35,16 @@ void blqh_dense_ground_state_impl(const blqh_hamil* H, double* e0, double* e_pri
     hipLaunchKernelGGL(k_csr_to_dense, dim3((unsigned)std::min<int64_t>(n, 65535)), dim3(64), 0, st, n,
                        H->rowptr.get(), H->col.get(), H->val.get(), A.get());
     BLQH_HIP(hipGetLastError());
-    rocblas_handle h;
-    BLQH_REQUIRE(rocblas_create_handle(&h) == rocblas_status_success, BLQH_EDEVICE, "rocblas_create_handle failed");
+    // the handle is destroyed on every exit, including a throwing copy below
+    struct Handle {
+        rocblas_handle h = nullptr;
+        ~Handle() {
+            if (h) rocblas_destroy_handle(h);
+        }
+    } hd_;
+    BLQH_REQUIRE(rocblas_create_handle(&hd_.h) == rocblas_status_success, BLQH_EDEVICE,
+                 "rocblas_create_handle failed");
+    rocblas_handle h = hd_.h;
     rocblas_set_stream(h, st);
     rocblas_status s = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n,
                                         (rocblas_double_complex*)A.get(), (rocblas_int)n, D.get(), E.get(), info.get());
@@ -47,15 +55,14 @@ void blqh_dense_ground_state_impl(const blqh_hamil* H, double* e0, double* e_pri
     BLQH_HIP(hipMemcpyAsync(v0.data(), A.get(), n * 16, hipMemcpyDeviceToHost, st));
     BLQH_HIP(hipMemcpyAsync(&hinfo, info.get(), sizeof(int), hipMemcpyDeviceToHost, st));
     BLQH_HIP(hipStreamSynchronize(st));
-    rocblas_destroy_handle(h);
     BLQH_REQUIRE(s == rocblas_status_success && hinfo == 0, BLQH_EINTERNAL, "rocsolver_zheevd failed");
     // Re<H psi_0 | psi_0> with the CSR on the host
     std::vector<int64_t> rp(n + 1);
     std::vector<int32_t> cl(H->nnz);
     std::vector<double2> vl(H->nnz);
-    BLQH_HIP(hipMemcpy(rp.data(), H->rowptr.get(), (n + 1) * 8, hipMemcpyDeviceToHost));
-    BLQH_HIP(hipMemcpy(cl.data(), H->col.get(), H->nnz * 4, hipMemcpyDeviceToHost));
-    BLQH_HIP(hipMemcpy(vl.data(), H->val.get(), H->nnz * 16, hipMemcpyDeviceToHost));
+    copy_sync(rp.data(), H->rowptr.get(), (n + 1) * 8, hipMemcpyDeviceToHost);
+    copy_sync(cl.data(), H->col.get(), H->nnz * 4, hipMemcpyDeviceToHost);
+    copy_sync(vl.data(), H->val.get(), H->nnz * 16, hipMemcpyDeviceToHost);
     double e = 0;
     for (int64_t i = 0; i < n; ++i) {
         double yr = 0, yi = 0;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <atomic>
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "bits.hpp"
@@ -25,6 +26,10 @@ struct blqh_basis {
     std::vector<int16_t> h_period;
     double seconds_host = 0, seconds_device = 0;
     mutable std::atomic<int32_t> max_row_nnz{0};  // largest row kept by a build with the pair terms (0: none yet)
+    // internal order of the SpMV formats (internal_order), computed once
+    mutable std::mutex ord_mu;
+    mutable bool ord_done = false;
+    mutable std::vector<int32_t> h_order;  // h_order[i] = internal position of row i (empty: identity)
 };
 
 struct blqh_hamil {
@@ -34,6 +39,18 @@ struct blqh_hamil {
     blqh::DevBuf<int64_t> rowptr;
     blqh::DevBuf<int32_t> col;
     blqh::DevBuf<double2> val;
+    // Internal row / column order of every SpMV format (panels, tiles and the
+    // CSR kernel): the CSR again with rows and columns permuted (internal_order).
+    // Vectors cross into and out of it at the solver's boundaries.
+    struct Order {
+        bool on = false;
+        blqh::DevBuf<int32_t> perm;      // perm[i]: internal position of reference row i
+        blqh::DevBuf<int32_t> iperm;     // iperm[r]: reference row at internal position r
+        std::vector<int32_t> h_iperm;
+        blqh::DevBuf<int64_t> rowptr;    // the CSR in internal order, columns ascending
+        blqh::DevBuf<int32_t> col;
+        blqh::DevBuf<double2> val;
+    } ord;
     blqh::Panels pan;                // panel-major copy for the SpMV hot loop
     blqh::Tiles til;                 // column-tile copy (SpMV kind 4), built on demand
     int spmv_kind = -1;              // autotuned SpMV: 0 = row-major CSR, 1 = panel-major (-1 = not tuned)
@@ -51,6 +68,7 @@ RankSpec make_rank_spec(const blqh_sector& s, bool with_spin_constraint);
 std::vector<uint32_t> make_rank_table(const RankSpec& spec, int64_t* total);
 void exclusive_scan(const int32_t* in, int64_t* out, int64_t n, hipStream_t st);  // out has n+1 entries
 std::vector<uint64_t> hash_iteration_order(const blqh_sector& s);
+std::vector<uint64_t> hash_iteration_order_map(const blqh_sector& s);  // from a real std::unordered_map
 
 // Event timing of the fused Lanczos SpMV (blqh_profile_*).
 struct Profile {
@@ -89,6 +107,24 @@ size_t panel_layout_scratch_bytes(const blqh_hamil* H);
 void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scratch, size_t scratch_bytes,
                        bool greedy, int rows = 1);
 size_t panel_layout_scratch_bytes(blqh_hamil* const* hs, int count);
+
+// The CSR that the SpMV formats are built from and the CSR kernel reads:
+// the internal order's when it is on, else the Hamiltonian's own.
+struct CsrView {
+    const int64_t* rowptr;
+    const int32_t* col;
+    const double2* val;
+};
+inline CsrView spmv_csr(const blqh_hamil* H) {
+    if (H->ord.on) return CsrView{H->ord.rowptr.get(), H->ord.col.get(), H->ord.val.get()};
+    return CsrView{H->rowptr.get(), H->col.get(), H->val.get()};
+}
+// Internal order: a permutation of the basis rows (empty = identity) and the
+// moves of vectors between the reference order and the internal one.
+const std::vector<int32_t>& internal_order(const blqh_basis* B);
+void build_order(blqh_hamil* H);
+void order_to_internal(const blqh_hamil* H, const double2* src, double2* dst, hipStream_t st);
+void order_to_reference(const blqh_hamil* H, const double2* src, double2* dst, hipStream_t st);
 
 // Row context with the basis' device pointers (tables left null).
 inline RowCtx make_row_ctx(const blqh_basis* B) {

@@ -16,12 +16,16 @@
 //   the layers; the diagonal (background, charging, bias, Zeeman) comes last.
 // Entries with |h_ij| > 1e-10 are kept, columns ascending.
 //
-// The build runs twice over the rows: a count pass (row lengths), an
-// exclusive scan into rowptr, and a fill pass that writes columns and values.
-// Each row is owned by exactly one wavefront, so its LDS accumulation order is
-// fixed and both passes produce the same entries; the fill pass checks this.
-// Rows whose distinct columns overflow the small hash are redone by a second
-// kernel instance with a large hash.
+// Default: one staged pass.  When a fixed-size staging slot per row fits in a
+// quarter of free device memory, each row is hashed, sorted and written to
+// its slot with its length; the lengths are scanned into rowptr and a copy
+// kernel compacts the slots into the CSR.  Otherwise the build runs twice over
+// the rows: a count pass (row lengths), an exclusive scan into rowptr, and a
+// fill pass that writes columns and values.  Each row is owned by exactly one
+// wavefront, so its LDS accumulation order is fixed and both schemes produce
+// the same entries bit for bit; the fill pass checks this.  Rows whose
+// distinct columns overflow the small hash are redone by a second kernel
+// instance with a large hash.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -624,7 +628,7 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p_in, blqh_ha
     phase(1);
     H->rowptr.alloc(B->n + 1);
     exclusive_scan(nnz.get(), H->rowptr.get(), B->n, st);
-    BLQH_HIP(hipMemcpy(&H->nnz, H->rowptr.get() + B->n, 8, hipMemcpyDeviceToHost));
+    copy_sync(&H->nnz, H->rowptr.get() + B->n, 8, hipMemcpyDeviceToHost);
     H->col.alloc(std::max<int64_t>(H->nnz, 1));
     H->val.alloc(std::max<int64_t>(H->nnz, 1));
     phase(2);

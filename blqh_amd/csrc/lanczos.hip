@@ -929,8 +929,9 @@ void launch_spmv_plan(const blqh_hamil* H, const Plan& pl, const double2* x, dou
     } else if (pl.kind == 1) {
         panel_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
     } else {
-        BLQH_LPR_SWITCH(pl.lpr, k_spmv, dim3(spmv_grid(H->n, pl.lpr)), dim3(kThreads), 0, st, H->n,
-                        H->rowptr.get(), H->col.get(), H->val.get(), x, y);
+        const CsrView A = spmv_csr(H);
+        BLQH_LPR_SWITCH(pl.lpr, k_spmv, dim3(spmv_grid(H->n, pl.lpr)), dim3(kThreads), 0, st, H->n, A.rowptr,
+                        A.col, A.val, x, y);
     }
 }
 
@@ -954,8 +955,9 @@ void launch_lz1(const blqh_hamil* H, const Plan& pl, const double2* W, const dou
     } else if (pl.kind == 1) {
         panel_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
     } else {
-        BLQH_LPR_SWITCH(pl.lpr, k_lz1, dim3(spmv_grid(H->n, pl.lpr)), dim3(kThreads), 0, st, H->n,
-                        H->rowptr.get(), H->col.get(), H->val.get(), W, Wp, u, sc, j, partials, ticket);
+        const CsrView A = spmv_csr(H);
+        BLQH_LPR_SWITCH(pl.lpr, k_lz1, dim3(spmv_grid(H->n, pl.lpr)), dim3(kThreads), 0, st, H->n, A.rowptr,
+                        A.col, A.val, W, Wp, u, sc, j, partials, ticket);
     }
 }
 
@@ -1170,8 +1172,10 @@ static int sturm_count(int m, const double* a, const double* b, double x) {
     return c;
 }
 
-// Lowest eigenvalue by bisection, eigenvector by inverse iteration.
-double tridiag_lowest(int m, const double* a, const double* b, double* vec) {
+// Lowest eigenvalue by bisection, eigenvector by inverse iteration, with
+// caller work space: ws holds 8 m doubles, piv m ints (both used only when
+// vec is given).
+double tridiag_lowest_ws(int m, const double* a, const double* b, double* vec, double* ws, int* piv) {
     double lo = a[0], hi = a[0];
     for (int i = 0; i < m; ++i) {
         const double r = (i ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(b[i]) : 0.0);
@@ -1186,20 +1190,28 @@ double tridiag_lowest(int m, const double* a, const double* b, double* vec) {
     const double e = 0.5 * (lo + hi);
     if (!vec) return e;
     // inverse iteration: (T - e I) x = y with partial pivoting (Gaussian elimination on the band)
-    std::vector<double> x(m, 1.0 / std::sqrt((double)m)), y(m);
+    double* x = ws;
+    double* y = ws + m;
+    double* d0 = ws + 2 * m;
+    double* d1 = ws + 3 * m;
+    double* d2 = ws + 4 * m;
+    double* rhs = ws + 5 * m;
+    double* l = ws + 6 * m;
+    double* sub = ws + 7 * m;
+    for (int i = 0; i < m; ++i) x[i] = 1.0 / std::sqrt((double)m);
     const double shift = e - 1e-14 * std::max(1.0, std::fabs(e));
     for (int iter = 0; iter < 4; ++iter) {
         // band LU of (T - shift I) with row pivoting: rows hold (d0, d1, d2) = cols (i, i+1, i+2)
-        std::vector<double> d0(m), d1(m), d2(m), rhs(x);
-        std::vector<double> l(m, 0.0);
-        std::vector<int> piv(m, 0);
         for (int i = 0; i < m; ++i) {
             d0[i] = a[i] - shift;
             d1[i] = i + 1 < m ? b[i] : 0.0;
             d2[i] = 0.0;
+            rhs[i] = x[i];
+            l[i] = 0.0;
+            piv[i] = 0;
+            sub[i] = 0.0;
         }
         // sub-diagonal of row i+1 is b[i]
-        std::vector<double> sub(m, 0.0);
         for (int i = 0; i + 1 < m; ++i) sub[i + 1] = b[i];
         for (int i = 0; i + 1 < m; ++i) {
             // rows i (pivot candidate) and i+1 (entry sub[i+1] in column i)
@@ -1236,6 +1248,13 @@ double tridiag_lowest(int m, const double* a, const double* b, double* vec) {
     return e;
 }
 
+double tridiag_lowest(int m, const double* a, const double* b, double* vec) {
+    if (!vec) return tridiag_lowest_ws(m, a, b, nullptr, nullptr, nullptr);
+    std::vector<double> ws(8 * (size_t)m);
+    std::vector<int> piv(m);
+    return tridiag_lowest_ws(m, a, b, vec, ws.data(), piv.data());
+}
+
 }  // namespace blqh
 
 // ---- kernel timing -----------------------------------------------------------------
@@ -1257,8 +1276,19 @@ void blqh_device_copy_impl(const void* src, void* dst, int64_t bytes) {
 
 void blqh_spmv_device_impl(blqh_hamil* H, const void* x, void* y) {
     autotune(H);
-    launch_spmv_plan(H, plan_of(H), (const double2*)x, (double2*)y, blqh::stream());
+    hipStream_t st = blqh::stream();
+    if (!H->ord.on) {
+        launch_spmv_plan(H, plan_of(H), (const double2*)x, (double2*)y, st);
+        BLQH_HIP(hipGetLastError());
+        return;
+    }
+    // x and y are in the reference order; the SpMV formats in the internal one
+    DevBuf<double2> xi(H->n), yi(H->n);
+    order_to_internal(H, (const double2*)x, xi.get(), st);
+    launch_spmv_plan(H, plan_of(H), xi.get(), yi.get(), st);
+    order_to_reference(H, yi.get(), (double2*)y, st);
     BLQH_HIP(hipGetLastError());
+    BLQH_HIP(hipStreamSynchronize(st));  // xi / yi are freed on return
 }
 
 namespace {
@@ -1325,50 +1355,67 @@ void device_start(int64_t n, uint32_t seed, const double* start, double2* dst, h
 }
 }  // namespace
 
-void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_result& res, double* psi_out,
-                       double* alpha_out, double* beta_out) {
-    using namespace blqh;
-    auto t0 = std::chrono::steady_clock::now();
+namespace {
+
+// Work areas of one solve.  Device: Krylov block (store: nvec = cap + 1
+// vectors) or ring of three, u, psi, the replay start v0, the per-step
+// scalars (3 cap + 8), coefficients (cap + 1), reductions (4), block
+// partials (npart) and the reduction ticket.  Host: arrays of cap + 1
+// doubles and the tridiagonal work space (8 cap doubles, cap ints).
+struct DevWork {
+    double2* Wb;
+    double2* u;
+    double2* psi;
+    double2* v0;
+    double* sc;
+    double* coef;
+    double* red;
+    double* partials;
+    int64_t npart;
+    unsigned* ticket;
+};
+struct HostWork {
+    double *ha, *hb, *ritz, *svec, *c, *aux, *aux2, *tri;
+    int* piv;
+};
+
+// Passes 1-3 and the Rayleigh quotient of blqh_lanczos on given work areas.
+// start(dst) writes the normalised start vector into device memory dst.
+// timed: bracket the fused SpMV launches with the profiling events.
+template <class Start>
+void lanczos_core(blqh_hamil* H, const Plan& plan, const blqh_lanczos_opts& o, bool store, const DevWork& D,
+                  const HostWork& hw, Start&& start, bool timed, blqh_lanczos_result& res, double* psi_out,
+                  double* alpha_out, double* beta_out, std::chrono::steady_clock::time_point t0) {
     hipStream_t st = stream();
     const int64_t n = H->n;
     const int cap = o.lambda;
-    BLQH_REQUIRE(cap >= 1 && cap <= 100000, BLQH_EINVAL, "lambda out of range");
     const int chk = std::max(1, o.check_every);
-    autotune(H);
-    const Plan plan = plan_of(H);
-
-    // Krylov storage: keep all vectors when they fit comfortably
-    size_t free_b = 0, total_b = 0;
-    BLQH_HIP(hipMemGetInfo(&free_b, &total_b));
-    free_b += dev_cached_bytes();  // cached blocks are released if an allocation needs them
-    const size_t need = (size_t)(cap + 1) * n * sizeof(double2);
     const bool skip = o.skip_states != 0;
-    bool store = !skip && (o.store_basis > 0 || (o.store_basis < 0 && need < free_b / 2));
-    const int nvec = store ? cap + 1 : 3;
-    DevBuf<double2> Wb((size_t)nvec * n), u(n), psi(n);
-    DevBuf<double> sc_buf(3 * (size_t)cap + 8), coef(cap + 1), red(4);
-    const int gmax = std::max(std::max(lz1_grid(H, plan), spmv_grid(n, 4)), vec_grid(n));
-    DevBuf<double> partials(gmax);
-    DevBuf<unsigned> ticket(1);
-    BLQH_HIP(hipMemsetAsync(ticket.get(), 0, sizeof(unsigned), st));
-    Scal sc{sc_buf.get(), sc_buf.get() + cap, sc_buf.get() + 2 * cap + 2};
+    const int gv = vec_grid(n);
+    BLQH_REQUIRE(lz1_grid(H, plan) <= D.npart && spmv_grid(n, 4) <= D.npart && gv <= D.npart, BLQH_EINTERNAL,
+                 "partial-sum buffer smaller than the grid");
+    BLQH_HIP(hipMemsetAsync(D.ticket, 0, sizeof(unsigned), st));
+    Scal sc{D.sc, D.sc + cap, D.sc + 2 * cap + 2};
     const double one = 1.0;
     BLQH_HIP(hipMemcpyAsync(sc.beta, &one, 8, hipMemcpyHostToDevice, st));
     BLQH_HIP(hipMemcpyAsync(sc.invb, &one, 8, hipMemcpyHostToDevice, st));
 
-    auto slot = [&](int j) -> double2* { return Wb.get() + (size_t)(store ? j : j % 3) * n; };
-    DevBuf<double2> v0(store ? 0 : n);  // kept for the pass-3 replay
-    BLQH_REQUIRE(o.start_mix >= 0 && std::isfinite(o.start_mix), BLQH_EINVAL, "start_mix must be finite and >= 0");
-    device_start(n, o.seed, o.start, slot(0), st, o.start_mix);
-    if (!store) BLQH_HIP(hipMemcpyAsync(v0.get(), slot(0), n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    auto slot = [&](int j) -> double2* { return D.Wb + (size_t)(store ? j : j % 3) * n; };
+    if (H->ord.on) {  // start vector in the reference order, the recurrence in the internal one
+        start(D.u);
+        order_to_internal(H, D.u, slot(0), st);
+    } else {
+        start(slot(0));
+    }
+    if (!store) BLQH_HIP(hipMemcpyAsync(D.v0, slot(0), n * sizeof(double2), hipMemcpyDeviceToDevice, st));
 
-    const int gv = vec_grid(n);
     Profile& prof = profile();
+    const bool prof_on = timed && prof.on;
     int ntimed = 0;
     auto step = [&](int j, double2* psi_acc, const double* cf) {
         double2* W = slot(j);
         double2* Wp = j > 0 ? slot(j - 1) : slot(j);
-        if (prof.on) {
+        if (prof_on) {
             while ((int)prof.ev.size() < 2 * (ntimed + 1)) {
                 hipEvent_t e;
                 BLQH_HIP(hipEventCreate(&e));
@@ -1376,32 +1423,41 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
             }
             BLQH_HIP(hipEventRecord(prof.ev[2 * ntimed], st));
         }
-        launch_lz1(H, plan, W, Wp, u.get(), sc, j, partials.get(), ticket.get(), st);
-        if (prof.on) BLQH_HIP(hipEventRecord(prof.ev[2 * ntimed++ + 1], st));
-        hipLaunchKernelGGL(k_lz2, dim3(gv), dim3(kThreads), 0, st, n, u.get(), W, slot(j + 1), sc, j,
-                           psi_acc ? 1 : 0, psi_acc, cf, cap, partials.get(), ticket.get());
+        launch_lz1(H, plan, W, Wp, D.u, sc, j, D.partials, D.ticket, st);
+        if (prof_on) BLQH_HIP(hipEventRecord(prof.ev[2 * ntimed++ + 1], st));
+        hipLaunchKernelGGL(k_lz2, dim3(gv), dim3(kThreads), 0, st, n, D.u, W, slot(j + 1), sc, j,
+                           psi_acc ? 1 : 0, psi_acc, cf, cap, D.partials, D.ticket);
+    };
+    auto account = [&]() {
+        if (!prof_on) return;
+        for (int k = 0; k < ntimed; ++k) {
+            float ms = 0;
+            BLQH_HIP(hipEventElapsedTime(&ms, prof.ev[2 * k], prof.ev[2 * k + 1]));
+            prof.ms += ms;
+        }
+        prof.launches += ntimed;
+        prof.bytes += (double)ntimed * (20.0 * H->nnz + 56.0 * n + 8.0);
     };
 
     // pass 1
-    std::vector<double> ha(cap), hb(cap + 1), ritz;
+    double* ha = hw.ha;
+    double* hb = hw.hb;
     int m = 0;  // Krylov vectors in use
     bool conv = false;
     double dE = INFINITY, resid = INFINITY;
-    std::vector<double> svec(cap);
     int done = 0;
     while (done < cap && !conv) {
         const int upto = std::min(cap, done + chk);
         for (int j = done; j < upto; ++j) step(j, nullptr, nullptr);
         BLQH_HIP(hipGetLastError());
-        BLQH_HIP(hipMemcpyAsync(ha.data(), sc.alpha, upto * 8, hipMemcpyDeviceToHost, st));
-        BLQH_HIP(hipMemcpyAsync(hb.data(), sc.beta, (upto + 1) * 8, hipMemcpyDeviceToHost, st));
+        BLQH_HIP(hipMemcpyAsync(ha, sc.alpha, upto * 8, hipMemcpyDeviceToHost, st));
+        BLQH_HIP(hipMemcpyAsync(hb, sc.beta, (upto + 1) * 8, hipMemcpyDeviceToHost, st));
         BLQH_HIP(hipStreamSynchronize(st));
         for (int j = done; j < upto; ++j) {
             // T_{j+1}: alpha[0..j], off-diagonal beta[1..j]
-            const double e = tridiag_lowest(j + 1, ha.data(), hb.data() + 1, nullptr);
-            ritz.push_back(e);
+            hw.ritz[j] = tridiag_lowest_ws(j + 1, ha, hb + 1, nullptr, nullptr, nullptr);
             m = j + 1;
-            if (j > 0) dE = std::fabs(ritz[j] - ritz[j - 1]);
+            if (j > 0) dE = std::fabs(hw.ritz[j] - hw.ritz[j - 1]);
             const double scale = std::max(std::fabs(ha[j]), 1.0);
             if (!(hb[j + 1] > 1e-13 * scale) || !std::isfinite(hb[j + 1])) {  // invariant subspace
                 resid = 0;
@@ -1411,20 +1467,20 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
             if (o.tol > 0 && j > 0 && dE < o.tol) {
                 if (o.res_tol <= 0) { conv = true; break; }
                 // residual of the Ritz pair: |H y - theta y| = beta_{j+1} |s_j|
-                (void)tridiag_lowest(j + 1, ha.data(), hb.data() + 1, svec.data());
-                resid = hb[j + 1] * std::fabs(svec[j]);
+                (void)tridiag_lowest_ws(j + 1, ha, hb + 1, hw.svec, hw.tri, hw.piv);
+                resid = hb[j + 1] * std::fabs(hw.svec[j]);
                 if (resid < o.res_tol) { conv = true; break; }
             }
         }
         done = upto;
     }
-    BLQH_REQUIRE(std::isfinite(ritz.back()), BLQH_EINTERNAL, "Lanczos produced non-finite coefficients");
+    BLQH_REQUIRE(std::isfinite(hw.ritz[m - 1]), BLQH_EINTERNAL, "Lanczos produced non-finite coefficients");
 
     // pass 2: lowest eigenpair of T_m
-    std::vector<double> c(m);
-    const double E0 = tridiag_lowest(m, ha.data(), hb.data() + 1, c.data());
-    if (alpha_out) { std::fill(alpha_out, alpha_out + cap, 0.0); std::copy(ha.begin(), ha.begin() + m, alpha_out); }
-    if (beta_out) { std::fill(beta_out, beta_out + cap + 1, 0.0); std::copy(hb.begin(), hb.begin() + m + 1, beta_out); }
+    double* c = hw.c;
+    const double E0 = tridiag_lowest_ws(m, ha, hb + 1, c, hw.tri, hw.piv);
+    if (alpha_out) { std::fill(alpha_out, alpha_out + cap, 0.0); std::copy(ha, ha + m, alpha_out); }
+    if (beta_out) { std::fill(beta_out, beta_out + cap + 1, 0.0); std::copy(hb, hb + m + 1, beta_out); }
     res.E0 = E0;
     res.dE = dE;
     res.residual = std::isfinite(resid) ? resid : hb[m] * std::fabs(c[m - 1]);
@@ -1434,70 +1490,95 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
         res.E_rq = NAN;
         res.passes = 1;
         BLQH_HIP(hipStreamSynchronize(st));
-        if (prof.on) {
-            for (int k = 0; k < ntimed; ++k) {
-                float ms = 0;
-                BLQH_HIP(hipEventElapsedTime(&ms, prof.ev[2 * k], prof.ev[2 * k + 1]));
-                prof.ms += ms;
-            }
-            prof.launches += ntimed;
-            prof.bytes += (double)ntimed * (20.0 * H->nnz + 56.0 * n + 8.0);
-        }
+        account();
         res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return;
     }
     // pass 3
-    std::vector<double> invb(m);
-    {
-        std::vector<double> hib(m);
-        BLQH_HIP(hipMemcpy(hib.data(), sc.invb, m * 8, hipMemcpyDeviceToHost));
-        invb = hib;
-    }
     if (store) {
-        std::vector<double> w(m);
+        double* invb = hw.aux;
+        copy_sync(invb, sc.invb, m * 8, hipMemcpyDeviceToHost);
+        double* w = hw.aux2;
         for (int l = 0; l < m; ++l) w[l] = c[l] * invb[l];
-        BLQH_HIP(hipMemcpyAsync(coef.get(), w.data(), m * 8, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_combine, dim3(gv), dim3(kThreads), 0, st, n, m, Wb.get(), n, coef.get(), psi.get());
+        BLQH_HIP(hipMemcpyAsync(D.coef, w, m * 8, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_combine, dim3(gv), dim3(kThreads), 0, st, n, m, D.Wb, n, D.coef, D.psi);
         res.passes = 2;
     } else {
         // replay with the same kernels: alpha/beta are regenerated bit for bit
-        BLQH_HIP(hipMemcpyAsync(coef.get(), c.data(), m * 8, hipMemcpyHostToDevice, st));
-        BLQH_HIP(hipMemsetAsync(psi.get(), 0, n * sizeof(double2), st));
-        BLQH_HIP(hipMemcpyAsync(slot(0), v0.get(), n * sizeof(double2), hipMemcpyDeviceToDevice, st));
-        for (int j = 0; j + 1 < m; ++j) step(j, psi.get(), coef.get());
+        BLQH_HIP(hipMemcpyAsync(D.coef, c, m * 8, hipMemcpyHostToDevice, st));
+        BLQH_HIP(hipMemsetAsync(D.psi, 0, n * sizeof(double2), st));
+        BLQH_HIP(hipMemcpyAsync(slot(0), D.v0, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+        for (int j = 0; j + 1 < m; ++j) step(j, D.psi, D.coef);
         // last vector's weight needs invb[m-1], which the replay just recomputed
-        std::vector<double> hb2(m);
-        BLQH_HIP(hipMemcpyAsync(hb2.data(), sc.invb, m * 8, hipMemcpyDeviceToHost, st));
-        std::vector<double> ha2(m);
-        BLQH_HIP(hipMemcpyAsync(ha2.data(), sc.alpha, (m - 1) * 8, hipMemcpyDeviceToHost, st));
+        double* hb2 = hw.aux;
+        double* ha2 = hw.aux2;
+        BLQH_HIP(hipMemcpyAsync(hb2, sc.invb, m * 8, hipMemcpyDeviceToHost, st));
+        if (m > 1) BLQH_HIP(hipMemcpyAsync(ha2, sc.alpha, (m - 1) * 8, hipMemcpyDeviceToHost, st));
         BLQH_HIP(hipStreamSynchronize(st));
         for (int j = 0; j + 1 < m; ++j)
             BLQH_REQUIRE(ha2[j] == ha[j], BLQH_EINTERNAL, "pass-3 replay diverged from pass 1");
-        hipLaunchKernelGGL(k_accum, dim3(gv), dim3(kThreads), 0, st, n, slot(m - 1), c[m - 1] * hb2[m - 1], psi.get());
+        hipLaunchKernelGGL(k_accum, dim3(gv), dim3(kThreads), 0, st, n, slot(m - 1), c[m - 1] * hb2[m - 1], D.psi);
         res.passes = 3;
     }
     // normalise and Rayleigh quotient
-    hipLaunchKernelGGL(k_norm2, dim3(gv), dim3(kThreads), 0, st, n, psi.get(), red.get(), partials.get(), ticket.get());
-    hipLaunchKernelGGL(k_scale, dim3(gv), dim3(kThreads), 0, st, n, psi.get(), red.get());
-    launch_spmv_plan(H, plan, psi.get(), u.get(), st);
-    hipLaunchKernelGGL(k_redot, dim3(gv), dim3(kThreads), 0, st, n, psi.get(), u.get(), red.get() + 1, partials.get(),
-                       ticket.get());
+    hipLaunchKernelGGL(k_norm2, dim3(gv), dim3(kThreads), 0, st, n, D.psi, D.red, D.partials, D.ticket);
+    hipLaunchKernelGGL(k_scale, dim3(gv), dim3(kThreads), 0, st, n, D.psi, D.red);
+    launch_spmv_plan(H, plan, D.psi, D.u, st);
+    hipLaunchKernelGGL(k_redot, dim3(gv), dim3(kThreads), 0, st, n, D.psi, D.u, D.red + 1, D.partials, D.ticket);
     BLQH_HIP(hipGetLastError());
     double erq = 0;
-    BLQH_HIP(hipMemcpyAsync(&erq, red.get() + 1, 8, hipMemcpyDeviceToHost, st));
-    if (psi_out) BLQH_HIP(hipMemcpyAsync(psi_out, psi.get(), n * sizeof(double2), hipMemcpyDeviceToHost, st));
-    BLQH_HIP(hipStreamSynchronize(st));
-    if (prof.on) {
-        for (int k = 0; k < ntimed; ++k) {
-            float ms = 0;
-            BLQH_HIP(hipEventElapsedTime(&ms, prof.ev[2 * k], prof.ev[2 * k + 1]));
-            prof.ms += ms;
+    BLQH_HIP(hipMemcpyAsync(&erq, D.red + 1, 8, hipMemcpyDeviceToHost, st));
+    if (psi_out) {
+        const double2* out = D.psi;
+        if (H->ord.on) {
+            order_to_reference(H, D.psi, D.u, st);
+            out = D.u;
         }
-        prof.launches += ntimed;
-        prof.bytes += (double)ntimed * (20.0 * H->nnz + 56.0 * n + 8.0);
+        BLQH_HIP(hipMemcpyAsync(psi_out, out, n * sizeof(double2), hipMemcpyDeviceToHost, st));
     }
+    BLQH_HIP(hipStreamSynchronize(st));
+    account();
     res.E_rq = erq;
     res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_result& res, double* psi_out,
+                       double* alpha_out, double* beta_out) {
+    using namespace blqh;
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = stream();
+    const int64_t n = H->n;
+    const int cap = o.lambda;
+    BLQH_REQUIRE(cap >= 1 && cap <= 100000, BLQH_EINVAL, "lambda out of range");
+    BLQH_REQUIRE(o.start_mix >= 0 && std::isfinite(o.start_mix), BLQH_EINVAL, "start_mix must be finite and >= 0");
+    autotune(H);
+    const Plan plan = plan_of(H);
+
+    // Krylov storage: keep all vectors when they fit comfortably
+    size_t free_b = 0, total_b = 0;
+    BLQH_HIP(hipMemGetInfo(&free_b, &total_b));
+    free_b += dev_cached_bytes();  // cached blocks are released if an allocation needs them
+    const size_t need = (size_t)(cap + 1) * n * sizeof(double2);
+    const bool skip = o.skip_states != 0;
+    const bool store = !skip && (o.store_basis > 0 || (o.store_basis < 0 && need < free_b / 2));
+    const int nvec = store ? cap + 1 : 3;
+    DevBuf<double2> Wb((size_t)nvec * n), u(n), psi(n), v0(store ? 0 : n);
+    DevBuf<double> sc_buf(3 * (size_t)cap + 8), coef(cap + 1), red(4);
+    const int gmax = std::max(std::max(lz1_grid(H, plan), spmv_grid(n, 4)), vec_grid(n));
+    DevBuf<double> partials(gmax);
+    DevBuf<unsigned> ticket(1);
+    std::vector<double> hbuf(7 * ((size_t)cap + 1) + 8 * (size_t)cap);
+    std::vector<int> piv(cap);
+    double* hp = hbuf.data();
+    const size_t L = (size_t)cap + 1;
+    const HostWork hw{hp, hp + L, hp + 2 * L, hp + 3 * L, hp + 4 * L, hp + 5 * L, hp + 6 * L, hp + 7 * L, piv.data()};
+    const DevWork D{Wb.get(), u.get(), psi.get(), v0.get(), sc_buf.get(), coef.get(), red.get(), partials.get(),
+                    gmax, ticket.get()};
+    lanczos_core(H, plan, o, store, D, hw,
+                 [&](double2* dst) { device_start(n, o.seed, o.start, dst, st, o.start_mix); }, true, res, psi_out,
+                 alpha_out, beta_out, t0);
 }
 
 void blqh_lanczos_reconstruct_impl(blqh_hamil* H, int m, uint32_t seed, const double* start, const double* alpha,
@@ -1532,7 +1613,12 @@ void blqh_lanczos_reconstruct_impl(blqh_hamil* H, int m, uint32_t seed, const do
     BLQH_HIP(hipMemcpyAsync(sc.invb, &one, 8, hipMemcpyHostToDevice, st));
 
     auto slot = [&](int j) -> double2* { return Wb.get() + (size_t)(j % 3) * n; };
-    device_start(n, seed, start, slot(0), st);
+    if (H->ord.on) {
+        device_start(n, seed, start, u.get(), st);
+        order_to_internal(H, u.get(), slot(0), st);
+    } else {
+        device_start(n, seed, start, slot(0), st);
+    }
     BLQH_HIP(hipMemcpyAsync(coef.get(), coefs, (size_t)ns * m * 8, hipMemcpyHostToDevice, st));
     BLQH_HIP(hipMemsetAsync(psi.get(), 0, (size_t)ns * n * sizeof(double2), st));
     const int gv = vec_grid(n);
@@ -1571,7 +1657,15 @@ void blqh_lanczos_reconstruct_impl(blqh_hamil* H, int m, uint32_t seed, const do
     BLQH_HIP(hipGetLastError());
     std::vector<double> hred(2 * ns);
     BLQH_HIP(hipMemcpyAsync(hred.data(), red.get(), 2 * ns * 8, hipMemcpyDeviceToHost, st));
-    if (psi_out) BLQH_HIP(hipMemcpyAsync(psi_out, psi.get(), (size_t)ns * n * sizeof(double2), hipMemcpyDeviceToHost, st));
+    if (psi_out && H->ord.on) {
+        for (int s = 0; s < ns; ++s) {  // back to the reference order, through u
+            order_to_reference(H, psi.get() + (size_t)s * n, u.get(), st);
+            BLQH_HIP(hipMemcpyAsync(psi_out + (size_t)s * 2 * n, u.get(), n * sizeof(double2), hipMemcpyDeviceToHost,
+                                    st));
+        }
+    } else if (psi_out) {
+        BLQH_HIP(hipMemcpyAsync(psi_out, psi.get(), (size_t)ns * n * sizeof(double2), hipMemcpyDeviceToHost, st));
+    }
     BLQH_HIP(hipStreamSynchronize(st));
     for (int s = 0; s < ns; ++s) {
         BLQH_REQUIRE(hred[2 * s] > 0, BLQH_EINVAL, "a reconstructed state is zero (coefficients all zero?)");

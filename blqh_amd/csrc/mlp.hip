@@ -355,7 +355,7 @@ blqh_mlp* blqh_mlp_create_impl(int nbits, uint32_t seed) {
         M->m.alloc(n);
         M->v.alloc(n);
         M->lsum.alloc(1);
-        BLQH_HIP(hipMemcpy(M->P.get(), h.data(), n * 4, hipMemcpyHostToDevice));
+        copy_sync(M->P.get(), h.data(), n * 4, hipMemcpyHostToDevice);
         BLQH_HIP(hipMemset(M->m.get(), 0, n * 4));
         BLQH_HIP(hipMemset(M->v.get(), 0, n * 4));
     } catch (...) {
@@ -370,11 +370,11 @@ int blqh_mlp_nparams_impl(const blqh_mlp* M) { return M->D.n; }
 int blqh_mlp_nbits_impl(const blqh_mlp* M) { return M->D.nbits; }
 
 void blqh_mlp_get_params_impl(const blqh_mlp* M, float* out) {
-    BLQH_HIP(hipMemcpy(out, M->P.get(), M->D.n * 4, hipMemcpyDeviceToHost));
+    copy_sync(out, M->P.get(), M->D.n * 4, hipMemcpyDeviceToHost);
 }
 
 void blqh_mlp_set_params_impl(blqh_mlp* M, const float* in) {
-    BLQH_HIP(hipMemcpy(M->P.get(), in, M->D.n * 4, hipMemcpyHostToDevice));
+    copy_sync(M->P.get(), in, M->D.n * 4, hipMemcpyHostToDevice);
     BLQH_HIP(hipMemset(M->m.get(), 0, M->D.n * 4));
     BLQH_HIP(hipMemset(M->v.get(), 0, M->D.n * 4));
     M->adam_t = 0;

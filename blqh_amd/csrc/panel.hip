@@ -591,7 +591,7 @@ void relayout_pass(const std::vector<blqh_hamil*>& todo, bool jagged, bool greed
     // earlier work on the library stream may still read this scratch; then a
     // synchronous copy of the job table (the host vector goes out of scope)
     BLQH_HIP(hipStreamSynchronize(st));
-    BLQH_HIP(hipMemcpy(sc, jobs.data(), jobs.size() * sizeof(RelayoutJob), hipMemcpyHostToDevice));
+    copy_sync(sc, jobs.data(), jobs.size() * sizeof(RelayoutJob), hipMemcpyHostToDevice);
     const RelayoutJob* dj = (const RelayoutJob*)sc;
     const auto t0 = std::chrono::steady_clock::now();
     unsigned max_e = 0;
@@ -661,8 +661,9 @@ void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scra
         if (!P.built || (P.jagged == jagged && (!jagged || (P.greedy == greedy && P.rows == rows)))) continue;
         if (P.jagged && (P.greedy || P.rows > 1)) {
             const int gr = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(H->n, kBlk), 1), 65536);
-            hipLaunchKernelGGL(k_panel_rescatter, dim3(gr), dim3(kBlk), 0, st, H->n, H->rowptr.get(), H->col.get(),
-                               H->val.get(), P.pbase.get(), P.prow.get(), P.real ? 1 : 0, P.pcol.get(),
+            const CsrView A = spmv_csr(H);
+            hipLaunchKernelGGL(k_panel_rescatter, dim3(gr), dim3(kBlk), 0, st, H->n, A.rowptr, A.col, A.val,
+                               P.pbase.get(), P.prow.get(), P.real ? 1 : 0, P.pcol.get(),
                                P.pvr.get(), P.pvc.get());
             BLQH_HIP(hipGetLastError());
             P.jagged = P.greedy = false;
@@ -685,7 +686,189 @@ size_t panel_layout_scratch_bytes(const blqh_hamil* H) {
     return panel_layout_scratch_bytes(&h, 1);
 }
 
+// ---- internal order of the SpMV formats -----------------------------------------
+//
+// A basis state is a translation orbit: its members differ by one common
+// rotation of every species word.  The SpMV formats order the rows by a key
+// of the orbit that ignores which member the reference picked: rotate the
+// state so that the lower layer's word x is smallest (ties: the upper
+// layer's word y smallest), then key = (x, necklace class of y, position of
+// y in its necklace).  Rows of one 64-row slice then share x, so the hops
+// that move only lower-layer electrons send neighbouring rows to
+// neighbouring columns (the upper-layer word only turns inside its
+// necklace), and every other hop stays inside a few blocks of one x.  At
+// 2.35M states (N_phi = 16, all densities) this raises the lanes that hold an
+// entry at a step of the step-major panels from 24 to 54 of 64 and halves
+// the distinct x cache lines a step gathers from.  The Lanczos recurrence is
+// invariant under a permutation of the basis, so only the start vector and
+// the returned states move between the orders.
+const std::vector<int32_t>& internal_order(const blqh_basis* B) {
+    std::lock_guard<std::mutex> lk(B->ord_mu);
+    if (B->ord_done) return B->h_order;
+    B->ord_done = true;
+    const char* env = std::getenv("BLQH_ORDER");
+    const int S = B->geo.S, nphi = B->geo.nphi;
+    const int64_t n = B->n;
+    if ((env && env[0] == '0') || B->sec.K < 0 || n < 2 || S < 2) return B->h_order;
+    const int half = S / 2;  // species [0, half): upper layer; [half, S): lower layer
+    const uint64_t m = nphi >= 64 ? ~0ull : ((1ull << nphi) - 1);
+    auto group = [&](uint64_t w, int s0, int s1, int r) {
+        uint64_t g = 0;
+        for (int s = s0; s < s1; ++s) {
+            const uint64_t x = (w >> (s * nphi)) & m;
+            const uint64_t y = r ? (((x << r) | (x >> (nphi - r))) & m) : x;
+            g |= y << ((s - s0) * nphi);
+        }
+        return g;
+    };
+    struct Key {
+        uint64_t a, b;
+        int32_t i;
+    };
+    std::vector<Key> keys(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t w = B->h_id[i];
+        uint64_t bx = 0, by = 0, cls = 0;
+        int rs = -1, r0 = 0;
+        uint64_t ys[64];
+        for (int r = 0; r < nphi; ++r) {
+            const uint64_t x = group(w, half, S, r), y = group(w, 0, half, r);
+            ys[r] = y;
+            if (rs < 0 || x < bx || (x == bx && y < by)) { bx = x; by = y; rs = r; }
+            if (r == 0 || y < cls) { cls = y; r0 = r; }
+        }
+        int ridx = 0;
+        while (ys[(r0 + ridx) % nphi] != by) ++ridx;  // by is a rotation of cls: terminates
+        keys[i] = Key{bx, (cls << 6) | (uint64_t)ridx, (int32_t)i};
+    }
+    std::sort(keys.begin(), keys.end(), [](const Key& p, const Key& q) {
+        return p.a != q.a ? p.a < q.a : p.b != q.b ? p.b < q.b : p.i < q.i;
+    });
+    B->h_order.resize(n);
+    for (int64_t r = 0; r < n; ++r) B->h_order[keys[r].i] = (int32_t)r;
+    return B->h_order;
+}
+
+namespace {
+constexpr int kOrdMaxRow = 4096;  // longest CSR row (the build's largest row hash)
+
+__global__ void k_ord_len(int64_t n, const int32_t* __restrict__ iperm, const int64_t* __restrict__ rowptr,
+                          int32_t* __restrict__ len) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t o = iperm[r];
+        len[r] = (int32_t)(rowptr[o + 1] - rowptr[o]);
+    }
+}
+
+// Row r of the internal CSR: reference row iperm[r] with its columns mapped
+// through perm, placed in ascending order by rank (columns of a row are
+// distinct).  One wavefront (workgroup) per row; rows longer than
+// kOrdMaxRow are counted in *err and skipped.
+__global__ void __launch_bounds__(64) k_ord_fill(int64_t n, const int32_t* __restrict__ iperm,
+                                                 const int32_t* __restrict__ perm, const int64_t* __restrict__ rowptr,
+                                                 const int32_t* __restrict__ col, const double2* __restrict__ val,
+                                                 const int64_t* __restrict__ orp, int32_t* __restrict__ ocol,
+                                                 double2* __restrict__ oval, int32_t* __restrict__ err) {
+    __shared__ int32_t key[kOrdMaxRow];
+    const int lane = threadIdx.x;
+    for (int64_t r = blockIdx.x; r < n; r += gridDim.x) {
+        const int64_t o = iperm[r];
+        const int64_t s = rowptr[o];
+        const int len = (int)(rowptr[o + 1] - s);
+        if (len > kOrdMaxRow) {
+            if (lane == 0) atomicAdd(err, 1);
+            continue;  // uniform across the workgroup
+        }
+        __syncthreads();  // the previous row's keys are read
+        for (int k = lane; k < len; k += 64) key[k] = perm[col[s + k]];
+        __syncthreads();
+        const int64_t d = orp[r];
+        for (int k = lane; k < len; k += 64) {
+            const int32_t c = key[k];
+            int rank = 0;
+            for (int q = 0; q < len; ++q) rank += key[q] < c ? 1 : 0;
+            ocol[d + rank] = c;
+            oval[d + rank] = val[s + k];
+        }
+    }
+}
+
+// internal[r] = reference[iperm[r]]
+__global__ void k_ord_gather(int64_t n, const int32_t* __restrict__ iperm, const double2* __restrict__ src,
+                             double2* __restrict__ dst) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        dst[r] = src[iperm[r]];
+}
+
+// reference[iperm[r]] = internal[r]
+__global__ void k_ord_scatter(int64_t n, const int32_t* __restrict__ iperm, const double2* __restrict__ src,
+                              double2* __restrict__ dst) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        dst[iperm[r]] = src[r];
+}
+
+int ord_grid(int64_t n) { return (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, kBlk), 1), 65536); }
+}  // namespace
+
+// The internal CSR of H from its CSR and the basis order (no-op for the identity order).
+void build_order(blqh_hamil* H) {
+    blqh_hamil::Order& O = H->ord;
+    O.on = false;
+    const std::vector<int32_t>& perm = internal_order(H->basis);
+    if (perm.empty()) return;
+    const int64_t n = H->n;
+    BLQH_REQUIRE((int64_t)perm.size() == n, BLQH_EINTERNAL, "internal order and Hamiltonian sizes differ");
+    hipStream_t st = stream();
+    O.h_iperm.assign(n, 0);
+    for (int64_t i = 0; i < n; ++i) O.h_iperm[perm[i]] = (int32_t)i;
+    O.perm.alloc(n);
+    O.iperm.alloc(n);
+    BLQH_HIP(hipMemcpyAsync(O.perm.get(), perm.data(), n * 4, hipMemcpyHostToDevice, st));
+    BLQH_HIP(hipMemcpyAsync(O.iperm.get(), O.h_iperm.data(), n * 4, hipMemcpyHostToDevice, st));
+    DevBuf<int32_t> len(n);
+    DevBuf<int32_t> err(1);
+    BLQH_HIP(hipMemsetAsync(err.get(), 0, 4, st));
+    hipLaunchKernelGGL(k_ord_len, dim3(ord_grid(n)), dim3(kBlk), 0, st, n, O.iperm.get(), H->rowptr.get(), len.get());
+    BLQH_HIP(hipGetLastError());
+    O.rowptr.alloc(n + 1);
+    exclusive_scan(len.get(), O.rowptr.get(), n, st);
+    O.col.alloc(std::max<int64_t>(H->nnz, 1));
+    O.val.alloc(std::max<int64_t>(H->nnz, 1));
+    const int g = (int)std::min<int64_t>(std::max<int64_t>(n, 1), (int64_t)runtime().num_cus * 40);
+    hipLaunchKernelGGL(k_ord_fill, dim3(g), dim3(64), 0, st, n, O.iperm.get(), O.perm.get(), H->rowptr.get(),
+                       H->col.get(), H->val.get(), O.rowptr.get(), O.col.get(), O.val.get(), err.get());
+    BLQH_HIP(hipGetLastError());
+    int32_t herr = 0;
+    int64_t tot = 0;
+    BLQH_HIP(hipMemcpyAsync(&herr, err.get(), 4, hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipMemcpyAsync(&tot, O.rowptr.get() + n, 8, hipMemcpyDeviceToHost, st));
+    BLQH_HIP(hipStreamSynchronize(st));
+    BLQH_REQUIRE(herr == 0, BLQH_EINTERNAL, "a CSR row is longer than the internal-order kernel handles");
+    BLQH_REQUIRE(tot == H->nnz, BLQH_EINTERNAL, "internal CSR size differs");
+    O.on = true;
+}
+
+void order_to_internal(const blqh_hamil* H, const double2* src, double2* dst, hipStream_t st) {
+    if (!H->ord.on) {
+        if (src != dst) BLQH_HIP(hipMemcpyAsync(dst, src, H->n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+        return;
+    }
+    hipLaunchKernelGGL(k_ord_gather, dim3(ord_grid(H->n)), dim3(kBlk), 0, st, H->n, H->ord.iperm.get(), src, dst);
+    BLQH_HIP(hipGetLastError());
+}
+
+void order_to_reference(const blqh_hamil* H, const double2* src, double2* dst, hipStream_t st) {
+    if (!H->ord.on) {
+        if (src != dst) BLQH_HIP(hipMemcpyAsync(dst, src, H->n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+        return;
+    }
+    hipLaunchKernelGGL(k_ord_scatter, dim3(ord_grid(H->n)), dim3(kBlk), 0, st, H->n, H->ord.iperm.get(), src, dst);
+    BLQH_HIP(hipGetLastError());
+}
+
 void build_panels(blqh_hamil* H) {
+    build_order(H);
+    const CsrView A = spmv_csr(H);
     hipStream_t st = stream();
     const int64_t n = H->n;
     const int npan = (int)std::max<int64_t>(1, ceil_div(n, (int64_t)1 << kPanelShift));
@@ -694,7 +877,7 @@ void build_panels(blqh_hamil* H) {
     DevBuf<unsigned long long> mx(2);
     BLQH_HIP(hipMemsetAsync(mx.get(), 0, 16, st));
     const int g = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(H->nnz, kBlk * 8), 1), 2048);
-    hipLaunchKernelGGL(k_absmax, dim3(g), dim3(kBlk), 0, st, H->nnz, H->val.get(), mx.get());
+    hipLaunchKernelGGL(k_absmax, dim3(g), dim3(kBlk), 0, st, H->nnz, A.val, mx.get());
     unsigned long long hm[2];
     BLQH_HIP(hipMemcpyAsync(hm, mx.get(), 16, hipMemcpyDeviceToHost, st));
     BLQH_HIP(hipStreamSynchronize(st));
@@ -705,7 +888,7 @@ void build_panels(blqh_hamil* H) {
     DevBuf<int32_t> cnt((size_t)npan * n);
     DevBuf<int64_t> off((size_t)npan * n + 1);
     const int gr = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, kBlk), 1), 65536);
-    hipLaunchKernelGGL(k_panel_count, dim3(gr), dim3(kBlk), 0, st, n, npan, H->rowptr.get(), H->col.get(), cnt.get());
+    hipLaunchKernelGGL(k_panel_count, dim3(gr), dim3(kBlk), 0, st, n, npan, A.rowptr, A.col, cnt.get());
     BLQH_HIP(hipGetLastError());
     exclusive_scan(cnt.get(), off.get(), (int64_t)npan * n, st);
     P.prow.alloc((size_t)npan * (n + 1));
@@ -713,10 +896,20 @@ void build_panels(blqh_hamil* H) {
     const int gp = (int)std::min<int64_t>(ceil_div((int64_t)npan * (n + 1), kBlk), 65536);
     hipLaunchKernelGGL(k_panel_rows, dim3(gp), dim3(kBlk), 0, st, n, npan, off.get(), H->nnz, P.prow.get(),
                        P.pbase.get());
+    BLQH_HIP(hipGetLastError());
+    {
+        // prow holds row offsets inside a panel as 32-bit values
+        std::vector<int64_t> pb(npan + 1);
+        BLQH_HIP(hipMemcpyAsync(pb.data(), P.pbase.get(), (npan + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        BLQH_HIP(hipStreamSynchronize(st));
+        for (int p = 0; p < npan; ++p)
+            if (pb[p + 1] - pb[p] > (int64_t)UINT32_MAX)
+                throw Error(BLQH_ERANGE, "a column panel holds 2^32 or more entries (32-bit panel row offsets)");
+    }
     P.pcol.alloc(std::max<int64_t>(H->nnz, 1));
     if (P.real) P.pvr.alloc(std::max<int64_t>(H->nnz, 1));
     else P.pvc.alloc(std::max<int64_t>(H->nnz, 1));
-    hipLaunchKernelGGL(k_panel_scatter, dim3(gr), dim3(kBlk), 0, st, n, H->rowptr.get(), H->col.get(), H->val.get(),
+    hipLaunchKernelGGL(k_panel_scatter, dim3(gr), dim3(kBlk), 0, st, n, A.rowptr, A.col, A.val,
                        off.get(), P.real ? 1 : 0, P.pcol.get(), P.pvr.get(), P.pvc.get());
     BLQH_HIP(hipGetLastError());
     BLQH_HIP(hipStreamSynchronize(st));  // cnt / off are freed on return

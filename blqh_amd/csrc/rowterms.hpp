@@ -32,18 +32,25 @@ struct RowCtx {
 
 BLQH_HD double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
-// s mod d for 0 <= s < 2^16, d >= 1.  On the device the quotient comes from
-// the FP32 reciprocal (off by at most one for such s, corrected below)
-// instead of an integer division sequence.
+// s mod d from an FP32 reciprocal r of d: the truncated quotient s * r is off
+// by at most one for 0 <= s < 2^16, 1 <= d <= 64 and r within 2 ulp of 1/d,
+// and one correction step either way makes the remainder exact.
+BLQH_HD int mod_by_rcp(int s, int d, float r) {
+    const int q = (int)((float)s * r);
+    int rem = s - q * d;
+    if (rem < 0)
+        rem += d;
+    else if (rem >= d)
+        rem -= d;
+    return rem;
+}
+
+// s mod d for 0 <= s < 2^16, 1 <= d <= 64.  On the device the quotient comes
+// from the hardware's FP32 reciprocal (mod_by_rcp) instead of an integer
+// division sequence.
 BLQH_HD int mod_small(int s, int d) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    const int q = (int)((float)s * __builtin_amdgcn_rcpf((float)d));
-    int r = s - q * d;
-    if (r < 0)
-        r += d;
-    else if (r >= d)
-        r -= d;
-    return r;
+    return mod_by_rcp(s, d, __builtin_amdgcn_rcpf((float)d));
 #else
     return s % d;
 #endif

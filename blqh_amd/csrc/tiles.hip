@@ -154,7 +154,8 @@ bool build_tiles(blqh_hamil* H, int G) {
     BLQH_HIP(hipMemsetAsync(len.get(), 0, len.bytes(), st));
     BLQH_HIP(hipMemsetAsync(ovf.get(), 0, sizeof(int), st));
     const int gr = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, kBlk), 1), 65536);
-    hipLaunchKernelGGL(k_tile_len, dim3(gr), dim3(kBlk), 0, st, g, H->rowptr.get(), H->col.get(), len.get(),
+    const CsrView A = spmv_csr(H);
+    hipLaunchKernelGGL(k_tile_len, dim3(gr), dim3(kBlk), 0, st, g, A.rowptr, A.col, len.get(),
                        ovf.get());
     int hovf = 0;
     BLQH_HIP(hipMemcpyAsync(&hovf, ovf.get(), sizeof(int), hipMemcpyDeviceToHost, st));
@@ -172,11 +173,11 @@ bool build_tiles(blqh_hamil* H, int G) {
     if (T.real) T.vr.alloc(m);
     else T.vc.alloc(m);
     if (T.real)
-        hipLaunchKernelGGL(k_tile_scatter<true>, dim3(gw), dim3(kBlk), 0, st, g, H->rowptr.get(), H->col.get(),
-                           H->val.get(), len.get(), ws.get(), T.col.get(), T.vr.get(), T.vc.get());
+        hipLaunchKernelGGL(k_tile_scatter<true>, dim3(gw), dim3(kBlk), 0, st, g, A.rowptr, A.col,
+                           A.val, len.get(), ws.get(), T.col.get(), T.vr.get(), T.vc.get());
     else
-        hipLaunchKernelGGL(k_tile_scatter<false>, dim3(gw), dim3(kBlk), 0, st, g, H->rowptr.get(), H->col.get(),
-                           H->val.get(), len.get(), ws.get(), T.col.get(), T.vr.get(), T.vc.get());
+        hipLaunchKernelGGL(k_tile_scatter<false>, dim3(gw), dim3(kBlk), 0, st, g, A.rowptr, A.col,
+                           A.val, len.get(), ws.get(), T.col.get(), T.vr.get(), T.vc.get());
     BLQH_HIP(hipGetLastError());
     BLQH_HIP(hipStreamSynchronize(st));  // wl is freed on return
     T.len = std::move(len);

@@ -24,6 +24,16 @@
 //    the word in an int (BLQH_without_spin/basis.cpp:241), which is identical
 //    for 2*nphi <= 31 and corrupts keys at 2*nphi = 32; this oracle keeps the
 //    uncorrupted semantics there.
+//
+// What pins each part (tests/): the basis, translations and signs against the
+// reference's own basis.cpp compiled from its sources (oracle/_ref); the
+// README's dimensions, energies and amplitudes; the (J, K) Hamiltonian against
+// the projection of its K = -1 (plain Fock word) matrix onto truncated Bloch
+// vectors, entry by entry (phases, 1/sqrt(D_l D_r), translation signs); the
+// spinful pair blocks by the spin-polarised = spinless spectrum and by SU(2)
+// multiplets at zero Zeeman energy; Lanczos against LAPACK; the RNG against
+// MT19937's published 10000th draw.  g_plant plants mistakes for the tests
+// that show these checks fail on them.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -278,6 +288,12 @@ inline int period_of(const orc_sector& s, uint64_t r0) {
     return s.nphi;
 }
 
+// Planted mistakes, for the tests that show a check fails on them (0 = none):
+// 1 drops the spinful pair block (0,1); 2 reads the interlayer table for the
+// intra-layer block (2,3); 3 swaps the FT phase's indices (conjugate phase);
+// 4 divides by sqrt(D_l) instead of sqrt(D_l D_r).
+int g_plant = 0;
+
 // One Hamiltonian row: peer_set_hamil, BLQH_without_spin/lanczos_hamiltonian.cpp:97-347
 // and BLQH_with_spin/lanczos_hamiltonian.cpp:97-908.
 void build_row(const Basis& B, const Tables& T, const orc_params& p, long i, RowScratch& R,
@@ -297,7 +313,9 @@ void build_row(const Basis& B, const Tables& T, const orc_params& p, long i, Row
         for (int bi = 0; bi < nblk; ++bi) {
             const int a = blocks[bi][0], bb = blocks[bi][1];
             const int la = a / (S / 2), lb = bb / (S / 2);
-            const int alpha = la != lb ? 1 : 0;
+            int alpha = la != lb ? 1 : 0;
+            if (!spinless && g_plant == 1 && a == 0 && bb == 1) continue;
+            if (!spinless && g_plant == 2 && a == 2 && bb == 3) alpha = 1;
             bool twisted;
             if (spinless) twisted = (a != bb);
             else twisted = (a == 0 && bb == 3) || (a == 1 && bb == 2);
@@ -324,13 +342,14 @@ void build_row(const Basis& B, const Tables& T, const orc_params& p, long i, Row
                             if (j < 0) continue;
                             int sign = sign2(L, n, m, nt, mt) * signl * signr;
                             const double Cv = T.C[alpha * nphi * nphi + sx * nphi + std::abs(t)];
-                            cplx ft = T.FT[ql * nphi + qr];
+                            cplx ft = g_plant == 3 ? T.FT[qr * nphi + ql] : T.FT[ql * nphi + qr];
+                            const int Dd = g_plant == 4 ? Dl : Dl * Dr;
                             if (spinless) {
                                 if (twisted) ft = ft * cplx(std::cos(2 * p.theta_y * t / nphi), std::sin(2 * p.theta_y * t / nphi));
-                                add_to(R, j, ft * Cv / (std::sqrt(Dl * Dr) * sign));
+                                add_to(R, j, ft * Cv / (std::sqrt(Dd) * sign));
                             } else {
                                 if (twisted) ft = ft * cplx(std::cos(p.theta_y * t / nphi), std::sin(p.theta_y * t / nphi));
-                                add_to(R, j, Cv * sign * ft / std::sqrt(Dl * Dr));
+                                add_to(R, j, Cv * sign * ft / std::sqrt(Dd));
                             }
                         }
                     }
@@ -627,6 +646,7 @@ void orc_observables(void* h, const double* psi_in, double* out) {
     out[5 + 2 * nphi] = s.S == 4 ? expect({{0, nphi, 3 * nphi, 0.5}, {nphi, 2 * nphi, nphi, 0.5}}) : 0.0;
 }
 void orc_basis_free(void* h) { delete (Basis*)h; }
+void orc_plant(int which) { g_plant = which; }
 
 // Number of J-sector configurations before orbit reduction (the reference's
 // basis_set size after generate()).
@@ -751,6 +771,22 @@ void* orc_csr_wrap(int64_t n, const int64_t* rowptr, const int64_t* col, const d
 }
 
 void orc_spmv(void* h, const double* x, double* y) { spmv(*(Csr*)h, (const cplx*)x, (cplx*)y); }
+
+// The same row loop over caller arrays with 32-bit columns (no copy of a
+// large matrix); nthreads <= 0: OpenMP's default.
+void orc_spmv32(int64_t n, const int64_t* rowptr, const int32_t* col, const double* val, const double* x, double* y,
+                int nthreads) {
+    const cplx* v = (const cplx*)val;
+    const cplx* xx = (const cplx*)x;
+    cplx* yy = (cplx*)y;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+    for (long i = 0; i < n; ++i) {
+        cplx acc = 0;
+        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) acc += v[k] * xx[col[k]];
+        yy[i] = acc;
+    }
+}
 
 void orc_start_vector(int64_t n, unsigned seed, double* out) {
     std::vector<cplx> v;

@@ -76,6 +76,8 @@ def lib():
         L.orc_hamil_free.argtypes = [vp]
         L.orc_hamil_rows.restype = i64
         L.orc_hamil_rows.argtypes = [vp, C.POINTER(Params), vp, i64, C.c_int, vp, vp, vp, i64]
+        L.orc_plant.argtypes = [C.c_int]
+        L.orc_spmv32.argtypes = [i64, vp, vp, vp, vp, vp, C.c_int]
         L.orc_csr_wrap.restype = vp
         L.orc_csr_wrap.argtypes = [i64, vp, vp, vp]
         L.orc_spmv.argtypes = [vp, vp, vp]
@@ -287,3 +289,19 @@ def observables(basis, psi):
     lib().orc_observables(basis.h, _p(psi), _p(out))
     return {"pseudospin_sz": out[0], "pseudospin_sx": out[1], "sz": out[2], "upper_sz": out[3], "lower_sz": out[4],
             "occupation": out[5:5 + 2 * nphi].reshape(2, nphi), "spinflip": out[5 + 2 * nphi]}
+
+
+class planted:
+    """Context: the oracle builds Hamiltonians with a planted mistake (tests
+    that show a check fails on it): 1 no spinful (0,1) pair block, 2 the
+    interlayer table for the (2,3) block, 3 conjugated FT phase, 4 1/sqrt(D_l)
+    in place of 1/sqrt(D_l D_r)."""
+
+    def __init__(self, which):
+        self.which = which
+
+    def __enter__(self):
+        lib().orc_plant(self.which)
+
+    def __exit__(self, *a):
+        lib().orc_plant(0)

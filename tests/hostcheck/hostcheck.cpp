@@ -187,4 +187,32 @@ int64_t hc_hamil(const blqh_sector* s, const blqh_params* p, int64_t* rowptr, in
     return nnz;
 }
 
+// The SpMV formats' internal order (panel.hip internal_order) of a basis given
+// by its ids: out[i] = internal position of row i, or -1 everywhere for the
+// identity.
+void hc_internal_order(const blqh_sector* s, const uint64_t* ids, int64_t n, int32_t* out) {
+    blqh_basis B;
+    B.sec = *s;
+    B.geo = Geo{s->nspecies, s->nphi, s->nel};
+    B.n = n;
+    B.h_id.assign(ids, ids + n);
+    const std::vector<int32_t>& o = internal_order(&B);
+    for (int64_t i = 0; i < n; ++i) out[i] = o.empty() ? -1 : o[i];
+}
+
+// 1 when the replayed hash-map iteration order (basis.hip) equals the order
+// of a real std::unordered_map filled the same way; *n = words.
+int hc_hash_order_check(const blqh_sector* s, int64_t* n) {
+    const std::vector<uint64_t> a = hash_iteration_order(*s);
+    const std::vector<uint64_t> b = hash_iteration_order_map(*s);
+    *n = (int64_t)a.size();
+    return a == b ? 1 : 0;
+}
+
+// The build's reciprocal modulo (rowterms.hpp mod_by_rcp) for s = 0 .. n-1
+// with a given reciprocal r of d.
+void hc_mod_by_rcp(int n, int d, float r, int32_t* out) {
+    for (int s = 0; s < n; ++s) out[s] = mod_by_rcp(s, d, r);
+}
+
 }  // extern "C"

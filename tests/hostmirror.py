@@ -22,6 +22,12 @@ def lib():
                                 C.c_void_p, C.c_int64]
         _L.hc_sector_words.restype = C.c_int64
         _L.hc_sector_words.argtypes = [C.POINTER(blqh_amd.Sector), C.c_int, C.c_void_p, C.c_int64]
+        _L.hc_internal_order.restype = None
+        _L.hc_internal_order.argtypes = [C.POINTER(blqh_amd.Sector), C.c_void_p, C.c_int64, C.c_void_p]
+        _L.hc_hash_order_check.restype = C.c_int
+        _L.hc_hash_order_check.argtypes = [C.POINTER(blqh_amd.Sector), C.POINTER(C.c_int64)]
+        _L.hc_mod_by_rcp.restype = None
+        _L.hc_mod_by_rcp.argtypes = [C.c_int, C.c_int, C.c_float, C.c_void_p]
         _L.hc_xcd_lists.restype = None
         _L.hc_xcd_lists.argtypes = [C.c_int] + [C.c_void_p] * 6
     return _L
@@ -67,3 +73,25 @@ def xcd_lists(item0, nnz, npan, real):
     seg = np.empty(27, np.int32)
     lib().hc_xcd_lists(len(nnz), _p(item0), _p(nnz), _p(npan), _p(real), _p(items), _p(seg))
     return items, seg
+
+
+def mod_by_rcp(n, d, r):
+    """rowterms.hpp mod_by_rcp(s, d, r) for s = 0 .. n-1."""
+    out = np.empty(n, np.int32)
+    lib().hc_mod_by_rcp(n, d, float(np.float32(r)), _p(out))
+    return out
+
+
+def internal_order(sec, ids):
+    """panel.hip internal_order: internal position of every row (None: identity)."""
+    ids = np.ascontiguousarray(ids, np.uint64)
+    out = np.empty(len(ids), np.int32)
+    lib().hc_internal_order(C.byref(sec), _p(ids), len(ids), _p(out))
+    return None if len(ids) and out[0] < 0 else out
+
+
+def hash_order_check(sec):
+    """(replayed hash-map order == real std::unordered_map order, words)."""
+    n = C.c_int64()
+    ok = lib().hc_hash_order_check(C.byref(sec), C.byref(n))
+    return bool(ok), n.value

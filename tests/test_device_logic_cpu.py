@@ -117,8 +117,10 @@ def test_small_modulo_by_fp32_reciprocal():
     """The build's hop-target shift on the device: s mod d from the FP32
     reciprocal of d (the hardware's approximate reciprocal is within 1 ulp;
     checked here for every reciprocal within 2 ulp of 1/d), truncated
-    quotient, one correction step either way (mod_small in rowterms.hpp).
-    Exact for 0 <= s < 2^16, 1 <= d <= 64 (the build uses s < 64)."""
+    quotient, one correction step either way.  Runs the product's own helper
+    (mod_by_rcp in rowterms.hpp, which the device's mod_small calls with the
+    hardware reciprocal) through the CPU mirror.  Exact for 0 <= s < 2^16,
+    1 <= d <= 64 (the build uses s < 64)."""
     s = np.arange(1 << 16, dtype=np.int64)
     for d in range(1, 65):
         r0 = np.float32(1.0) / np.float32(d)
@@ -126,7 +128,26 @@ def test_small_modulo_by_fp32_reciprocal():
         dn1 = np.nextafter(r0, np.float32(0))
         cands = [r0, up1, np.nextafter(up1, np.float32(2)), dn1, np.nextafter(dn1, np.float32(0))]
         for rc in cands:
-            q = (s.astype(np.float32) * np.float32(rc)).astype(np.int64)
-            r = s - q * d
-            r = np.where(r < 0, r + d, np.where(r >= d, r - d, r))
-            np.testing.assert_array_equal(r, s % d)
+            np.testing.assert_array_equal(hostmirror.mod_by_rcp(1 << 16, d, rc), s % d)
+
+
+def test_mod_by_rcp_fails_with_a_bad_reciprocal():
+    """The check above can fail: a reciprocal 1 % off gives wrong remainders."""
+    d = 7
+    bad = hostmirror.mod_by_rcp(1 << 16, d, np.float32(1.01 / d))
+    assert np.any(bad != np.arange(1 << 16) % d)
+
+
+@pytest.mark.parametrize("case", [
+    (2, 8, 8, 4, -1, 4, 4), (2, 8, 8, -1, -1, 3, 2), (2, 10, 10, 5, -1, 1, 7), (2, 9, 6, -1, -1, 2, 1),
+    (2, 12, 12, -1, -1, 1, 2), (2, 14, 14, -1, -1, 0, 0), (2, 14, 14, 7, -1, 7, 7), (2, 16, 16, 8, -1, 0, 0),
+    (4, 6, 6, 3, 3, 0, 1), (4, 6, 6, -1, -1, 1, 2), (4, 8, 8, 4, 4, 3, 5), (4, 4, 4, 2, 2, 1, 1)],
+    ids=lambda c: "_".join(map(str, c)))
+def test_hash_order_replay_equals_the_real_map(case):
+    """The basis step's representative rule needs the iteration order of the
+    reference's std::unordered_map; the library replays the container's node
+    list with index arrays (basis.hip hash_iteration_order).  It must equal a
+    real map filled the same way, word for word, on every sector the tests
+    and the benchmarks use (up to 10.4M words here)."""
+    ok, n = hostmirror.hash_order_check(blqh_amd.Sector(*case))
+    assert n > 0 and ok, (case, n)

@@ -183,3 +183,29 @@ def test_invalid_sector_is_an_error():
     with pytest.raises(blqh_amd.BlqhError) as e:
         blqh_amd.Basis(8, 8, 9, 0, 0)
     assert e.value.code == -1
+
+
+COMPLEX = [c for c in CASES if c[0] in ("alldens_complex_tunnel", "split_complex", "spinful_alldens")]
+
+
+@pytest.mark.parametrize("name,case,kw", COMPLEX, ids=[c[0] for c in COMPLEX])
+def test_complex_sector_printed_energy_tight(name, case, kw):
+    """The reference prints Re<psi|H psi> with psi = sum c_l conj(v_l), which
+    for a complex sector is the Rayleigh quotient of conj(u0), not E0 (its
+    error is first order in the state's).  With the Ritz residual driven to
+    1e-12 the engine's printed value equals LAPACK's conj(u0)^H H conj(u0)
+    to 1e-9."""
+    print(f"case {name}", flush=True)
+    B, H = _build(case, kw)
+    rp, col, val = H.export()
+    Hd = np.zeros((H.n, H.n), np.complex128)
+    Hd[np.repeat(np.arange(H.n), np.diff(rp)), col] = val
+    assert np.abs(val.imag).max() > 1e-3
+    w, V = np.linalg.eigh(Hd)
+    assert w[1] - w[0] > 1e-6 * abs(w[0])
+    u = np.conj(V[:, 0])
+    printed = np.real(np.vdot(u, Hd @ u))
+    r = H.lanczos(lam=800, seed=4, tol=1e-13, res_tol=1e-12)
+    assert r["converged"] == 1, r["residual"]
+    assert r["E_rq"] == pytest.approx(printed, rel=1e-9)
+    print(f"printed {printed:.12f} vs E0 {w[0]:.12f}")
