@@ -481,8 +481,8 @@ def run_warm(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="nphi16_all", choices=sorted(WORKLOADS) + sorted(SWEEPS) + sorted(WARM))
     ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--check-every", type=int, default=5)

@@ -499,7 +499,13 @@ __global__ void __launch_bounds__(WPB * 64) k_jag(PanelView P, const double2* __
     __shared__ double2 yl[WPB * MAXS * 64];  // row sums of this wavefront's slices
     double2* my = yl + wv * MAXS * 64;
     double dacc = 0.0;
-    for (int64_t r0 = 0; r0 < nslices; r0 += W * MAXS) {
+    // Wavefront wid owns slices wid, wid + W, ... (T of them at most), taken
+    // in R rounds of m <= MAXS each: the rounds carry equal shares, so no
+    // round sweeps the panels with a single slice per wavefront.
+    const int64_t T = (nslices + W - 1) / W;
+    const int64_t R = (T + MAXS - 1) / MAXS;
+    const int m = (int)((T + R - 1) / R);
+    for (int64_t r0 = 0; r0 < R * m; r0 += m) {
         for (int k = 0; k < MAXS; ++k) my[k * 64 + lane] = make_double2(0.0, 0.0);
         for (int p = 0; p < P.npan; ++p) {
             __syncthreads();  // keep the workgroup on one panel
@@ -507,8 +513,8 @@ __global__ void __launch_bounds__(WPB * 64) k_jag(PanelView P, const double2* __
             const int64_t base = P.pbase[p];
             const double2* xp = x + (int64_t)p * kPanelWidth;
 #pragma unroll 1
-            for (int k = 0; k < MAXS; ++k) {
-                const int64_t slice = r0 + wid + (int64_t)k * W;
+            for (int k = 0; k < m; ++k) {
+                const int64_t slice = wid + (r0 + k) * W;
                 if (slice >= nslices) continue;  // wave-uniform
                 const int64_t row0 = slice * 64, myrow = row0 + lane;
                 const int64_t rend = row0 + 64 < n ? row0 + 64 : n;
@@ -524,8 +530,8 @@ __global__ void __launch_bounds__(WPB * 64) k_jag(PanelView P, const double2* __
                 my[k * 64 + lane] = tacc;
             }
         }
-        for (int k = 0; k < MAXS; ++k) {
-            const int64_t slice = r0 + wid + (int64_t)k * W;
+        for (int k = 0; k < m; ++k) {
+            const int64_t slice = wid + (r0 + k) * W;
             const int64_t myrow = slice * 64 + lane;
             if (slice >= nslices || myrow >= n) continue;
             const double2 yv = my[k * 64 + lane];

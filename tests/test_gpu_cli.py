@@ -90,7 +90,7 @@ def test_spinful_run():
     assert len(body) % 3 == 0 and body
     head = re.compile(r"^ {9}\((\d+)\): ([01]{32})  (\S+)$")
     up = re.compile(r"^upper layer:   \|((?: +(?:\d+|_)){4})\)\|((?: +(?:\d+|_)){4})\)\|$")
-    down = re.compile(r"^  down layer:   \|((?: +(?:\d+|_)){4})\)\|((?: +(?:\d+|_)){4})\)   $")
+    down = re.compile(r"^ down layer:   \|((?: +(?:\d+|_)){4})\)\|((?: +(?:\d+|_)){4})\)   $")
     amps = []
     for i in range(0, len(body), 3):
         h, u, w = head.match(body[i]), up.match(body[i + 1]), down.match(body[i + 2])

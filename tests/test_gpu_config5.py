@@ -48,16 +48,23 @@ def test_config5_sector_parity(oracle_mod, J, K):
     assert complex_sector == (K not in (0, NPHI // 2))
 
     A = sp.csr_matrix((OH.val, OH.col, OH.rowptr), shape=(OH.n, OH.n))
-    w, V = sla.eigsh(A, k=2, which="SA", tol=1e-14, v0=np.ones(OH.n, np.complex128))
+    # several levels with a wide Krylov space: ARPACK with k = 2 can miss a
+    # degenerate partner (the complex sector (1, 3) has a doubly degenerate
+    # ground level)
+    w, V = sla.eigsh(A, k=4, which="SA", tol=1e-13, ncv=40, v0=np.ones(OH.n, np.complex128))
     order = np.argsort(w)
     w, V = w[order], V[:, order]
-    assert w[1] - w[0] > 1e-6 * abs(w[0])  # non-degenerate: the state is comparable
     r = H.lanczos(lam=250, seed=5, tol=1e-10)
     assert r["converged"] == 1
     assert r["E0"] == pytest.approx(w[0], rel=1e-10)
-    # psi = sum_l c_l conj(v_l): the conjugate of the ground state
-    ov = abs(np.vdot(np.conj(V[:, 0]), r["psi"]))
-    assert ov >= 1 - 1e-10, ov
+    # psi = sum_l c_l conj(v_l): conj(psi) is a ground state of H
+    g = np.conj(r["psi"])
+    assert np.linalg.norm(A @ g - w[0] * g) < 1e-6 * abs(w[0])
+    if w[1] - w[0] > 1e-6 * abs(w[0]):  # non-degenerate: the state itself is fixed
+        ov = abs(np.vdot(V[:, 0], g))
+        assert ov >= 1 - 1e-10, ov
+    else:
+        print(f"degenerate ground level ({w[0]:.10f} x2): state compared by residual", flush=True)
     e0, _, _ = H.dense_ground_state()
     assert e0 == pytest.approx(w[0], rel=1e-10)
 
@@ -85,7 +92,9 @@ def test_config5_all_sectors_batched(all_sectors):
         assert b["converged"] == 1 and s["converged"] == 1, (J, K)
         assert b["E0"] == pytest.approx(s["E0"], rel=1e-10, abs=0), (J, K)
         assert b["E_rq"] == pytest.approx(np.vdot(b["psi"], H.spmv(b["psi"])).real, rel=1e-12)
-        # a degenerate ground state leaves the state free inside its multiplet
-        r = H.spmv(b["psi"]) - b["E0"] * b["psi"]
+        # a degenerate ground state leaves the state free inside its multiplet;
+        # psi = sum c_l conj(v_l), so conj(psi) is H's eigenvector
+        g = np.conj(b["psi"])
+        r = H.spmv(g) - b["E0"] * g
         assert np.linalg.norm(r) < 1e-5 * abs(b["E0"]), (J, K)
     bt.close()

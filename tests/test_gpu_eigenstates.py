@@ -117,7 +117,7 @@ def test_reconstruct_many_states_and_start_vector():
 
 def test_reconstruct_rejects_bad_arguments():
     B, H = _ham(README, 3.0, 0.0)
-    with pytest.raises(blqh_amd.BlqhError):
+    with pytest.raises(ValueError):  # 17 states: refused by the binding before the call
         H.reconstruct(np.zeros((17, 4)) + 1.0)
     with pytest.raises(blqh_amd.BlqhError):
         H.reconstruct(np.zeros((1, 4)))  # zero state

@@ -51,7 +51,9 @@ def test_internal_order_changes_nothing(case, kw):
     assert a["converged"] == 1 and b["converged"] == 1
     assert a["E0"] == pytest.approx(b["E0"], rel=1e-10, abs=0)
     assert a["E_rq"] == pytest.approx(np.vdot(a["psi"], Ho.spmv(a["psi"])).real, rel=1e-12)
-    r = Ho.spmv(a["psi"]) - a["E0"] * a["psi"]
+    # psi = sum c_l conj(v_l) (the reference's convention): conj(psi) is H's eigenvector
+    g = np.conj(a["psi"])
+    r = Ho.spmv(g) - a["E0"] * g
     assert np.linalg.norm(r) < 1e-5 * abs(a["E0"])
     # replay (three passes) in the internal order as well
     c = Ho.lanczos(lam=300, seed=5, store_basis=0)
