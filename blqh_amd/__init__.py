@@ -88,6 +88,8 @@ _SIGS = {
     "blqh_spmv_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "blqh_lanczos": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.POINTER(LanczosResult), C.c_void_p,
                                C.c_void_p, C.c_void_p]),
+    "blqh_lanczos_noalloc": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.POINTER(LanczosResult), C.c_void_p,
+                                       C.c_void_p, C.c_void_p]),
     "blqh_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "blqh_batch_lanczos": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.c_void_p, C.c_void_p,
                                      C.POINTER(LanczosResult), C.c_void_p]),
@@ -316,6 +318,14 @@ class Hamiltonian:
         weight of the seeded random unit vector added to a given start."""
         return self._lanczos(lib().blqh_lanczos, lam, seed, tol, res_tol, check_every, store_basis, start,
                              want_psi, skip_states, start_mix)
+
+    def lanczos_noalloc(self, lam=200, seed=0, tol=1e-10, res_tol=1e-7, check_every=1, start=None, want_psi=True,
+                        skip_states=False):
+        """lanczos() through blqh_lanczos_noalloc: no device allocation (the
+        solver's static per-GPU scratch; at most 131072 rows, lam <= 1024),
+        three passes as lanczos(store_basis=0)."""
+        return self._lanczos(lib().blqh_lanczos_noalloc, lam, seed, tol, res_tol, check_every, 0, start, want_psi,
+                             skip_states, 0.0)
 
     def _lanczos(self, fn, lam, seed, tol, res_tol, check_every, store_basis, start, want_psi, skip_states,
                  start_mix):

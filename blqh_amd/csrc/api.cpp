@@ -16,6 +16,8 @@ void blqh_hamil_combine_impl(const blqh_hamil* base, const blqh_hamil* tun, doub
 void blqh_spmv_device_impl(blqh_hamil* H, const void* x, void* y);
 void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_result& r, double* psi, double* alpha,
                        double* beta);
+void blqh_lanczos_noalloc_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_result& r, double* psi,
+                               double* alpha, double* beta);
 blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count, int lanes);
 void blqh_batch_destroy_impl(blqh_batch* b);
 void blqh_batch_rebind_impl(blqh_batch* b, blqh_hamil* const* hs, int count);
@@ -48,6 +50,7 @@ void blqh_observables_impl(const blqh_basis* B, const double* psi_host, blqh_obs
 void blqh_dense_ground_state_impl(const blqh_hamil* H, double* e0, double* e_printed, double* vec);
 void blqh_device_copy_impl(const void* src, void* dst, int64_t bytes);
 namespace blqh {
+void noalloc_setup(int device);
 void start_vector(int64_t n, uint32_t seed, std::vector<double2>& v);
 double tridiag_lowest(int m, const double* a, const double* b, double* vec);
 }  // namespace blqh
@@ -94,6 +97,7 @@ void init_device(int dev) {
     g_rt.device = dev;
     g_rt.num_cus = prop.multiProcessorCount;
     BLQH_HIP(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
+    blqh::noalloc_setup(dev);  // device address of the allocation-free solver's static scratch
     g_rt_ok = true;
 }
 }  // namespace
@@ -368,6 +372,15 @@ int blqh_lanczos(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_r
     return guarded([&] {
         std::memset(r, 0, sizeof(*r));
         blqh_lanczos_impl(const_cast<blqh_hamil*>(h), *o, *r, psi, alpha, beta);
+    });
+}
+
+int blqh_lanczos_noalloc(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_result* r, double* psi,
+                         double* alpha, double* beta) {
+    if (!h || !o || !r) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] {
+        std::memset(r, 0, sizeof(*r));
+        blqh_lanczos_noalloc_impl(const_cast<blqh_hamil*>(h), *o, *r, psi, alpha, beta);
     });
 }
 

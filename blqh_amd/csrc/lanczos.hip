@@ -1587,6 +1587,189 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
                  alpha_out, beta_out, t0);
 }
 
+// ---- the allocation-free solve (blqh_lanczos_noalloc) ----------------------------
+//
+// Every device array of the solve lives in `scratch`, one static array per
+// GPU of the library, sized for kNoAllocMaxRows rows and kNoAllocMaxLambda
+// steps.  Host code never names the array itself: blqh::noalloc_setup (run by
+// the library's device initialisation, with the device current) asks the
+// runtime for this device's address and every use goes through that pointer.
+// Calls are serialised by a mutex (they share the array); each call
+// initialises everything it reads (reduction ticket, scalars, psi), so
+// nothing carries over from the previous call's contents.
+namespace {
+constexpr int64_t kNoAllocMaxRows = int64_t(1) << 17;
+constexpr int kNoAllocMaxLambda = 1024;
+constexpr int64_t kNoAllocPartials = 16384;
+constexpr size_t kNoAllocVecBytes = (size_t)kNoAllocMaxRows * sizeof(double2);
+// ring of three Krylov vectors, u, psi, v0
+constexpr size_t kNoAllocScalOff = 6 * kNoAllocVecBytes;
+constexpr size_t kNoAllocScalDoubles = (3 * (size_t)kNoAllocMaxLambda + 8) + ((size_t)kNoAllocMaxLambda + 1) + 4 +
+                                       (size_t)kNoAllocPartials;
+constexpr size_t kNoAllocBytes = kNoAllocScalOff + kNoAllocScalDoubles * sizeof(double) + 256;
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+std::mutex g_noalloc_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+// host work arrays of the allocation-free solve (used under g_noalloc_mu)
+double g_noalloc_host[7 * (kNoAllocMaxLambda + 1) + 8 * kNoAllocMaxLambda];
+int g_noalloc_piv[kNoAllocMaxLambda];
+
+// The std::mt19937(seed) start vector on the device, bit for bit the entries
+// and running norm of start_entries: one workgroup twists the 624-word state
+// in LDS (the standard's recurrence in its four dependency phases: words
+// 0..226 from the old state, 227..453 and 454..622 from words the previous
+// phase wrote, 623 from words 0 and 396), tempers, and turns draws 2i / 2i+1
+// into the imaginary / real part of entry i; one lane adds |z_i|^2 in entry
+// order.  *nrm2 = the sum (entries left unnormalised).
+__global__ void __launch_bounds__(1024) k_mt_start(int64_t n, uint32_t seed, double2* __restrict__ v,
+                                                   double* __restrict__ nrm2) {
+#pragma clang fp contract(off)
+    constexpr uint32_t kUp = 0x80000000u, kLo = 0x7fffffffu, kA = 0x9908b0dfu;
+    __shared__ uint32_t mt[624];
+    __shared__ double2 ent[312];
+    const int t = threadIdx.x;
+    if (t == 0) {
+        uint32_t x = seed;
+        mt[0] = x;
+        for (uint32_t i = 1; i < 624; ++i) {
+            x = 1812433253u * (x ^ (x >> 30)) + i;
+            mt[i] = x;
+        }
+    }
+    __syncthreads();
+    double s = 0.0;
+    for (int64_t e0 = 0; e0 < n; e0 += 312) {
+        const int lo[4] = {0, 227, 454, 623}, hi[4] = {227, 454, 623, 624};
+        for (int ph = 0; ph < 4; ++ph) {
+            const int i = lo[ph] + t;
+            uint32_t nv = 0;
+            if (i < hi[ph]) {
+                const uint32_t y = (mt[i] & kUp) | (mt[i + 1 < 624 ? i + 1 : 0] & kLo);
+                nv = mt[i + 397 < 624 ? i + 397 : i - 227] ^ (y >> 1) ^ ((y & 1u) ? kA : 0u);
+            }
+            __syncthreads();
+            if (i < hi[ph]) mt[i] = nv;
+            __syncthreads();
+        }
+        if (t < 312) {
+            uint32_t z[2];
+            for (int k = 0; k < 2; ++k) {
+                uint32_t w = mt[2 * t + k];
+                w ^= w >> 11;
+                w ^= (w << 7) & 0x9d2c5680u;
+                w ^= (w << 15) & 0xefc60000u;
+                w ^= w >> 18;
+                z[k] = w;
+            }
+            const double im = z[0] * 1.0 / 4294967295u - 0.5;
+            const double re = z[1] * 1.0 / 4294967295u - 0.5;
+            ent[t] = make_double2(re, im);
+            if (e0 + t < n) v[e0 + t] = make_double2(re, im);
+        }
+        __syncthreads();
+        if (t == 0) {
+            const int cnt = (int)(n - e0 < 312 ? n - e0 : 312);
+            for (int k = 0; k < cnt; ++k) {
+                const double re = ent[k].x, im = ent[k].y;
+                s += re * re + im * im;
+            }
+        }
+        // ent is rewritten only after the next block's twist barriers
+    }
+    if (t == 0) *nrm2 = s;
+}
+
+// The SpMV plan the Hamiltonian's current layout supports without any
+// conversion (the tuned one when its layout is in place, else the CSR
+// kernel, whose arrays always exist).
+Plan noalloc_plan(const blqh_hamil* H) {
+    if (H->spmv_kind >= 0) {
+        const Plan p = plan_of(H);
+        const Panels& P = H->pan;
+        bool ok = false;
+        switch (p.kind) {
+            case 0: ok = true; break;
+            case 1:
+            case 2: ok = P.built && !P.jagged; break;
+            case 3: ok = P.built && P.jagged && P.greedy && P.rows == 1; break;
+            case 4: ok = H->til.built; break;
+            case 5: ok = P.built && P.jagged && P.rows == p.lpr; break;
+            default: break;
+        }
+        if (ok) return p;
+    }
+    const double mean = H->n ? (double)H->nnz / (double)H->n : 0.0;
+    int lpr = 4;
+    while (lpr < 64 && 4.0 * lpr < mean) lpr *= 2;
+    return Plan{0, lpr, kThreads, 0, 0};
+}
+}  // namespace
+
+namespace blqh {
+// This device's address of `scratch` (the library's device initialisation
+// calls it once per device, with that device current).
+void noalloc_setup(int device) {
+    BLQH_REQUIRE(device >= 0 && device < kMaxDevices, BLQH_EINVAL, "device index out of range");
+    void* p = nullptr;
+    BLQH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    BLQH_REQUIRE(p != nullptr, BLQH_EDEVICE, "no device address for the solver's static scratch array");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    g_scratch_ptr[device] = p;
+}
+}  // namespace blqh
+
+void blqh_lanczos_noalloc_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_result& res, double* psi_out,
+                               double* alpha_out, double* beta_out) {
+    using namespace blqh;
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = stream();
+    const int64_t n = H->n;
+    const int cap = o.lambda;
+    BLQH_REQUIRE(cap >= 1, BLQH_EINVAL, "lambda out of range");
+    BLQH_REQUIRE(cap <= kNoAllocMaxLambda, BLQH_ERANGE, "blqh_lanczos_noalloc: lambda above 1024");
+    BLQH_REQUIRE(n >= 1 && n <= kNoAllocMaxRows, BLQH_ERANGE, "blqh_lanczos_noalloc: more than 131072 rows");
+    BLQH_REQUIRE(o.start_mix == 0.0, BLQH_EINVAL, "blqh_lanczos_noalloc: start_mix is not supported");
+    const Plan plan = noalloc_plan(H);
+    const int dev = runtime().device;  // initialised above (stream()), outside the lock: set-up takes it too
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    unsigned char* base = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+    BLQH_REQUIRE(base != nullptr, BLQH_EINTERNAL, "solver scratch not set up for this device");
+    double2* vec = reinterpret_cast<double2*>(base);
+    double* sd = reinterpret_cast<double*>(base + kNoAllocScalOff);
+    // scalars, coefficients, reductions and partials packed for this lambda
+    double* sc = sd;
+    double* coef = sc + 3 * (size_t)cap + 8;
+    double* red = coef + cap + 1;
+    double* partials = red + 4;
+    unsigned* ticket = reinterpret_cast<unsigned*>(sd + kNoAllocScalDoubles);
+    const DevWork D{vec, vec + 3 * n, vec + 4 * n, vec + 5 * n, sc, coef, red, partials, kNoAllocPartials, ticket};
+    double* hp = g_noalloc_host;
+    const size_t L = (size_t)cap + 1;
+    const HostWork hw{hp, hp + L, hp + 2 * L, hp + 3 * L, hp + 4 * L, hp + 5 * L, hp + 6 * L, hp + 7 * L,
+                      g_noalloc_piv};
+    auto start = [&](double2* dst) {
+        double s = 0;
+        if (o.start) {
+            BLQH_HIP(hipMemcpyAsync(dst, o.start, n * sizeof(double2), hipMemcpyHostToDevice, st));
+            const double2* z = reinterpret_cast<const double2*>(o.start);
+            for (int64_t i = 0; i < n; ++i) s += z[i].x * z[i].x + z[i].y * z[i].y;
+            BLQH_REQUIRE(s > 0 && std::isfinite(s), BLQH_EINVAL, "start vector is zero or not finite");
+        } else {
+            hipLaunchKernelGGL(k_mt_start, dim3(1), dim3(1024), 0, st, n, o.seed, dst, red + 2);
+            BLQH_HIP(hipGetLastError());
+            copy_sync(&s, red + 2, sizeof(double), hipMemcpyDeviceToHost);
+        }
+        hipLaunchKernelGGL(k_div, dim3(vec_grid(n)), dim3(kThreads), 0, st, n, dst, std::sqrt(s));
+        BLQH_HIP(hipGetLastError());
+    };
+    lanczos_core(H, plan, o, false, D, hw, start, false, res, psi_out, alpha_out, beta_out, t0);
+}
+
 void blqh_lanczos_reconstruct_impl(blqh_hamil* H, int m, uint32_t seed, const double* start, const double* alpha,
                                    const double* beta, int ns, const double* coefs, double* psi_out, double* erq_out,
                                    double* dev_out) {

@@ -173,6 +173,20 @@ typedef struct blqh_lanczos_result {
 int blqh_lanczos(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_result* r,
                  double* psi, double* alpha, double* beta);
 
+/* blqh_lanczos without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops that forbid hipMalloc).  Same arguments, result
+ * and output as blqh_lanczos with store_basis = 0 (three passes, pass 3 a
+ * bit-identical replay); every device array of the solve lives in one static
+ * array per GPU of about 12.3 MiB, held until the process ends.  Limits
+ * (BLQH_ERANGE beyond them): at most 131072 rows and lambda <= 1024;
+ * start_mix must be 0 (BLQH_EINVAL).  The SpMV is the tuned variant when the
+ * Hamiltonian's layout is already in place for it (after a blqh_lanczos or
+ * blqh_spmv on it), else the CSR kernel.  Calls in one process are serialised
+ * (they share the array, whose contents each call re-initialises). */
+int blqh_lanczos_noalloc(const blqh_hamil* h, const blqh_lanczos_opts* o, blqh_lanczos_result* r, double* psi,
+                         double* alpha, double* beta);
+
 /* Pass 3 on its own: eigenstates_reconstruction of both reference trees
  * (BLQH_without_spin/lanczos_hamiltonian.cpp:610-644; the spinful one,
  * BLQH_with_spin/lanczos_hamiltonian.cpp:1143-1191, builds psir_0 and psir_1
