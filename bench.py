@@ -180,6 +180,10 @@ def run_sweep(args):
             hs = list(cache["base"])
         return hs, time.time() - t
 
+    # convergence checks: plain solves every --check-every steps, the batch
+    # every --batch-check-every (a batch runs one speculative replay past
+    # each check, so a shorter interval wastes fewer steps per problem)
+    bchk = args.batch_check_every or args.check_every
     plain = None
     batch = None
     solved = solves = 0
@@ -199,7 +203,7 @@ def run_sweep(args):
             batch = blqh_amd.Batch(hs)  # autotunes the batched SpMV once for the sweep
         else:
             batch.rebind(hs)  # same sectors, next parameter point: pools and replay kept
-        res = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=args.check_every, want_psi=False)
+        res = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False)
         dt = time.perf_counter() - t
         t = time.time()
         del prev  # the previous point's Hamiltonians (device frees)
@@ -211,7 +215,7 @@ def run_sweep(args):
         if args.continuation:
             t = time.perf_counter()
             starts = [np.conj(p) for p in cont_psi] if cont_psi is not None else None
-            wres = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=args.check_every,
+            wres = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk,
                                  starts=starts, want_psi=True, start_mix=args.start_mix if starts else 0.0)
             dtw = time.perf_counter() - t
             cont_psi = [r["psi"] for r in wres]
@@ -250,7 +254,7 @@ def run_sweep(args):
             # roofline pass: one more solve with HIP events around every batched K1 launch
             blqh_amd.profile_reset()
             blqh_amd.profile_enable(True)
-            batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=args.check_every, want_psi=False)
+            batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False)
             blqh_amd.profile_enable(False)
             launches, kms, kbytes = blqh_amd.profile_read()
             nnz_total = sum(H.nnz for H in hs)
@@ -281,6 +285,7 @@ def run_sweep(args):
             "lanczos_cap": args.lam,
             "tolerance": 1e-10,
             "check_every": args.check_every,
+            "batch_check_every": bchk,
             "iterations_mean": float(np.mean(iters)) if iters else None,
             "batched_rate": value,
             "plain_rate": plain["rate"] if plain else None,
@@ -487,6 +492,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--check-every", type=int, default=5)
     ap.add_argument("--lam", type=int, default=200)
+    ap.add_argument("--batch-check-every", type=int, default=2,
+                    help="steps between convergence checks of batched solves (0: --check-every)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sweep-points", type=int, default=2,
                     help="default workload: also time the sweep workloads at this many parameter points (0: skip)")

@@ -414,14 +414,22 @@ struct blqh_batch {
 
 namespace {
 
+// Planner view of the members (all, or those with live[p] set).
+std::vector<XcdMember> xcd_members(const blqh_batch* b, const std::vector<char>* live = nullptr) {
+    const int P = (int)b->hs.size();
+    std::vector<XcdMember> mem;
+    mem.reserve(P);
+    for (int p = 0; p < P; ++p)
+        if (!live || (*live)[p])
+            mem.push_back({b->item0[p], b->item0[p + 1], b->hs[p]->nnz, b->hs[p]->pan.npan,
+                           b->hs[p]->pan.real ? 1 : 0});
+    return mem;
+}
+
 // Work lists of k_b1j (xcd_lists), uploaded when they change.
 void build_xcd_lists(blqh_batch* b) {
-    const int P = (int)b->hs.size();
-    std::vector<XcdMember> mem(P);
-    for (int p = 0; p < P; ++p)
-        mem[p] = {b->item0[p], b->item0[p + 1], b->hs[p]->nnz, b->hs[p]->pan.npan, b->hs[p]->pan.real ? 1 : 0};
     std::vector<int32_t> items, seg;
-    xcd_lists(mem, items, seg);
+    xcd_lists(xcd_members(b), items, seg);
     if (items == b->h_xitems && seg == b->h_xseg) return;
     b->h_xitems.swap(items);
     b->h_xseg.swap(seg);
@@ -663,7 +671,9 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
         pl.psi.alloc(b->rows);
         if (b->pin) (void)hipHostFree(b->pin);
         b->pin = nullptr;
-        const size_t pin_bytes = ((size_t)P * cap + (size_t)P * ld) * 8 + 2 * (size_t)P * 4;
+        // alpha, beta, two slots of active flags, two slots of live work lists
+        const size_t pin_bytes = ((size_t)P * cap + (size_t)P * ld) * 8 + 2 * (size_t)P * 4 +
+                                 2 * (2 * (size_t)b->nitems + 27) * 4;
         BLQH_HIP(hipHostMalloc((void**)&b->pin, pin_bytes, hipHostMallocDefault));
         b->pool_cap = cap;
     }
@@ -786,6 +796,15 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     double* ha = (double*)pin;
     double* hb = ha + (size_t)P * cap;
     int32_t* actpin = (int32_t*)(hb + (size_t)P * ld);
+    // Work lists of the live members only, rebuilt whenever problems
+    // converge: the lists deal whole members to XCDs, so without a rebuild
+    // the last live members would sit on a few XCDs while the others idle.
+    // Every item's sums keep their owner and order, so results do not depend
+    // on the lists.  Two pinned slots, as for the active flags.
+    int32_t* listpin = actpin + 2 * (size_t)P;
+    const size_t list_slot = 2 * (size_t)b->nitems + 27;
+    int lslot = 0;
+    bool lists_live = false;
     hipEvent_t evc;
     BLQH_HIP(hipEventCreateWithFlags(&evc, hipEventDisableTiming));
     struct EvGuard {
@@ -875,10 +894,25 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
             for (int p = 0; p < P; ++p) dst[p] = live[p];
             BLQH_HIP(hipMemcpyAsync(pl.active.get(), dst, P * 4, hipMemcpyHostToDevice, st));
             slot ^= 1;
+            if (b->lpr == 64) {  // k_b1j walks the lists
+                std::vector<int32_t> items, seg;
+                xcd_lists(xcd_members(b, &live), items, seg);
+                int32_t* lp = listpin + (size_t)lslot * list_slot;
+                std::copy(items.begin(), items.end(), lp);
+                std::copy(seg.begin(), seg.end(), lp + 2 * b->nitems);
+                BLQH_HIP(hipMemcpyAsync(b->xitems.get(), lp, items.size() * 4, hipMemcpyHostToDevice, st));
+                BLQH_HIP(hipMemcpyAsync(b->xseg.get(), lp + 2 * b->nitems, 27 * 4, hipMemcpyHostToDevice, st));
+                lslot ^= 1;
+                lists_live = true;
+            }
         }
         readback();
     }
     BLQH_HIP(hipStreamSynchronize(st));  // a speculative replay may still run
+    if (lists_live) {  // every member takes part in the final products again
+        copy_sync(b->xitems.get(), b->h_xitems.data(), b->h_xitems.size() * 4, hipMemcpyHostToDevice);
+        copy_sync(b->xseg.get(), b->h_xseg.data(), 27 * 4, hipMemcpyHostToDevice);
+    }
 
     phase("iterate");
     // pass 2 per problem, then psi = sum_l c_l conj(v_l), normalised, and Re<psi|H psi>
