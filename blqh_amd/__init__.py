@@ -209,7 +209,9 @@ def params(nphi, gamma=1.0, nLL=0, d=0.0, delta_sas=0.0, delta_v=0.0, delta_z=0.
 
 class Basis:
     """(J,K) sector basis: orbit representatives ``id`` (ascending) and
-    periods ``period`` (the reference's id[] and basis_C[])."""
+    periods ``period`` (the reference's id[] and basis_C[]).  nspecies 2 is
+    the spinless bilayer, 4 the spinful one, 1 the monolayer (Monolayer/basis.cpp:
+    translations by nphi / gcd(nel, nphi) orbitals, K < gcd(nel, nphi), no nel_up)."""
 
     def __init__(self, nphi, nel, nel_up=-1, J=-1, K=-1, nspecies=2, nel_su=-1):
         self.sector = Sector(nspecies, nphi, nel, nel_up, nel_su, J, K)

@@ -181,12 +181,15 @@ __global__ void k_compact(int64_t M, RankSpec s, const uint32_t* __restrict__ cn
         if (flags[r]) id[offs[r]] = unrank_word(s, cnt, r);
 }
 
-// basis_C: smallest n in [1, nphi) with inv_translate(id, n) == id, else nphi.
+// basis_C: smallest n in [1, order) with inv_translate(id, n) == id, else the
+// order of the translation group (nphi in the bilayer trees, gcd(nel, nphi)
+// in the monolayer).
 __global__ void k_period(int64_t n, const uint64_t* __restrict__ id, Geo g, int16_t* __restrict__ period) {
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t c = id[j];
-        int per = g.nphi;
-        for (int k = 1; k < g.nphi; ++k) {
+        const int ord = g.order();
+        int per = ord;
+        for (int k = 1; k < ord; ++k) {
             int sg;
             if (inv_translate(g, c, k, sg) == c) { per = k; break; }
         }
@@ -250,7 +253,8 @@ void dfs_words(const blqh_sector& s, F&& emit) {
 }
 
 void validate_sector(const blqh_sector& s) {
-    BLQH_REQUIRE(s.nspecies == 2 || s.nspecies == 4, BLQH_EINVAL, "nspecies must be 2 or 4");
+    BLQH_REQUIRE(s.nspecies == 1 || s.nspecies == 2 || s.nspecies == 4, BLQH_EINVAL, "nspecies must be 1, 2 or 4");
+    BLQH_REQUIRE(s.nspecies != 1 || s.nel_up < 0, BLQH_EINVAL, "a monolayer sector has no layer occupation (nel_up = -1)");
     BLQH_REQUIRE(s.nphi >= 2 && s.nphi <= 31 && s.nspecies * s.nphi <= 64, BLQH_EINVAL,
                  "nphi must satisfy 2 <= nphi <= 31 and nspecies*nphi <= 64");
     BLQH_REQUIRE(s.nel >= 1 && s.nel <= s.nspecies * s.nphi, BLQH_EINVAL, "nel out of range");
@@ -258,6 +262,7 @@ void validate_sector(const blqh_sector& s) {
     BLQH_REQUIRE(s.nel_up < 0 || (s.nel_up <= s.nspecies * s.nphi / 2 && s.nel - s.nel_up <= s.nspecies * s.nphi / 2),
                  BLQH_EINVAL, "layer occupation exceeds the layer");
     BLQH_REQUIRE(s.J < s.nphi && s.K < s.nphi, BLQH_EINVAL, "J and K must be < nphi");
+    BLQH_REQUIRE(s.nspecies != 1 || s.K < make_geo(s).order(), BLQH_EINVAL, "monolayer K must be < gcd(nel, nphi)");
     BLQH_REQUIRE(s.nel_su < 0 || s.nspecies == 4, BLQH_EINVAL, "nel_su needs nspecies = 4");
 }
 
@@ -371,7 +376,7 @@ void blqh_basis_build_impl(const blqh_sector& s, blqh_basis* B) {
     validate_sector(s);
     hipStream_t st = stream();
     B->sec = s;
-    B->geo = Geo{s.nspecies, s.nphi, s.nel};
+    B->geo = make_geo(s);
     const bool reduce = s.K >= 0;
     const bool spin_filter = s.nspecies == 4 && s.nel_su >= 0;
 

@@ -25,16 +25,22 @@ BLQH_HD uint64_t lowbits(int k) { return k >= 64 ? ~0ULL : ((1ULL << k) - 1ULL);
 BLQH_HD uint64_t range_bits(int a, int b) { return b <= a ? 0ULL : (lowbits(b) & ~lowbits(a)); }
 
 struct Geo {
-    int S;     // species blocks
-    int nphi;  // orbitals per block
+    int S;         // species blocks (1: monolayer)
+    int nphi;      // orbitals per block
     int nel;
+    int step = 1;  // orbitals per translation unit: 1 in the bilayer trees,
+                   // nphi / gcd(nel, nphi) in the monolayer
+    // translation units before the identity
+    BLQH_HD int order() const { return nphi / step; }
 };
 
-// Magnetic translation by k (rotate every block left by k) with fermion sign
-// (-1)^{sum_b (n_b - w_b) w_b}, w_b = electrons of block b that wrap.
-// Reference: BLQH_without_spin/basis.cpp:292-316.
+// Magnetic translation by k units (rotate every block left by k * step) with
+// fermion sign (-1)^{sum_b (n_b - w_b) w_b}, w_b = electrons of block b that
+// wrap.  Reference: BLQH_without_spin/basis.cpp:292-316 (step 1),
+// Monolayer/basis.cpp:257-272 (step nphi / gcd).
 BLQH_HD uint64_t translate(const Geo& g, uint64_t c, int k, int& sign) {
     if (k == 0) { sign = 1; return c; }
+    k *= g.step;
     const uint64_t m = lowbits(g.nphi);
     uint64_t out = 0;
     int ns = 0;
@@ -48,9 +54,11 @@ BLQH_HD uint64_t translate(const Geo& g, uint64_t c, int k, int& sign) {
     return out;
 }
 
-// Inverse translation (rotate right by k).  BLQH_without_spin/basis.cpp:318-340.
+// Inverse translation (rotate right by k units).  BLQH_without_spin/basis.cpp:318-340,
+// Monolayer/basis.cpp:274-289.
 BLQH_HD uint64_t inv_translate(const Geo& g, uint64_t c, int k, int& sign) {
     if (k == 0) { sign = 1; return c; }
+    k *= g.step;
     const uint64_t m = lowbits(g.nphi);
     uint64_t out = 0;
     int ns = 0;

@@ -64,6 +64,27 @@ struct blqh_hamil {
 
 namespace blqh {
 
+// Translation geometry of a sector: the bilayer trees translate by one
+// orbital (nphi units), the monolayer by nphi / gcd(nel, nphi) orbitals
+// (gcd(nel, nphi) units, Monolayer/basis.cpp:46-58).
+inline Geo make_geo(const blqh_sector& s) {
+    Geo g{s.nspecies, s.nphi, s.nel};
+    if (s.nspecies == 1) {
+        int c = 1;
+        for (int i = 1; i <= s.nel; ++i)
+            if (s.nel % i == 0 && s.nphi % i == 0) c = i;
+        g.step = s.nphi / c;
+    }
+    return g;
+}
+
+// Species-dependent switches of the row terms.
+inline void set_row_species(RowCtx& c, const blqh_sector& s) {
+    c.spinless = s.nspecies == 2;
+    c.twist_sel = s.nspecies == 2 ? 1 : 2;
+    c.mono_odd = s.nspecies == 1 && (s.nphi & 1);
+}
+
 RankSpec make_rank_spec(const blqh_sector& s, bool with_spin_constraint);
 std::vector<uint32_t> make_rank_table(const RankSpec& spec, int64_t* total);
 void exclusive_scan(const int32_t* in, int64_t* out, int64_t n, hipStream_t st);  // out has n+1 entries
@@ -133,8 +154,7 @@ inline RowCtx make_row_ctx(const blqh_basis* B) {
     c.spec = B->spec;
     c.M = B->M;
     c.K = B->sec.K;
-    c.spinless = B->sec.nspecies == 2;
-    c.twist_sel = B->sec.nspecies == 2 ? 1 : 2;
+    set_row_species(c, B->sec);
     c.period = B->period.get();
     c.lut = B->lut.get();
     c.cnt = B->cnt.get();

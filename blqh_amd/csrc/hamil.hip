@@ -72,6 +72,48 @@ struct CoulombEntry {
 std::mutex g_coulomb_mu;
 std::vector<CoulombEntry> g_coulomb;  // most recent last, at most 8
 
+// Monolayer (Monolayer/lanczos_hamiltonian.cpp:53-76): V(sx, ky) sums kx over
+// [-(nphi/2), nphi/2] (C integer division) and the element is 2 V(|dm|, |t|)
+// for t in [-(nphi/2), nphi/2].  The row kernel runs t over (-nphi, nphi), so
+// every target of shift t is met twice (at t and at t -/+ nphi) except t = 0;
+// the table is folded to give the same sums: T(sx, 0) = 2 V(sx, 0), for even
+// nphi T(sx, nphi/2) = 2 V(sx, nphi/2) (the tree visits both t = +-nphi/2),
+// else T(sx, a) = V(sx, min(a, nphi - a)).  Background
+// E_cl = (-2 + sum_{(i,j) in [0,nphi)^2 \ 0} sqrt(pi/z) erfc(sqrt(z))) / sqrt(lx ly),
+// z = pi (i^2 lx/ly + j^2 ly/lx): the closed form of the tree's integral
+// int_1^inf t^-1/2 e^{-zt} dt.  The interlayer half of the table stays zero.
+void monolayer_tables(const blqh_sector& s, const blqh_params& p, std::vector<double>& C, double& Ec) {
+    const int nphi = s.nphi;
+    const double lx = p.lx, ly = p.ly;
+    auto v = [&](int kx, int ky) {
+        const double q = std::sqrt(kx * kx / (lx * lx) + ky * ky / (ly * ly)) * 2.0 * M_PI;
+        return 2.0 * M_PI / (q + 1e-30) * std::exp(-q * q / 2.0) * std::pow(1.0 - std::exp(-q * q / 2.0), p.nLL * 2);
+    };
+    std::vector<double> V(nphi * nphi, 0.0);
+    for (int sx = 0; sx < nphi; ++sx)
+        for (int ky = 0; ky <= nphi / 2; ++ky) {
+            double x = 0;
+            for (int kx = -nphi / 2; kx <= nphi / 2; ++kx)
+                if (!(ky == 0 && kx == 0)) x += v(kx, ky) * std::cos(2.0 * M_PI * sx * kx / nphi) / (2.0 * lx * ly);
+            V[sx * nphi + ky] = x;
+        }
+    C.assign(2 * nphi * nphi, 0.0);
+    for (int sx = 0; sx < nphi; ++sx)
+        for (int a = 0; a < nphi; ++a) {
+            const bool both = a == 0 || 2 * a == nphi;
+            const int ky = a <= nphi - a ? a : nphi - a;
+            C[sx * nphi + a] = both ? 2.0 * V[sx * nphi + ky] : V[sx * nphi + ky];
+        }
+    Ec = -2.0;
+    for (int i = 0; i < nphi; ++i)
+        for (int j = 0; j < nphi; ++j)
+            if (!(i == 0 && j == 0)) {
+                const double z = (i * i * lx / ly + j * j * ly / lx) * M_PI;
+                Ec += std::sqrt(M_PI / z) * std::erfc(std::sqrt(z));
+            }
+    Ec /= std::sqrt(lx * ly);
+}
+
 void coulomb_tables(const blqh_sector& s, const blqh_params& p, std::vector<double>& C, double& Ec) {
     const CoulombKey key{s.nphi, s.nspecies, p.nLL, p.lx, p.ly, p.d, p.theta_x};
     {
@@ -85,6 +127,13 @@ void coulomb_tables(const blqh_sector& s, const blqh_params& p, std::vector<doub
     }
     const int nphi = s.nphi;
     const double lx = p.lx, ly = p.ly;
+    if (s.nspecies == 1) {
+        monolayer_tables(s, p, C, Ec);
+        std::lock_guard<std::mutex> lk(g_coulomb_mu);
+        if (g_coulomb.size() >= 8) g_coulomb.erase(g_coulomb.begin());
+        g_coulomb.push_back({key, C, Ec});
+        return;
+    }
     Ec = -2.0;
     for (int i = 0; i < nphi; ++i)
         for (int j = 0; j < nphi; ++j)
@@ -121,11 +170,19 @@ void coulomb_tables(const blqh_sector& s, const blqh_params& p, std::vector<doub
 HostTables host_tables(const blqh_sector& s, const blqh_params& p) {
     HostTables H;
     const int nphi = s.nphi;
+    if (s.nspecies == 1)
+        BLQH_REQUIRE(p.d == 0 && p.delta_sas == 0 && p.delta_v == 0 && p.delta_z == 0 && p.theta_x == 0 &&
+                         p.theta_y == 0 && p.theta_b == 0,
+                     BLQH_EINVAL, "a monolayer takes no layer, Zeeman or twist parameters (d, Delta_*, theta_* = 0)");
     coulomb_tables(s, p, H.C, H.Ec);
-    // the reference's FT[a][b] depends on a - b only (same argument, same bits)
+    // the reference's FT[a][b] depends on a - b only (same argument, same bits);
+    // the monolayer's phase is exp(2 pi i K dq / gcd(nel, nphi)) with |dq| < gcd
     H.FT.resize(2 * nphi - 1);
-    for (int dq = -nphi + 1; dq < nphi; ++dq)
-        H.FT[dq + nphi - 1] = make_double2(std::cos(2.0 * M_PI * dq * s.K / nphi), std::sin(2.0 * M_PI * dq * s.K / nphi));
+    const int ord = make_geo(s).order();
+    for (int dq = -nphi + 1; dq < nphi; ++dq) {
+        const double ang = s.nspecies == 1 ? 2.0 * M_PI * s.K * dq / ord : 2.0 * M_PI * dq * s.K / nphi;
+        H.FT[dq + nphi - 1] = make_double2(std::cos(ang), std::sin(ang));
+    }
     H.tw.resize(2 * nphi - 1);
     for (int t = -nphi + 1; t < nphi; ++t) {
         const double ang = s.nspecies == 2 ? 2 * p.theta_y * t / nphi : p.theta_y * t / nphi;

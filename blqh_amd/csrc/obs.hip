@@ -163,6 +163,7 @@ void blqh_observables_impl(const blqh_basis* B, const double* psi_host, blqh_obs
     const int64_t n = B->n;
     const Geo& g = B->geo;
     BLQH_REQUIRE(g.nphi <= 64, BLQH_EINVAL, "too many orbitals");
+    BLQH_REQUIRE(g.S >= 2, BLQH_EINVAL, "the pseudospin and spin observables need a bilayer basis");
     DevBuf<double2> psi(std::max<int64_t>(n, 1));
     BLQH_HIP(hipMemcpyAsync(psi.get(), psi_host, n * sizeof(double2), hipMemcpyHostToDevice, st));
     // FT phases of the sector

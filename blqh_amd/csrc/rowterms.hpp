@@ -22,11 +22,12 @@ struct RowCtx {
     int K;          // < 0: no translation reduction (D_l = D_r = 1)
     int spinless;   // operand order of the reference's spinless / spinful expressions
     int twist_sel;  // 1: spinless interlayer pairs, 2: spinful (0,3)/(1,2) pairs
+    int mono_odd;   // monolayer with odd nphi: the pair sign counts occupied orbitals (below)
     const int16_t* period;
     const int32_t* lut;
     const uint32_t* cnt;
     const double* C;     // [2][nphi][nphi] Coulomb table V[alpha][|dm|][|t|]
-    const double2* FT;   // [2 nphi - 1]: FT[ql - qr + nphi - 1] = exp(2 pi i (ql - qr) K / nphi)
+    const double2* FT;   // [2 nphi - 1]: FT[ql - qr + nphi - 1] = exp(2 pi i (ql - qr) K / order)
     const double2* tw;   // [2 nphi - 1] twist phase per t
 };
 
@@ -61,7 +62,7 @@ BLQH_HD int mod_small(int s, int d) {
 // orbit member in the sector comes later.
 template <class T>
 BLQH_HD bool is_survivor(const Geo& g, const RankSpec& s, const T* cnt, const int32_t* posOf, uint64_t c, int32_t p) {
-    for (int n = 1; n < g.nphi; ++n) {
+    for (int n = 1; n < g.order(); ++n) {
         int sg;
         const uint64_t c2 = inv_translate(g, c, n, sg);
         if (c2 != c && in_sector(s, c2) && posOf[rank_word(s, cnt, c2)] > p) return false;
@@ -111,6 +112,7 @@ BLQH_HD bool pair_term(const RowCtx& c, uint64_t L, int signl, int Dl, int ql, i
     const int b = m >= nphi ? (m >= 2 * nphi ? (m >= 3 * nphi ? 3 : 2) : 1) : 0;
     const int n0 = n - a * nphi, m0 = m - b * nphi;
     int xn = n0 + t, xm = m0 - t;
+    const bool wn = xn < 0 || xn >= nphi, wm = xm < 0 || xm >= nphi;
     xn += xn < 0 ? nphi : (xn >= nphi ? -nphi : 0);
     xm += xm < 0 ? nphi : (xm >= nphi ? -nphi : 0);
     const int nt = a * nphi + xn;
@@ -120,7 +122,11 @@ BLQH_HD bool pair_term(const RowCtx& c, uint64_t L, int signl, int Dl, int ql, i
     if (nt == mt || (rest & tgt)) return false;
     int qr, Dr, sr;
     if (!locate(c, rest | tgt, ql, j, qr, Dr, sr)) return false;
-    const int sign = sign2(L, n, m, nt, mt) * signl * sr;
+    // The monolayer counts occupied orbitals between n and nt (and m and mt)
+    // where sign2 counts empty ones (Monolayer/basis.cpp:169-195).  The counts
+    // add up to |nt - n| - 1 and |mt - m| - 1, whose sum is even unless
+    // exactly one of the two electrons wraps, and then has the parity of nphi.
+    const int sign = sign2(L, n, m, nt, mt) * signl * sr * (c.mono_odd && wn != wm ? -1 : 1);
     const int sh = c.g.S == 4 ? 1 : 0;  // layer of block a: a / (S / 2)
     const int alpha = (a >> sh) != (b >> sh) ? 1 : 0;
     int dm = xm - n0;

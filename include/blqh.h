@@ -28,9 +28,13 @@ extern "C" {
 
 /* A (J,K) sector.  Reference: basis::init(nphi, nel, nel_up, J, K),
  * BLQH_without_spin/basis.cpp:141-149; spinful
- * basis::init(nphi, nel, nel_up, nel_su, J, K), BLQH_with_spin/basis.cpp:177-188. */
+ * basis::init(nphi, nel, nel_up, nel_su, J, K), BLQH_with_spin/basis.cpp:177-188;
+ * monolayer basis::init(nphi, nel, J, K), Monolayer/basis.cpp:55-131 (its
+ * translations rotate by nphi / gcd(nel, nphi) orbitals, so K < gcd(nel, nphi);
+ * nel_up = nel_su = -1; the Hamiltonian takes only lx, ly and nLL). */
 typedef struct blqh_sector {
-    int32_t nspecies; /* 2: spinless bilayer; 4: spinful bilayer (upper-up, upper-down, lower-up, lower-down) */
+    int32_t nspecies; /* 1: monolayer; 2: spinless bilayer; 4: spinful bilayer (upper-up, upper-down,
+                         lower-up, lower-down) */
     int32_t nphi;     /* flux quanta (orbitals) per species */
     int32_t nel;      /* electrons */
     int32_t nel_up;   /* electrons in the upper layer; -1 = all layer densities */

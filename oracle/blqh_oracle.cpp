@@ -32,8 +32,11 @@
 // vectors, entry by entry (phases, 1/sqrt(D_l D_r), translation signs); the
 // spinful pair blocks by the spin-polarised = spinless spectrum and by SU(2)
 // multiplets at zero Zeeman energy; Lanczos against LAPACK; the RNG against
-// MT19937's published 10000th draw.  g_plant plants mistakes for the tests
-// that show these checks fail on them.
+// MT19937's published 10000th draw.  The monolayer sectors against the
+// Monolayer tree's check data (35 ground energies per particle over three
+// electron numbers and the 10 lowest levels of all 108 (J, K) sectors at
+// nphi = 18, nel = 6, 6 significant digits) and the same Bloch projection.
+// g_plant plants mistakes for the tests that show these checks fail on them.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -771,6 +774,234 @@ void* orc_csr_wrap(int64_t n, const int64_t* rowptr, const int64_t* col, const d
 }
 
 void orc_spmv(void* h, const double* x, double* y) { spmv(*(Csr*)h, (const cplx*)x, (cplx*)y); }
+
+// ---- Monolayer (single layer) in the conventions of the Monolayer tree ----
+// Basis: Monolayer/basis.cpp:55-115 (init), 169-195 (get_sign), 215-241
+// (generate), 257-289 (translate / inv_translate).  Hamiltonian:
+// Monolayer/lanczos_hamiltonian.cpp:53-176 (init_Coulomb_matrix, set_hamil).
+//  * sector words: nel electrons in nphi orbitals, depth-first (lowest orbital
+//    of the first electron slowest), J filter sum(orbitals) % nphi == J;
+//  * translations: C = gcd(nel, nphi) of them, each a left rotation by
+//    q = nphi / C orbitals, sign (-1)^(w (nel - w)) with w the electrons that
+//    wrap; with K >= 0 an orbit member is erased while iterating the
+//    std::unordered_map when another (different) member is still present,
+//    the survivors sorted ascending; basis_C = the orbit's period;
+//  * V(s, ky) = sum over kx in [-(nphi/2), nphi/2] (C integer division),
+//    (kx, ky) != 0, of 2 pi / (q + 1e-30) exp(-q^2/2) (1 - exp(-q^2/2))^(2 nLL)
+//    cos(2 pi s kx / nphi) / (2 lx ly);
+//  * pair (n < m) occupied in the k1-translated left word, t in
+//    [-(nphi/2), nphi/2], nt = n + t, mt = m - t wrapped once, s = |n - mt|;
+//    both targets empty: for every k2 below the target's period, the
+//    k2-inverse-translated target, when a basis row, receives
+//    2 V(s, |t|) sign FT, FT = exp(2 pi i K (k1 - k2) / C) / sqrt(C_l C_r);
+//  * sign: (-1)^(occupied orbitals strictly between n and nt, and between m
+//    and mt, in the word without n and m), times -1 when the pair crosses
+//    (nt > mt and m > n, or mt > nt and m < n), times both translation signs;
+//  * diagonal E_cl * nel, E_cl = (-2 + sum_{(i,j) in [0,nphi)^2 \ 0}
+//    I(pi (i^2 lx/ly + j^2 ly/lx))) / sqrt(lx ly) with
+//    I(z) = int_1^inf t^-1/2 e^{-zt} dt = sqrt(pi/z) erfc(sqrt(z)) (the tree
+//    integrates it numerically with GSL);
+//  * entries with |h| > 1e-10 kept, columns ascending.
+// A target with nt == mt has nel - 1 electrons in the tree (its mask adds the
+// same bit twice) and never matches a row; it is skipped here.
+namespace {
+struct MonoBasis {
+    int nphi = 0, nel = 0, J = -1, K = -1, C = 1, q = 1;
+    std::vector<uint64_t> id;
+    std::vector<int> per;
+    std::unordered_map<unsigned long, long> row;
+
+    uint64_t mask() const { return (1ull << nphi) - 1; }
+    uint64_t fwd(uint64_t c, int k, int& sign) const {
+        const int r = k * q;
+        c &= mask();
+        if (r == 0 || r == nphi) { sign = 1; return c; }
+        const int w = __builtin_popcountll(c >> (nphi - r));
+        sign = ((w * (nel - w)) & 1) ? -1 : 1;
+        return ((c << r) | (c >> (nphi - r))) & mask();
+    }
+    uint64_t bwd(uint64_t c, int k, int& sign) const {
+        const int r = k * q;
+        c &= mask();
+        if (r == 0 || r == nphi) { sign = 1; return c; }
+        const int w = __builtin_popcountll(c & ((1ull << r) - 1));
+        sign = ((w * (nel - w)) & 1) ? -1 : 1;
+        return ((c >> r) | (c << (nphi - r))) & mask();
+    }
+};
+
+void mono_words(int nphi, int nel, int J, int k, int from, uint64_t c, int jsum,
+                std::unordered_map<unsigned long, long>& set) {
+    for (int i = from; i < nphi - (nel - k - 1); ++i) {
+        const uint64_t w = c | (1ull << i);
+        if (k + 1 < nel)
+            mono_words(nphi, nel, J, k + 1, i + 1, w, jsum + i, set);
+        else if (J < 0 || (jsum + i) % nphi == J)
+            set[(unsigned long)w] = (long)w;
+    }
+}
+
+void mono_basis(int nphi, int nel, int J, int K, MonoBasis& B) {
+    B.nphi = nphi;
+    B.nel = nel;
+    B.J = J;
+    B.K = K;
+    B.C = 1;
+    for (int i = 1; i <= nel; ++i)
+        if (nel % i == 0 && nphi % i == 0) B.C = i;
+    B.q = nphi / B.C;
+    std::unordered_map<unsigned long, long> set;
+    mono_words(nphi, nel, J, 0, 0, 0, 0, set);
+    if (K >= 0) {
+        for (auto it = set.begin(); it != set.end();) {
+            const uint64_t c = it->first;
+            bool gone = false;
+            for (int k = 1; k < B.C && !gone; ++k) {
+                int sg;
+                const uint64_t y = B.bwd(c, k, sg);
+                gone = y != c && set.find((unsigned long)y) != set.end();
+            }
+            if (gone)
+                it = set.erase(it);
+            else
+                ++it;
+        }
+    }
+    B.id.clear();
+    for (const auto& kv : set) B.id.push_back((uint64_t)kv.second);
+    std::sort(B.id.begin(), B.id.end());
+    B.row.clear();
+    for (size_t i = 0; i < B.id.size(); ++i) B.row[(unsigned long)B.id[i]] = (long)i;
+    B.per.assign(B.id.size(), B.C);
+    for (size_t i = 0; i < B.id.size(); ++i)
+        for (int k = 1; k < B.C; ++k) {
+            int sg;
+            if (B.fwd(B.id[i], k, sg) == B.id[i]) { B.per[i] = k; break; }
+        }
+}
+
+double mono_tables(int nphi, double lx, double ly, int nLL, std::vector<double>& Cm) {
+    auto V = [&](int kx, int ky) {
+        const double q = std::sqrt(kx * kx / (lx * lx) + ky * ky / (ly * ly)) * 2.0 * M_PI;
+        return 2.0 * M_PI / (q + 1e-30) * std::exp(-q * q / 2.0) * std::pow(1.0 - std::exp(-q * q / 2.0), nLL * 2);
+    };
+    Cm.assign(nphi * nphi, 0.0);
+    for (int s = 0; s < nphi; ++s)
+        for (int ky = 0; ky < nphi; ++ky) {
+            double v = 0;
+            for (int kx = -nphi / 2; kx <= nphi / 2; ++kx)
+                if (!(ky == 0 && kx == 0)) v += V(kx, ky) * std::cos(2.0 * M_PI * s * kx / nphi) / (2.0 * lx * ly);
+            Cm[s * nphi + ky] = v;
+        }
+    double ecl = -2.0;
+    for (int i = 0; i < nphi; ++i)
+        for (int j = 0; j < nphi; ++j)
+            if (!(i == 0 && j == 0)) {
+                const double z = (i * i * lx / ly + j * j * ly / lx) * M_PI;
+                ecl += std::sqrt(M_PI / z) * std::erfc(std::sqrt(z));
+            }
+    return ecl / std::sqrt(lx * ly);
+}
+
+// (-1)^(occupied orbitals of the word without n and m strictly between n and
+// nt and between m and mt), times -1 for a crossing pair.
+int mono_sign(uint64_t c, int n, int m, int nt, int mt) {
+    const uint64_t b = c ^ ((1ull << n) | (1ull << m));
+    const int occ = g_plant == 5 ? 0 : 1;  // plant 5: count empty orbitals (the bilayer trees' rule)
+    int ns = 0;
+    for (int k = std::min(n, nt) + 1; k < std::max(n, nt); ++k) ns += (int)((b >> k) & 1) == occ;
+    for (int k = std::min(m, mt) + 1; k < std::max(m, mt); ++k) ns += (int)((b >> k) & 1) == occ;
+    if ((nt > mt && m > n) || (mt > nt && m < n)) ++ns;
+    return (ns & 1) ? -1 : 1;
+}
+}  // namespace
+
+// Basis of the (J, K) sector (K < 0: no translation reduction): returns n and
+// writes ids / periods when n <= cap (else -n).
+int64_t orc_monolayer_basis(int nphi, int nel, int J, int K, uint64_t* id, int16_t* period, int64_t cap) {
+    MonoBasis B;
+    mono_basis(nphi, nel, J, K, B);
+    const int64_t n = (int64_t)B.id.size();
+    if (n > cap) return -n;
+    for (int64_t i = 0; i < n; ++i) {
+        id[i] = B.id[i];
+        period[i] = (int16_t)B.per[i];
+    }
+    return n;
+}
+
+void* orc_monolayer_new(int nphi, int nel, int J, int K, double lx, double ly, int nLL, double* ecl_out) {
+    MonoBasis B;
+    mono_basis(nphi, nel, J, K, B);
+    std::vector<double> Cm;
+    const double ecl = mono_tables(nphi, lx, ly, nLL, Cm);
+    if (ecl_out) *ecl_out = ecl;
+    const long n = (long)B.id.size();
+    Csr* H = new Csr;
+    H->n = n;
+    H->rowptr.assign(n + 1, 0);
+    std::vector<cplx> row(n, cplx(0, 0));
+    std::vector<char> hit(n, 0);
+    std::vector<long> touched;
+    for (long i = 0; i < n; ++i) {
+        touched.clear();
+        auto add = [&](long j, cplx v) {
+            if (!hit[j]) { hit[j] = 1; touched.push_back(j); }
+            row[j] += v;
+        };
+        const int Cl = K < 0 ? 1 : B.per[i];
+        for (int k1 = 0; k1 < Cl; ++k1) {
+            int signl = 1;
+            const uint64_t L = k1 == 0 ? B.id[i] : B.fwd(B.id[i], k1, signl);
+            for (int a = 0; a < nphi - 1; ++a)
+                for (int b = a + 1; b < nphi; ++b) {
+                    const uint64_t pm = (1ull << a) | (1ull << b);
+                    if ((L & pm) != pm) continue;
+                    const uint64_t rest = L ^ pm;
+                    for (int t = -nphi / 2; t <= nphi / 2; ++t) {
+                        int nt = a + t, mt = b - t;
+                        nt += nt >= nphi ? -nphi : (nt < 0 ? nphi : 0);
+                        mt += mt >= nphi ? -nphi : (mt < 0 ? nphi : 0);
+                        if (nt == mt) continue;
+                        const int sx = std::abs(a - mt);
+                        const uint64_t tg = (1ull << nt) | (1ull << mt);
+                        if (tg & rest) continue;
+                        const uint64_t R = rest | tg;
+                        int Cr = 1;
+                        if (K >= 0) {
+                            Cr = B.C;
+                            for (int k2 = 1; k2 <= B.C; ++k2) {
+                                int sg;
+                                if (B.fwd(R, k2, sg) == R) { Cr = k2; break; }
+                            }
+                        }
+                        for (int k2 = 0; k2 < Cr; ++k2) {
+                            int signr = 1;
+                            const uint64_t Rb = k2 == 0 ? R : B.bwd(R, k2, signr);
+                            auto it = B.row.find((unsigned long)Rb);
+                            if (it == B.row.end()) continue;
+                            const int sign = mono_sign(L, a, b, nt, mt) * signl * signr;
+                            const double ang = 2.0 * M_PI * K * (g_plant == 6 ? k2 - k1 : k1 - k2) / B.C;
+                            const cplx ft = cplx(std::cos(ang), std::sin(ang)) / std::sqrt((double)(Cl * Cr));
+                            add(it->second, 2.0 * Cm[sx * nphi + std::abs(t)] * sign * ft);
+                        }
+                    }
+                }
+        }
+        add(i, cplx(ecl * nel, 0.0));
+        std::sort(touched.begin(), touched.end());
+        for (long j : touched) {
+            if (std::abs(row[j]) > 1e-10) {
+                H->col.push_back(j);
+                H->val.push_back(row[j]);
+            }
+            row[j] = cplx(0, 0);
+            hit[j] = 0;
+        }
+        H->rowptr[i + 1] = (int64_t)H->col.size();
+    }
+    return H;
+}
 
 // The same row loop over caller arrays with 32-bit columns (no copy of a
 // large matrix); nthreads <= 0: OpenMP's default.

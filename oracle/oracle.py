@@ -77,6 +77,11 @@ def lib():
         L.orc_hamil_rows.restype = i64
         L.orc_hamil_rows.argtypes = [vp, C.POINTER(Params), vp, i64, C.c_int, vp, vp, vp, i64]
         L.orc_plant.argtypes = [C.c_int]
+        L.orc_monolayer_new.restype = vp
+        L.orc_monolayer_new.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                        C.POINTER(C.c_double)]
+        L.orc_monolayer_basis.restype = i64
+        L.orc_monolayer_basis.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, i64]
         L.orc_spmv32.argtypes = [i64, vp, vp, vp, vp, vp, C.c_int]
         L.orc_csr_wrap.restype = vp
         L.orc_csr_wrap.argtypes = [i64, vp, vp, vp]
@@ -295,7 +300,8 @@ class planted:
     """Context: the oracle builds Hamiltonians with a planted mistake (tests
     that show a check fails on it): 1 no spinful (0,1) pair block, 2 the
     interlayer table for the (2,3) block, 3 conjugated FT phase, 4 1/sqrt(D_l)
-    in place of 1/sqrt(D_l D_r)."""
+    in place of 1/sqrt(D_l D_r); monolayer: 5 the pair sign counts empty
+    orbitals instead of occupied ones, 6 conjugated FT phase."""
 
     def __init__(self, which):
         self.which = which
@@ -305,3 +311,33 @@ class planted:
 
     def __exit__(self, *a):
         lib().orc_plant(0)
+
+
+def monolayer(nphi, nel, J=-1, K=-1, gamma=1.0, nLL=0, lx=None, ly=None):
+    """The (J, K) sector of the single-layer Hamiltonian in the Monolayer
+    tree's conventions (K < 0: no translation reduction).  The torus is
+    lx x ly with lx ly = 2 pi nphi and lx / ly = gamma unless lx, ly are
+    given.  Returns a dict with the basis (id, period), the CSR (rowptr, col,
+    val) and E_cl."""
+    if lx is None or ly is None:
+        lx, ly = geometry(nphi, gamma)
+    cap = 1
+    while True:
+        ids = np.empty(cap, np.uint64)
+        per = np.empty(cap, np.int16)
+        n = lib().orc_monolayer_basis(nphi, nel, J, K, _p(ids), _p(per), cap)
+        if n >= 0:
+            break
+        cap = -n
+    ecl = C.c_double()
+    h = lib().orc_monolayer_new(nphi, nel, J, K, lx, ly, nLL, C.byref(ecl))
+    nr = lib().orc_hamil_n(h)
+    nnz = lib().orc_hamil_nnz(h)
+    rp = np.empty(nr + 1, np.int64)
+    col = np.empty(nnz, np.int64)
+    val = np.empty(nnz, np.complex128)
+    lib().orc_hamil_export(h, _p(rp), _p(col), _p(val))
+    lib().orc_hamil_free(h)
+    assert nr == n
+    return dict(id=ids[:n].copy(), period=per[:n].copy(), rowptr=rp, col=col, val=val, ecl=ecl.value,
+                lx=lx, ly=ly)

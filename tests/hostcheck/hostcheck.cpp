@@ -31,7 +31,7 @@ struct HostBasis {
 
 void build(const blqh_sector& s, HostBasis& B) {
     B.s = s;
-    B.g = Geo{s.nspecies, s.nphi, s.nel};
+    B.g = make_geo(s);
     RankSpec smap = make_rank_spec(s, false);
     int64_t Mmap = 0;
     std::vector<uint32_t> tmap = make_rank_table(smap, &Mmap);
@@ -50,9 +50,9 @@ void build(const blqh_sector& s, HostBasis& B) {
         if (keep) B.id.push_back(c);
     }
     const int64_t n = (int64_t)B.id.size();
-    B.period.assign(n, (int16_t)s.nphi);
+    B.period.assign(n, (int16_t)B.g.order());
     for (int64_t j = 0; j < n; ++j)
-        for (int k = 1; k < s.nphi; ++k) {
+        for (int k = 1; k < B.g.order(); ++k) {
             int sg;
             if (inv_translate(B.g, B.id[j], k, sg) == B.id[j]) { B.period[j] = (int16_t)k; break; }
         }
@@ -128,8 +128,7 @@ int64_t hc_hamil(const blqh_sector* s, const blqh_params* p, int64_t* rowptr, in
     c.spec = B.spec;
     c.M = B.M;
     c.K = s->K;
-    c.spinless = s->nspecies == 2;
-    c.twist_sel = s->nspecies == 2 ? 1 : 2;
+    set_row_species(c, *s);
     c.period = B.period.data();
     c.lut = B.lut.data();
     c.cnt = B.cnt.data();

@@ -151,3 +151,25 @@ def test_hash_order_replay_equals_the_real_map(case):
     and the benchmarks use (up to 10.4M words here)."""
     ok, n = hostmirror.hash_order_check(blqh_amd.Sector(*case))
     assert n > 0 and ok, (case, n)
+
+
+MONO = [(9, 3, 4, 2), (9, 3, 1, -1), (12, 4, 3, 3), (12, 6, 1, 2), (10, 5, 2, 1), (8, 4, 1, 1), (7, 3, 2, 0),
+        (18, 6, 3, 3), (11, 4, 5, 0), (14, 4, 0, 1)]
+
+
+@pytest.mark.parametrize("nphi,nel,J,K", MONO, ids=[f"n{a}_e{b}_J{c}_K{d}" for a, b, c, d in MONO])
+def test_mirror_monolayer_matches_oracle(oracle_mod, nphi, nel, J, K):
+    """Monolayer mode (nspecies = 1): gcd translations, folded Coulomb table,
+    occupied-orbital sign at odd nphi, E_cl background."""
+    O = oracle_mod
+    gamma = nel / 4.0
+    d = O.monolayer(nphi, nel, J, K, gamma=gamma)
+    sec = blqh_amd.Sector(1, nphi, nel, -1, -1, J, K)
+    ids, per = hostmirror.basis(sec)
+    np.testing.assert_array_equal(ids, d["id"])
+    np.testing.assert_array_equal(per, d["period"])
+    par = blqh_amd.params(nphi, gamma=gamma)
+    rp, col, val = hostmirror.hamil(sec, par, len(ids), 2 * len(d["col"]) + 64)
+    np.testing.assert_array_equal(rp, d["rowptr"])
+    np.testing.assert_array_equal(col, d["col"])
+    np.testing.assert_allclose(val, d["val"], rtol=0, atol=1e-13 * np.abs(d["val"]).max())
