@@ -330,6 +330,154 @@ def run_sweep(args):
     return out
 
 
+def run_sweep_shared(args):
+    """A sweep whose d values each carry several DSAS values: one step is one
+    d, all sectors at all its DSAS values in one shared-scale batch
+    (Batch.scaled: each sector's H(d,0) + T entries are read once per Lanczos
+    step for all of its DSAS problems).  Plain rate: single-problem solves of
+    the warm-up d's merged Hamiltonians H(d,0) + DSAS T, every sector and DSAS."""
+    import blqh_amd
+    sw = SWEEPS[args.workload]
+    S, nphi, nel = sw["S"], sw["nphi"], sw["nel"]
+    log(f"workload {args.workload} (shared-scale batches): {sw['label']}")
+    t0 = time.time()
+    bases = [blqh_amd.Basis(nphi, nel, sw["nu"], J, K, nspecies=S, nel_su=sw["nsu"]) for J, K in sw["sectors"]]
+    t_basis = time.time() - t0
+    log(f"{len(bases)} bases, N {min(b.n for b in bases)}..{max(b.n for b in bases)}, {t_basis:.1f} s")
+    d_list, ds_list = [], []
+    for d, ds in sw["points"]:
+        if d not in d_list:
+            d_list.append(d)
+        if ds not in ds_list:
+            ds_list.append(ds)
+    G = len(ds_list)
+    nwarm = max(1, args.warmup)
+    order = d_list[:nwarm] + d_list[nwarm:nwarm + args.steps]
+    seeds = list(range(1, len(bases) * G + 1))
+    bchk = args.batch_check_every or args.check_every
+
+    def par(d):
+        return blqh_amd.params(nphi, d=d, delta_sas=0.0, delta_v=sw["dv"], delta_z=sw["dz"])
+
+    tun = None  # T does not depend on d without theta_B phases: built once
+    batch = None
+    plain = None
+    solved = solves = 0
+    t_solve = t_build = 0.0
+    iters = []
+    for k, d in enumerate(order):
+        warm = k < nwarm
+        t = time.time()
+        base = [blqh_amd.Hamiltonian(B, par(d), terms=1) for B in bases]
+        if tun is None:
+            tun = [blqh_amd.Hamiltonian(B, par(d), terms=2) for B in bases]
+        if batch is None:
+            batch = blqh_amd.Batch.scaled(base, tun, ds_list)
+        else:
+            batch.rebind_scaled(base, tun)
+        tb = time.time() - t
+        t = time.perf_counter()
+        res = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False)
+        dt = time.perf_counter() - t
+        ok = sum(int(r["converged"] == 1 and r["dE"] < 1e-10) for r in res)
+        log(f"{'warmup' if warm else 'step'} d={d} DSAS={ds_list}: build {tb:.2f} s, shared-scale batched solve "
+            f"{dt:.3f} s, {ok}/{len(res)} converged, iterations {min(r['iterations'] for r in res)}.."
+            f"{max(r['iterations'] for r in res)}")
+        if warm and plain is None:
+            # plain single-problem solves of the same problems (each autotuned by a first solve)
+            hs = [blqh_amd.Hamiltonian.combine(b, tn, x) for b, tn in zip(base, tun) for x in ds_list]
+            for i, H in enumerate(hs):
+                H.lanczos(lam=args.lam, seed=seeds[i], tol=1e-10, check_every=args.check_every, want_psi=False)
+            tp = time.perf_counter()
+            pres = [H.lanczos(lam=args.lam, seed=seeds[i], tol=1e-10, check_every=args.check_every, want_psi=False)
+                    for i, H in enumerate(hs)]
+            tp = time.perf_counter() - tp
+            pok = sum(int(r["converged"] == 1 and r["dE"] < 1e-10) for r in pres)
+            dmax = max(abs(a["E0"] - b["E0"]) / abs(b["E0"]) for a, b in zip(res, pres))
+            plain = {"solves": len(pres), "seconds": tp, "rate": pok / tp, "max_rel_dE0_vs_batched": dmax}
+            log(f"plain: {len(pres)} single-problem solves in {tp:.3f} s ({pok / tp:.1f}/s); "
+                f"max |E0 batched - plain|/|E0| = {dmax:.1e}")
+            del hs, pres
+        if not warm:
+            solved += ok
+            solves += len(res)
+            t_solve += dt
+            t_build += tb
+            iters += [r["iterations"] for r in res]
+        if k == len(order) - 1:
+            blqh_amd.profile_reset()
+            blqh_amd.profile_enable(True)
+            batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False)
+            blqh_amd.profile_enable(False)
+            launches, kms, kbytes = blqh_amd.profile_read()
+            nnz_total = sum(H.nnz for H in base) + sum(H.nnz for H in tun)
+        del base  # the batch keeps its own copy of H(d,0) + T
+    avg_ms = kms / max(launches, 1)
+    achieved = (kbytes / max(launches, 1)) / (avg_ms * 1e-3) / 1e9 if launches else 0.0
+    value = solved / t_solve if t_solve > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "K-sector ground states/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": nwarm,
+        "ms_per_step": t_solve / max(1, args.steps) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": sw["label"],
+            "step": f"one d: all sectors at the {G} DSAS values, one shared-scale batched solve, H resident",
+            "sectors": len(bases),
+            "dsas_values": ds_list,
+            "points_timed": args.steps * G,
+            "solves_timed": solves,
+            "converged": solved,
+            "lanczos_cap": args.lam,
+            "tolerance": 1e-10,
+            "check_every": args.check_every,
+            "batch_check_every": bchk,
+            "iterations_mean": float(np.mean(iters)) if iters else None,
+            "batched_rate": value,
+            "plain_rate": plain["rate"] if plain else None,
+            "batched_over_plain": value / plain["rate"] if plain and plain["rate"] else None,
+            "plain": plain,
+            "end_to_end_rate_incl_build": solved / (t_solve + t_build) if t_solve + t_build > 0 else None,
+            "build_s_per_step": t_build / max(1, args.steps),
+            "build_scheme": "H(d,0) per d; T once per sector; the batch merges H(d,0) + T once per d and "
+                            "flags T's entries",
+            "basis_s_once": round(t_basis, 2),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": f"k_b1s<LZ=true, G={G}> (shared-scale batched fused SpMV + Lanczos K1: each sector's "
+                      f"entries read once for its {G} DSAS problems)",
+            "avg_launch_ms": avg_ms,
+            "launches": launches,
+            "algorithmic_bytes_per_launch": kbytes / max(launches, 1),
+            "algorithmic_bytes_definition": "per problem and step 20*nnz/G (the sector's entries, shared by "
+                                            "its G problems) + 56*N + 8",
+            "batch_nnz": nnz_total,
+            "measured": "separate profiled solve of the last d (plain launches with HIP events)",
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle on one sector)")
+        try:
+            out["cpu_baseline"] = sweep_cpu_baseline(sw, iters_ref=args.lam)
+        except Exception as e:  # report, never hide, a failed baseline
+            out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    return out
+
+
 # Learned warm start: train on ground states at coarse d, solve at other d.
 WARM = {
     "warm_nphi14": dict(S=2, nphi=14, nel=14, nu=-1, nsu=-1, dv=0.0, dz=0.0, sectors=[(0, 0), (7, 7)], dsas=0.05,
@@ -499,6 +647,9 @@ def main():
                     help="default workload: also time the sweep workloads at this many parameter points (0: skip)")
     ap.add_argument("--start-mix", type=float, default=0.1,
                     help="weight of the seeded random unit vector added to warm starts (guards level crossings)")
+    ap.add_argument("--shared", type=int, default=1,
+                    help="sweeps with several DSAS values per d: one shared-scale batch per d (1) or one batch "
+                         "per parameter point (0)")
     ap.add_argument("--continuation", type=int, default=0,
                     help="sweeps: also solve every point from the previous point's ground states (1) or not (0)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -518,7 +669,8 @@ def main():
         sys.exit(2)
 
     if args.workload in SWEEPS:
-        print(json.dumps(run_sweep(args)), flush=True)
+        shared = args.shared and len({ds for _, ds in SWEEPS[args.workload]["points"]}) > 1
+        print(json.dumps(run_sweep_shared(args) if shared else run_sweep(args)), flush=True)
         return
     if args.workload in WARM:
         run_warm(args)
@@ -618,7 +770,8 @@ def main():
             sa = argparse.Namespace(**vars(args))
             sa.workload, sa.steps, sa.warmup, sa.continuation = name, args.sweep_points, 1, 0
             sa.no_cpu_baseline = args.no_cpu_baseline
-            r = run_sweep(sa)
+            shared = args.shared and len({ds for _, ds in SWEEPS[name]["points"]}) > 1
+            r = run_sweep_shared(sa) if shared else run_sweep(sa)
             c = r["config"]
             out["sweeps"][name] = {
                 "workload": c["workload"], "sectors": c["sectors"], "points_timed": c["points_timed"],

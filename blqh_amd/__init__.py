@@ -94,6 +94,9 @@ _SIGS = {
     "blqh_batch_lanczos": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.c_void_p, C.c_void_p,
                                      C.POINTER(LanczosResult), C.c_void_p]),
     "blqh_batch_rebind": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32]),
+    "blqh_batch_create_scaled": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32,
+                                           C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_void_p)]),
+    "blqh_batch_rebind_scaled": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32]),
     "blqh_batch_get_lanes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "blqh_batch_set_lanes": (C.c_int, [C.c_void_p, C.c_int32]),
     "blqh_batch_destroy": (None, [C.c_void_p]),
@@ -432,6 +435,39 @@ class Batch:
         h = C.c_void_p()
         _check(self._lib.blqh_batch_create(arr, len(self.hams), lanes, C.byref(h)))
         self.h = h
+
+    @classmethod
+    def scaled(cls, bases, tuns, scales):
+        """Shared-scale batch (blqh_batch_create_scaled): problem q * len(scales) + g
+        is bases[q] + scales[g] tuns[q] (bases built with terms=1, tuns with terms=2).
+        The batch keeps its own copy of the matrices; ``hams`` lists, per
+        problem, the sector's base (its size and basis)."""
+        bases, tuns, scales = list(bases), list(tuns), [float(x) for x in scales]
+        if len(bases) != len(tuns) or not bases:
+            raise ValueError(f"{len(bases)} base and {len(tuns)} tunnelling matrices")
+        if not 1 <= len(scales) <= 4:
+            raise ValueError(f"{len(scales)} scales (1 to 4)")
+        self = cls.__new__(cls)
+        self._lib = lib()
+        a = (C.c_void_p * len(bases))(*[H.h.value for H in bases])
+        t = (C.c_void_p * len(tuns))(*[H.h.value for H in tuns])
+        sc = (C.c_double * len(scales))(*scales)
+        h = C.c_void_p()
+        _check(self._lib.blqh_batch_create_scaled(a, t, len(bases), sc, len(scales), C.byref(h)))
+        self.h = h
+        self.scales = scales
+        self.hams = [H for H in bases for _ in scales]
+        return self
+
+    def rebind_scaled(self, bases, tuns):
+        """The next d of a shared-scale batch (same sectors, same order)."""
+        bases, tuns = list(bases), list(tuns)
+        if len(bases) != len(tuns):
+            raise ValueError(f"{len(bases)} base and {len(tuns)} tunnelling matrices")
+        a = (C.c_void_p * len(bases))(*[H.h.value for H in bases])
+        t = (C.c_void_p * len(tuns))(*[H.h.value for H in tuns])
+        _check(self._lib.blqh_batch_rebind_scaled(self.h, a, t, len(bases)))
+        self.hams = [H for H in bases for _ in self.scales]
 
     def rebind(self, hams):
         """Members of the same sizes, same order (next point of a sweep)."""

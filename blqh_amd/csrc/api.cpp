@@ -21,6 +21,10 @@ void blqh_lanczos_noalloc_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_l
 blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count, int lanes);
 void blqh_batch_destroy_impl(blqh_batch* b);
 void blqh_batch_rebind_impl(blqh_batch* b, blqh_hamil* const* hs, int count);
+blqh_batch* blqh_batch_create_scaled_impl(const blqh_hamil* const* base, const blqh_hamil* const* tun, int nsec,
+                                          const double* scales, int nscale);
+void blqh_batch_rebind_scaled_impl(blqh_batch* b, const blqh_hamil* const* base, const blqh_hamil* const* tun,
+                                   int nsec);
 int blqh_batch_lanes_impl(const blqh_batch* b);
 void blqh_batch_set_lanes_impl(blqh_batch* b, int lanes);
 void blqh_lanczos_reconstruct_impl(blqh_hamil* H, int m, uint32_t seed, const double* start, const double* alpha,
@@ -403,6 +407,19 @@ int blqh_batch_create(blqh_hamil* const* hs, int32_t count, int32_t lanes, blqh_
 int blqh_batch_rebind(blqh_batch* b, blqh_hamil* const* hs, int32_t count) {
     if (!b || !hs) return fail(BLQH_EINVAL, "null argument");
     return guarded([&] { blqh_batch_rebind_impl(b, hs, count); });
+}
+
+int blqh_batch_create_scaled(const blqh_hamil* const* base, const blqh_hamil* const* tun, int32_t nsector,
+                             const double* scales, int32_t nscale, blqh_batch** out) {
+    if (!base || !tun || !scales || !out || nsector < 1) return fail(BLQH_EINVAL, "bad argument");
+    *out = nullptr;
+    return guarded([&] { *out = blqh_batch_create_scaled_impl(base, tun, nsector, scales, nscale); });
+}
+
+int blqh_batch_rebind_scaled(blqh_batch* b, const blqh_hamil* const* base, const blqh_hamil* const* tun,
+                             int32_t nsector) {
+    if (!b || !base || !tun) return fail(BLQH_EINVAL, "null argument");
+    return guarded([&] { blqh_batch_rebind_scaled_impl(b, base, tun, nsector); });
 }
 
 int blqh_batch_lanczos(blqh_batch* b, const blqh_lanczos_opts* o, const uint32_t* seeds, const double* const* starts,

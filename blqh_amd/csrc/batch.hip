@@ -35,6 +35,7 @@ namespace blqh {
 void start_vector(int64_t n, uint32_t seed, std::vector<double2>& v);
 double tridiag_lowest(int m, const double* a, const double* b, double* vec);
 }  // namespace blqh
+void blqh_hamil_combine_impl(const blqh_hamil* base, const blqh_hamil* tun, double dsas, blqh_hamil* H);
 
 namespace {
 
@@ -275,6 +276,224 @@ __global__ void __launch_bounds__(kBTj) k_b1j(BState S, int jrel, const double2*
     }
 }
 
+// ---- shared-scale groups ----------------------------------------------------
+// G problems of one sector whose Hamiltonians are H0 + s_g T (the Delta_SAS
+// points of one d) share one panel layout of H0 + T: the entries of T (those
+// between words of different upper-layer occupation) carry bit 15 of their
+// column (a sector of at most 32,768 states, one panel), and problem g scales
+// them by s_g.  One pass over the entries serves all G problems: the column
+// and value of an entry are loaded once and gathered against G vectors.
+constexpr uint16_t kTunBit = 0x8000;
+
+struct SGroups {
+    const int32_t* gitem_grp;  // group work item -> group
+    const int64_t* gitem0;     // [ngroups + 1] first group item
+    const double* tscale;      // [P] scale of the flagged entries, problem p
+};
+
+// jag_row_sum_part for G vectors: lane sums of row `lane` of a 64-row slice
+// over the steps of its wave (see jag_row_sum_part), entry values scaled by
+// ts[g] where the column carries kTunBit.  act[g] == false skips vector g.
+template <bool REAL, int NP, int G>
+__device__ __forceinline__ void jag_row_sum_shared(const uint16_t* __restrict__ cr, const double* __restrict__ vr,
+                                                   const double2* __restrict__ vc, const double2* const* xp,
+                                                   const double* ts, const bool* act, uint32_t len, int par,
+                                                   double* ar, double* ai) {
+    typedef double d2v_t __attribute__((ext_vector_type(2)));
+    uint32_t off = 0;
+    auto issue = [&](uint32_t t, int* c, double2* v, bool& any) {
+        uint64_t m[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) m[u] = __ballot(t + u < len);
+        any = m[0] != 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t q = off + lanes_below(m[u]);
+            off += (uint32_t)__popcll(m[u]);
+            const bool a = t + u < len;
+            c[u] = a ? (int)__builtin_nontemporal_load(&cr[q]) : 0;
+            if (REAL) {
+                v[u] = make_double2(a ? __builtin_nontemporal_load(&vr[q]) : 0.0, 0.0);
+            } else if (a) {
+                const d2v_t w = __builtin_nontemporal_load(reinterpret_cast<const d2v_t*>(&vc[q]));
+                v[u] = make_double2(w.x, w.y);
+            } else {
+                v[u] = make_double2(0.0, 0.0);
+            }
+        }
+    };
+    auto skip = [&](uint32_t t) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) off += (uint32_t)__popcll(__ballot(t + u < len));
+    };
+    int c0[4], c1[4];
+    double2 v0[4], v1[4];
+    bool any0, any1;
+    for (int g = 0; g < par; ++g) skip(4u * g);
+    issue(4u * par, c0, v0, any0);
+    for (uint32_t t = 4u * par + 4u * NP; any0; t += 4u * NP) {
+#pragma unroll
+        for (int g = NP - 1; g > 0; --g) skip(t - 4u * g);
+        issue(t, c1, v1, any1);
+        double2 xv[G][4];
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (act[g]) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) xv[g][u] = xp[g][c0[u] & (kTunBit - 1)];
+            }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (act[g]) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double sc = (c0[u] & kTunBit) ? ts[g] : 1.0;
+                    const double wr = sc * v0[u].x;
+                    if (REAL) {
+                        ar[g] += wr * xv[g][u].x;
+                        ai[g] += wr * xv[g][u].y;
+                    } else {
+                        const double wi = sc * v0[u].y;
+                        ar[g] += wr * xv[g][u].x - wi * xv[g][u].y;
+                        ai[g] += wr * xv[g][u].y + wi * xv[g][u].x;
+                    }
+                }
+            }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c0[u] = c1[u];
+            v0[u] = v1[u];
+        }
+        any0 = any1;
+    }
+}
+
+// k_b1j for shared-scale groups: the work lists hold group items (256 rows
+// of a sector); problems q G .. q G + G - 1 of group q share the sector's
+// panels and own their vectors, scalars and partial sums (problem p's item
+// li is item0 + li, as in k_b2 / k_b3).
+template <bool LZ, int KIND, int G>
+__global__ void __launch_bounds__(kBTj) k_b1s(BState S, SGroups Q, int jrel, const double2* __restrict__ xs,
+                                              double2* __restrict__ ys, double* __restrict__ part) {
+    __shared__ double sh[kBTj / 64];
+    __shared__ double2 red[kItemRows / 64][kSplit - 1][G][64];
+    const int j = LZ ? *S.jbase + jrel : 0;
+    if (LZ && j >= S.cap) return;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int sl = wv / kSplit, par = wv % kSplit;
+    const int32_t* seg = S.xseg + KIND * 9 + (blockIdx.x & 7);
+    const int32_t kend = seg[1];
+    for (int32_t k = seg[0] + (int32_t)(blockIdx.x >> 3); k < kend; k += (int32_t)(gridDim.x >> 3)) {
+        const int64_t it = S.xitems[k];
+        const int q = Q.gitem_grp[it];
+        const int64_t li = it - Q.gitem0[q];
+        bool act[G];
+        bool anyact = false;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            act[g] = S.active[q * G + g] != 0;
+            anyact = anyact || act[g];
+        }
+        if (!anyact) continue;  // uniform over the workgroup
+        const BProb P0 = S.probs[q * G];
+        const int64_t n = P0.n;
+        const double2* x[G];
+        double2* y[G];
+        const double2* Wp[G];
+        double ib[G], cp[G], ts[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const BProb& Pg = S.probs[q * G + g];
+            ts[g] = Q.tscale[q * G + g];
+            ib[g] = 1.0;
+            cp[g] = 0.0;
+            if (LZ) {
+                const int64_t ld = S.cap + 1;
+                const int p = q * G + g;
+                ib[g] = S.invb[p * ld + j];
+                cp[g] = j > 0 ? S.beta[p * ld + j] * S.invb[p * ld + j - 1] : 0.0;
+                x[g] = Pg.W + (int64_t)j * n;
+                Wp[g] = j > 0 ? x[g] - n : x[g];
+                y[g] = S.u + Pg.voff;
+            } else {
+                x[g] = xs + Pg.voff;
+                Wp[g] = x[g];
+                y[g] = ys + Pg.voff;
+            }
+        }
+        const int64_t row0 = li * kItemRows + sl * 64, myrow = row0 + lane;
+        double ar[G], ai[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) ar[g] = ai[g] = 0.0;
+        if (row0 < n) {  // uniform over the wavefront; one panel (n <= 32768)
+            const int64_t rend = row0 + 64 < n ? row0 + 64 : n;
+            const uint32_t s = myrow < n ? P0.prow[myrow] : P0.prow[rend];
+            const uint32_t e = myrow < n ? P0.prow[myrow + 1] : P0.prow[rend];
+            const int64_t run = P0.pbase[0] + (uint32_t)__builtin_amdgcn_readfirstlane((int)s);
+            if (KIND == 2 ? P0.real != 0 : KIND == 0)
+                jag_row_sum_shared<true, kSplit, G>(P0.pcol + run, P0.pvr + run, P0.pvc + run, x, ts, act, e - s,
+                                                    par, ar, ai);
+            else
+                jag_row_sum_shared<false, kSplit, G>(P0.pcol + run, P0.pvr + run, P0.pvc + run, x, ts, act, e - s,
+                                                     par, ar, ai);
+        }
+        if (par) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) red[sl][par - 1][g][lane] = make_double2(ar[g], ai[g]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (!act[g]) continue;  // uniform over the workgroup
+            double acc = 0.0;
+            if (!par && myrow < n) {
+                double sr = ar[g], si = ai[g];
+#pragma unroll
+                for (int w = 0; w < kSplit - 1; ++w) {
+                    const double2 o = red[sl][w][g][lane];
+                    sr += o.x;
+                    si += o.y;
+                }
+                double ur = sr, ui = si;
+                if (LZ) {
+                    ur = ib[g] * sr;
+                    ui = ib[g] * si;
+                    if (j > 0) {
+                        const double2 qv = Wp[g][myrow];
+                        ur -= cp[g] * qv.x;
+                        ui -= cp[g] * qv.y;
+                    }
+                }
+                y[g][myrow] = make_double2(ur, ui);
+                const double2 w = x[g][myrow];
+                acc = w.x * ur + w.y * ui;
+            }
+            const double t = block_sum<kBTj>(acc, sh);
+            if (threadIdx.x == 0) part[S.probs[q * G + g].item0 + li] = t;
+        }
+    }
+}
+
+// T-entry flags of a one-panel sector in row-major panel order: bit 15 of
+// the column of every entry between words of different upper-layer
+// occupation (nu, internal row order); the flagged entries are counted.
+__global__ void k_mark_tun(int64_t n, const int64_t* __restrict__ pbase, const uint32_t* __restrict__ prow,
+                           uint16_t* __restrict__ pcol, const uint8_t* __restrict__ nu, int32_t* __restrict__ cnt) {
+    int32_t c = 0;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = pbase[0];
+        for (uint32_t k = prow[r]; k < prow[r + 1]; ++k) {
+            const uint16_t col = pcol[b + k];
+            if (nu[col] != nu[r]) {
+                pcol[b + k] = (uint16_t)(col | kTunBit);
+                ++c;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, c);
+}
+
 // K2: alpha_j, W_{j+1} = u - alpha_j W_j / beta_j, |W_{j+1}|^2 partials
 __global__ void __launch_bounds__(kBT) k_b2(BState S, int jrel) {
     __shared__ double sh[kBT / 64];
@@ -405,19 +624,43 @@ struct blqh_batch {
     int g_chk = -1, g_lpr = -1, g_cap = -1, g_kinds = -1;
     char* pin = nullptr;
     DevBuf<char> relayout;       // scratch for reordering members' panels
+    // Shared-scale groups (blqh_batch_create_scaled); G = 0: every member
+    // has its own matrix.  Problem p = s G + g is H0[s] + scale[g] T[s];
+    // hs[p] is the sector's template tpl[s] (owned, flagged T entries).
+    int G = 0;
+    std::vector<blqh_hamil*> tpl;
+    std::vector<double> tscale;   // [P]
+    std::vector<int64_t> gitem0;  // [nsector + 1] group work items
+    int64_t ngitems = 0;
+    DevBuf<int32_t> gitem_grp;
+    DevBuf<int64_t> dgitem0;
+    DevBuf<double> dtscale;
     ~blqh_batch() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         if (pin) (void)hipHostFree(pin);
+        for (blqh_hamil* H : tpl) delete H;
     }
 };
 
 namespace {
 
-// Planner view of the members (all, or those with live[p] set).
+// Planner view of the members (all, or those with live[p] set); with
+// shared-scale groups, of the groups (live when one of their problems is).
 std::vector<XcdMember> xcd_members(const blqh_batch* b, const std::vector<char>* live = nullptr) {
-    const int P = (int)b->hs.size();
     std::vector<XcdMember> mem;
+    if (b->G) {
+        const int ns = (int)b->tpl.size();
+        mem.reserve(ns);
+        for (int q = 0; q < ns; ++q) {
+            bool on = !live;
+            for (int g = 0; g < b->G && !on; ++g) on = (*live)[q * b->G + g] != 0;
+            if (on)
+                mem.push_back({b->gitem0[q], b->gitem0[q + 1], b->tpl[q]->nnz * b->G, 1, b->tpl[q]->pan.real ? 1 : 0});
+        }
+        return mem;
+    }
+    const int P = (int)b->hs.size();
     mem.reserve(P);
     for (int p = 0; p < P; ++p)
         if (!live || (*live)[p])
@@ -447,7 +690,11 @@ void refresh_probs(blqh_batch* b) {
     for (blqh_hamil* H : b->hs) (H->pan.real ? b->has_real : b->has_complex) = true;
     // k_b1j reads step-major runs, k_b1 row-major ones: one relayout launch for all members
     // plain step order: a rebind re-lays out every member at each parameter point
-    set_panel_layouts(b->hs.data(), (int)b->hs.size(), b->lpr == 64, b->relayout.get(), b->relayout.bytes(), false);
+    if (b->G)  // the templates, once each
+        set_panel_layouts(b->tpl.data(), (int)b->tpl.size(), true, b->relayout.get(), b->relayout.bytes(), false);
+    else
+        set_panel_layouts(b->hs.data(), (int)b->hs.size(), b->lpr == 64, b->relayout.get(), b->relayout.bytes(),
+                          false);
     build_xcd_lists(b);
     for (size_t p = 0; p < b->hs.size(); ++p) {
         blqh_hamil* H = b->hs[p];
@@ -460,10 +707,37 @@ void refresh_probs(blqh_batch* b) {
     }
 }
 
+SGroups make_groups(const blqh_batch* b) {
+    return SGroups{b->gitem_grp.get(), b->dgitem0.get(), b->dtscale.get()};
+}
+
+template <bool LZ, int KIND>
+void launch_b1s_kind(const blqh_batch* b, const BState& S, int jrel, const double2* xs, double2* ys, double* part,
+                     hipStream_t st) {
+    const dim3 g(b->grid), tj(kBTj);
+    const SGroups Q = make_groups(b);
+    switch (b->G) {
+        case 1: hipLaunchKernelGGL((k_b1s<LZ, KIND, 1>), g, tj, 0, st, S, Q, jrel, xs, ys, part); break;
+        case 2: hipLaunchKernelGGL((k_b1s<LZ, KIND, 2>), g, tj, 0, st, S, Q, jrel, xs, ys, part); break;
+        case 3: hipLaunchKernelGGL((k_b1s<LZ, KIND, 3>), g, tj, 0, st, S, Q, jrel, xs, ys, part); break;
+        case 4: hipLaunchKernelGGL((k_b1s<LZ, KIND, 4>), g, tj, 0, st, S, Q, jrel, xs, ys, part); break;
+        default: throw Error(BLQH_EINTERNAL, "unsupported group size");
+    }
+}
+
 template <bool LZ>
 void launch_b1(const blqh_batch* b, const BState& S, int jrel, const double2* xs, double2* ys, double* part,
                hipStream_t st) {
     const dim3 g(b->grid), t(kBT), tj(kBTj);
+    if (b->G) {  // shared-scale groups: one pass over each sector's entries for all its problems
+        if (b->has_real && b->has_complex) {
+            launch_b1s_kind<LZ, 2>(b, S, jrel, xs, ys, part, st);
+        } else {
+            if (b->has_real) launch_b1s_kind<LZ, 0>(b, S, jrel, xs, ys, part, st);
+            if (b->has_complex) launch_b1s_kind<LZ, 1>(b, S, jrel, xs, ys, part, st);
+        }
+        return;
+    }
     switch (b->lpr) {
         case 64:
             if (b->has_real && b->has_complex) {  // measured faster than one launch per kind
@@ -592,6 +866,7 @@ blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count, int lanes) 
 // New members of the same shapes (a sweep's next parameter point): the work
 // items, pools, captured replay and lane choice are kept.
 void blqh_batch_rebind_impl(blqh_batch* b, blqh_hamil* const* hs, int count) {
+    BLQH_REQUIRE(b->G == 0, BLQH_EINVAL, "a shared-scale batch is rebound with blqh_batch_rebind_scaled");
     BLQH_REQUIRE(count == (int)b->hs.size(), BLQH_EINVAL, "rebind needs as many Hamiltonians as the batch has");
     int64_t nnz = 0;
     for (int p = 0; p < count; ++p) {
@@ -608,6 +883,147 @@ void blqh_batch_rebind_impl(blqh_batch* b, blqh_hamil* const* hs, int count) {
     refresh_probs(b);
 }
 
+namespace {
+// H0 + 1 T of one sector (blqh_hamil_combine) with bit 15 set on the column
+// of every T entry, in row-major panel order.  The tunnelling terms are the
+// only ones that move an electron between the layers, so T's entries are
+// exactly those between words of different upper-layer occupation; their
+// count must equal T's.
+blqh_hamil* make_template(const blqh_hamil* base, const blqh_hamil* tun) {
+    BLQH_REQUIRE(base != nullptr && tun != nullptr, BLQH_EINVAL, "null hamiltonian in batch");
+    BLQH_REQUIRE(base->n <= (int64_t)kTunBit, BLQH_EINVAL,
+                 "a shared-scale batch takes sectors of at most 32768 states");
+    auto* H = new blqh_hamil;
+    try {
+        blqh_hamil_combine_impl(base, tun, 1.0, H);
+        BLQH_REQUIRE(H->pan.built && H->pan.npan == 1, BLQH_EINTERNAL, "template is not a one-panel matrix");
+        set_panel_layout(H, false);  // row-major: the marking kernel walks rows
+        const blqh_basis* B = H->basis;
+        const int64_t n = H->n;
+        const uint64_t upper = lowbits(B->geo.S * B->geo.nphi / 2);
+        std::vector<uint8_t> nu(n);
+        for (int64_t r = 0; r < n; ++r) {
+            const int64_t i = H->ord.on ? H->ord.h_iperm[r] : r;
+            nu[r] = (uint8_t)popc(B->h_id[i] & upper);
+        }
+        hipStream_t st = stream();
+        DevBuf<uint8_t> dnu(n);
+        DevBuf<int32_t> cnt(1);
+        BLQH_HIP(hipMemcpyAsync(dnu.get(), nu.data(), n, hipMemcpyHostToDevice, st));
+        BLQH_HIP(hipMemsetAsync(cnt.get(), 0, 4, st));
+        const int g = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(n, 256), 1), 4096);
+        hipLaunchKernelGGL(k_mark_tun, dim3(g), dim3(256), 0, st, n, H->pan.pbase.get(), H->pan.prow.get(),
+                           H->pan.pcol.get(), dnu.get(), cnt.get());
+        BLQH_HIP(hipGetLastError());
+        int32_t hc = 0;
+        copy_sync(&hc, cnt.get(), 4, hipMemcpyDeviceToHost);
+        BLQH_REQUIRE(hc == tun->nnz, BLQH_EINTERNAL, "the tunnelling entries are not the layer-changing entries");
+    } catch (...) {
+        delete H;
+        throw;
+    }
+    return H;
+}
+}  // namespace
+
+blqh_batch* blqh_batch_create_scaled_impl(const blqh_hamil* const* base, const blqh_hamil* const* tun, int nsec,
+                                          const double* scales, int nscale) {
+    BLQH_REQUIRE(nsec >= 1, BLQH_EINVAL, "empty batch");
+    BLQH_REQUIRE(nscale >= 1 && nscale <= 4, BLQH_EINVAL, "a shared-scale batch takes 1 to 4 scales");
+    for (int g = 0; g < nscale; ++g)
+        BLQH_REQUIRE(std::isfinite(scales[g]) && scales[g] >= 0.0, BLQH_EINVAL, "scales must be finite and >= 0");
+    auto* b = new blqh_batch;
+    try {
+        const int G = nscale;
+        b->G = G;
+        for (int q = 0; q < nsec; ++q) b->tpl.push_back(make_template(base[q], tun[q]));
+        const int P = nsec * G;
+        b->hs.resize(P);
+        b->item0.assign(P + 1, 0);
+        b->gitem0.assign(nsec + 1, 0);
+        b->tscale.resize(P);
+        std::vector<int32_t> ip, gp;
+        for (int q = 0; q < nsec; ++q) {
+            blqh_hamil* H = b->tpl[q];
+            const int64_t nit = ceil_div(H->n, kItemRows);
+            b->gitem0[q + 1] = b->gitem0[q] + nit;
+            for (int64_t i = 0; i < nit; ++i) gp.push_back(q);
+            b->nnz += H->nnz;
+            for (int g = 0; g < G; ++g) {
+                const int p = q * G + g;
+                b->hs[p] = H;
+                b->tscale[p] = scales[g];
+                BProb d{};
+                d.n = H->n;
+                d.npan = 1;
+                d.real = H->pan.real ? 1 : 0;
+                d.item0 = b->nitems;
+                d.voff = b->rows;
+                for (int64_t i = 0; i < nit; ++i) ip.push_back(p);
+                b->nitems += nit;
+                b->rows += H->n;
+                b->item0[p + 1] = b->nitems;
+                b->h_probs.push_back(d);
+            }
+        }
+        b->ngitems = b->gitem0[nsec];
+        BLQH_REQUIRE(b->nitems < (int64_t)1 << 31, BLQH_ERANGE, "batch too large");
+        b->item_prob.alloc(ip.size());
+        copy_sync(b->item_prob.get(), ip.data(), ip.size() * 4, hipMemcpyHostToDevice);
+        b->gitem_grp.alloc(gp.size());
+        copy_sync(b->gitem_grp.get(), gp.data(), gp.size() * 4, hipMemcpyHostToDevice);
+        b->dgitem0.alloc(nsec + 1);
+        copy_sync(b->dgitem0.get(), b->gitem0.data(), (nsec + 1) * 8, hipMemcpyHostToDevice);
+        b->dtscale.alloc(P);
+        copy_sync(b->dtscale.get(), b->tscale.data(), P * 8, hipMemcpyHostToDevice);
+        b->grid = (int)(8 * ceil_div(std::min<int64_t>(b->nitems, (int64_t)runtime().num_cus * 8), 8));
+        b->xitems.alloc(2 * b->nitems);
+        b->xseg.alloc(27);
+        b->relayout.alloc(panel_layout_scratch_bytes(b->tpl.data(), nsec));
+        b->lpr = 64;
+        refresh_probs(b);
+    } catch (...) {
+        delete b;
+        throw;
+    }
+    return b;
+}
+
+// The next parameter point of a shared-scale batch: new H0 and T of the same
+// sectors (same sizes, same order); the scales stay.
+void blqh_batch_rebind_scaled_impl(blqh_batch* b, const blqh_hamil* const* base, const blqh_hamil* const* tun,
+                                   int nsec) {
+    BLQH_REQUIRE(b->G > 0, BLQH_EINVAL, "not a shared-scale batch");
+    BLQH_REQUIRE(nsec == (int)b->tpl.size(), BLQH_EINVAL, "rebind needs as many sectors as the batch has");
+    for (int q = 0; q < nsec; ++q)
+        BLQH_REQUIRE(base[q] != nullptr && base[q]->n == b->tpl[q]->n, BLQH_EINVAL,
+                     "rebind needs sectors of the same sizes, in the same order");
+    std::vector<blqh_hamil*> fresh;
+    try {
+        for (int q = 0; q < nsec; ++q) fresh.push_back(make_template(base[q], tun[q]));
+    } catch (...) {
+        for (blqh_hamil* H : fresh) delete H;
+        throw;
+    }
+    BLQH_HIP(hipStreamSynchronize(stream()));  // no queued work reads the old templates
+    for (blqh_hamil* H : b->tpl) delete H;
+    b->tpl.swap(fresh);
+    b->nnz = 0;
+    for (int q = 0; q < nsec; ++q) {
+        b->nnz += b->tpl[q]->nnz;
+        for (int g = 0; g < b->G; ++g) {
+            const int p = q * b->G + g;
+            b->hs[p] = b->tpl[q];
+            b->h_probs[p].real = b->tpl[q]->pan.real ? 1 : 0;
+        }
+    }
+    const size_t scratch = panel_layout_scratch_bytes(b->tpl.data(), nsec);
+    if (scratch > b->relayout.bytes()) b->relayout.alloc(scratch);
+    refresh_probs(b);
+}
+
+int blqh_batch_group_size_impl(const blqh_batch* b) { return b->G; }
+
 void blqh_batch_destroy_impl(blqh_batch* b) { delete b; }
 
 int blqh_batch_lanes_impl(const blqh_batch* b) { return b->lpr; }
@@ -615,6 +1031,7 @@ int blqh_batch_lanes_impl(const blqh_batch* b) { return b->lpr; }
 void blqh_batch_set_lanes_impl(blqh_batch* b, int lanes) {
     BLQH_REQUIRE(lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64, BLQH_EINVAL,
                  "unsupported lanes per row");
+    BLQH_REQUIRE(b->G == 0 || lanes == 64, BLQH_EINVAL, "a shared-scale batch runs one lane per row");
     b->lpr = lanes;
 }
 
@@ -972,7 +1389,9 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
         double erq = 0;
         for (int64_t i = b->item0[p]; i < b->item0[p + 1]; ++i) erq += hp2[i];
         // bytes of every K1 launch this problem took part in (speculative steps included)
-        if (timed) prof.bytes += (double)steps_run[p] * (20.0 * b->hs[p]->nnz + 56.0 * hp[p].n + 8.0);
+        // (a shared-scale group reads its matrix once for its G problems)
+        if (timed)
+            prof.bytes += (double)steps_run[p] * (20.0 * b->hs[p]->nnz / (b->G ? b->G : 1) + 56.0 * hp[p].n + 8.0);
         blqh_lanczos_result& r = res[p];
         std::memset(&r, 0, sizeof(r));
         r.E0 = E0[p];

@@ -967,6 +967,16 @@ void launch_lz1(const blqh_hamil* H, const Plan& pl, const double2* W, const dou
     }
 }
 
+// Shape of a tuned matrix (the plan cache's key).
+struct TuneKey {
+    int npan, real, lg_n, lg_row;  // log2 of rows and of the mean row length, in quarters
+    bool operator==(const TuneKey& o) const {
+        return npan == o.npan && real == o.real && lg_n == o.lg_n && lg_row == o.lg_row;
+    }
+};
+std::mutex g_tune_mu;
+std::vector<std::pair<TuneKey, Plan>> g_tune_cache;  // most recent last, at most 64
+
 // Autotuner: time every variant on this matrix (random x) and keep the fastest.
 void tune(blqh_hamil* H) {
     // BLQH_SPMV_PLAN="kind,lanes,threads,wgs_per_cu,rows" pins a variant (profiling, reproducibility)
@@ -982,17 +992,42 @@ void tune(blqh_hamil* H) {
             return;
         }
     }
-    hipStream_t st = stream();
+    // A matrix of the shape of one tuned before (same panels and value kind,
+    // rows and mean row length within 19 %) takes that plan without timing:
+    // the sectors of a sweep tune once.  BLQH_TUNE_CACHE=0 times every matrix.
     const double mean_row = H->n ? (double)H->nnz / H->n : 0;
+    const char* cenv = std::getenv("BLQH_TUNE_CACHE");
+    const bool use_cache = !(cenv && cenv[0] == '0');
+    const TuneKey key{H->pan.built ? H->pan.npan : 0, H->pan.real ? 1 : 0,
+                      (int)std::lround(4.0 * std::log2((double)std::max<int64_t>(H->n, 1))),
+                      (int)std::lround(4.0 * std::log2(std::max(mean_row, 1.0)))};
+    if (use_cache) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (const auto& e : g_tune_cache)
+            if (e.first == key && (e.second.kind == 0 || H->pan.built)) {
+                const Plan& p = e.second;
+                H->spmv_kind = p.kind;
+                H->spmv_lpr = p.lpr;
+                H->spmv_threads = p.threads;
+                H->spmv_wgs_per_cu = p.wpc;
+                H->spmv_rows = p.rows;
+                return;
+            }
+    }
+    hipStream_t st = stream();
     std::vector<Plan> cands;
     for (int lpr : {4, 8, 16, 32, 64})
         if (!(lpr > 4 * mean_row + 8 && lpr > 8)) cands.push_back(Plan{0, lpr, kThreads, 0, 0});
-    if (H->pan.built)
+    // Kinds 1, 2, 4 and 5 never won on any measured matrix (79 to 2.35M rows);
+    // BLQH_TUNE_ALL=1 times them too.
+    const char* aenv = std::getenv("BLQH_TUNE_ALL");
+    const bool all_kinds = aenv && aenv[0] == '1';
+    if (H->pan.built && all_kinds)
         for (int pt : {512, 1024})
             for (int lpr : {4, 8, 16})
                 for (int wpc : {1, 2})
                     for (int u : {2, 4}) cands.push_back(Plan{1, lpr, pt, wpc, u});
-    if (H->pan.built)
+    if (H->pan.built && all_kinds)
         for (int pt : {512, 1024})
             for (int wpc : {1, 2, 4}) {
                 // slices per wavefront: the smallest supported count that covers the matrix in one round
@@ -1003,7 +1038,7 @@ void tune(blqh_hamil* H) {
                 cands.push_back(Plan{2, 64, pt, wpc, ms});
             }
     // column tiles: an x tile per LDS, worth it when x does not fit the caches
-    if (H->pan.built && H->n > 4 * kTileCols && build_tiles(H, runtime().num_cus))
+    if (H->pan.built && all_kinds && H->n > 4 * kTileCols && build_tiles(H, runtime().num_cus))
         cands.push_back(Plan{4, 64, kT4Threads, 1, H->til.R});
     if (H->pan.built)  // jagged kernel last: one layout conversion
         for (int pt : {256, 512, 1024})
@@ -1014,7 +1049,7 @@ void tune(blqh_hamil* H) {
                 while (ms < need && ms < 8) ms <<= 1;
                 cands.push_back(Plan{3, 64, pt, wpc, ms});
             }
-    if (H->pan.built)  // R rows per lane: one layout conversion per R
+    if (H->pan.built && all_kinds)  // R rows per lane: one layout conversion per R
         for (int R : {2, 4, 8})
             for (int pt : {512, 1024})
                 for (int wpc : {1, 2}) {
@@ -1069,6 +1104,11 @@ void tune(blqh_hamil* H) {
     H->spmv_threads = bp.threads;
     H->spmv_wgs_per_cu = bp.wpc;
     H->spmv_rows = bp.rows;
+    if (use_cache) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        if (g_tune_cache.size() >= 64) g_tune_cache.erase(g_tune_cache.begin());
+        g_tune_cache.push_back({key, bp});
+    }
 }
 
 // Tunes on first use (unless a variant is pinned) and puts the panels in the

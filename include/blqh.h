@@ -238,6 +238,21 @@ int blqh_batch_lanczos(blqh_batch* b, const blqh_lanczos_opts* o, const uint32_t
  * 64: one lane per row over step-major panels; autotuned at create). */
 int blqh_batch_get_lanes(const blqh_batch* b, int32_t* lanes);
 int blqh_batch_set_lanes(blqh_batch* b, int32_t lanes);
+/* Shared-scale batch: the Delta_SAS points of one d, the inner loop of
+ * BLQH_without_spin/main.cpp:133-165.  For nsector sectors and nscale
+ * (1..4) values s_g, problem p = q * nscale + g is H0[q] + s_g T[q], with
+ * base[q] a build without tunnelling (blqh_hamil_build_terms terms = 1) and
+ * tun[q] the unit tunnelling matrix of the same sector (terms = 2), as for
+ * blqh_hamil_combine; sectors of at most 32768 states.  The batch keeps its
+ * own panel copy of H0 + T (base and tun may be destroyed afterwards) and
+ * reads each sector's entries once per Lanczos step for all of its problems.
+ * Results follow the problem order.  The entries that blqh_hamil_combine
+ * drops (|s_g t| <= 1e-10) contribute their values here. */
+int blqh_batch_create_scaled(const blqh_hamil* const* base, const blqh_hamil* const* tun, int32_t nsector,
+                             const double* scales, int32_t nscale, blqh_batch** out);
+/* The next d of a shared-scale batch: H0 and T of the same sectors, same order. */
+int blqh_batch_rebind_scaled(blqh_batch* b, const blqh_hamil* const* base, const blqh_hamil* const* tun,
+                             int32_t nsector);
 void blqh_batch_destroy(blqh_batch* b);
 
 /* ---- Learned warm start (no counterpart in the reference: it supplies the
