@@ -34,7 +34,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the fused SpMV from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # passes, scaled by the same counters on a plain 1 GiB copy (profiles/);
 # reported only when the live run uses the profiled workload and variant
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {
+    # k_jag<true, true, 8, 8>, internal row order, greedy steps: 6,544,086 KiB FETCH_SIZE x 2.0 + 36,721.5 KiB
+    # WRITE_SIZE x 1.0 per dispatch (profiles/k_jag_nphi16_counters.md)
+    ("nphi16_all", (3, 64, 512, 2, 8)): (6544086 * 2.0 * 1024 + 36721.5 * 1024,
+                                         "profiles/k_jag_nphi16_counters.md"),
+}
 # fused Lanczos K1 kernel of each SpMV variant (lanczos.hip)
 KERNEL_NAMES = {0: "k_lz1", 1: "k_pspmv<LZ=true>", 2: "k_sps<LZ=true>", 3: "k_jag<LZ=true>", 4: "k_t4<LZ=true>",
                 5: "k_jagr<LZ=true>"}

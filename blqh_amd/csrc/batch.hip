@@ -25,7 +25,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <thread>
 
 #include "engine.hpp"
@@ -34,6 +38,7 @@
 namespace blqh {
 void start_vector(int64_t n, uint32_t seed, std::vector<double2>& v);
 double tridiag_lowest(int m, const double* a, const double* b, double* vec);
+double tridiag_lowest_next(int m, const double* a, const double* b, double prev, double step);
 }  // namespace blqh
 void blqh_hamil_combine_impl(const blqh_hamil* base, const blqh_hamil* tun, double dsas, blqh_hamil* H);
 
@@ -335,27 +340,25 @@ __device__ __forceinline__ void jag_row_sum_shared(const uint16_t* __restrict__ 
 #pragma unroll
         for (int g = NP - 1; g > 0; --g) skip(t - 4u * g);
         issue(t, c1, v1, any1);
-        double2 xv[G][4];
+        // one vector at a time: four gathers in flight per vector keep the
+        // register count at 144 (all G x 4 at once: 162, and measured 9 % slower)
 #pragma unroll
         for (int g = 0; g < G; ++g)
             if (act[g]) {
+                double2 xv[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) xv[g][u] = xp[g][c0[u] & (kTunBit - 1)];
-            }
-#pragma unroll
-        for (int g = 0; g < G; ++g)
-            if (act[g]) {
+                for (int u = 0; u < 4; ++u) xv[u] = xp[g][c0[u] & (kTunBit - 1)];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const double sc = (c0[u] & kTunBit) ? ts[g] : 1.0;
                     const double wr = sc * v0[u].x;
                     if (REAL) {
-                        ar[g] += wr * xv[g][u].x;
-                        ai[g] += wr * xv[g][u].y;
+                        ar[g] += wr * xv[u].x;
+                        ai[g] += wr * xv[u].y;
                     } else {
                         const double wi = sc * v0[u].y;
-                        ar[g] += wr * xv[g][u].x - wi * xv[g][u].y;
-                        ai[g] += wr * xv[g][u].y + wi * xv[g][u].x;
+                        ar[g] += wr * xv[u].x - wi * xv[u].y;
+                        ai[g] += wr * xv[u].y + wi * xv[u].x;
                     }
                 }
             }
@@ -596,6 +599,66 @@ __global__ void __launch_bounds__(kBT) k_bscale(BState S, const double* __restri
 
 namespace {
 // Pooled per-problem scalars and vectors of a batched solve.
+// Host worker threads for the per-replay convergence tests, kept for the
+// batch's lifetime: starting 16 threads costs about 0.5 ms, as much as a
+// replay of two Lanczos steps.  run(nt, f) calls f(t) for t < nt, the caller
+// taking t = 0, and returns when every call has returned.
+class HostPool {
+public:
+    explicit HostPool(int workers) {
+        for (int w = 1; w <= workers; ++w) th_.emplace_back([this, w] { loop(w); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    int size() const { return (int)th_.size() + 1; }
+    void run(int nt, const std::function<void(int)>& f) {
+        nt = std::max(1, std::min(nt, size()));
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &f;
+            nt_ = nt;
+            left_ = nt - 1;
+            ++gen_;
+        }
+        if (nt > 1) cv_.notify_all();
+        f(0);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [this] { return left_ == 0; });
+        job_ = nullptr;
+    }
+
+private:
+    void loop(int w) {
+        int seen = 0;
+        while (true) {
+            const std::function<void(int)>* job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                if (w >= nt_) continue;
+                job = job_;
+            }
+            (*job)(w);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--left_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)>* job_ = nullptr;
+    int nt_ = 0, left_ = 0, gen_ = 0;
+    bool stop_ = false;
+};
+
 struct Pools {
     DevBuf<BProb> probs;
     DevBuf<int32_t> active, jbase, mvec;
@@ -635,6 +698,7 @@ struct blqh_batch {
     DevBuf<int32_t> gitem_grp;
     DevBuf<int64_t> dgitem0;
     DevBuf<double> dtscale;
+    std::unique_ptr<HostPool> hpool;  // convergence-test threads (batches of 32 or more problems)
     ~blqh_batch() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -1235,7 +1299,10 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     std::vector<double> dE(P, INFINITY), resid(P, INFINITY);
     // host threads for the convergence tests: worth it only for many problems
     const int nthr = P >= 32 ? (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())) : 1;
+    if (nthr > 1 && (!b->hpool || b->hpool->size() != nthr)) b->hpool.reset(new HostPool(nthr - 1));
     std::vector<std::vector<double>> svecs(nthr, std::vector<double>(cap));
+    std::vector<int> live_ids;
+    live_ids.reserve(P);
     // Replay r+1 is queued before the host tests replay r, so the host test
     // overlaps the GPU; a problem that converged in replay r runs one more
     // replay of steps that are never used.
@@ -1264,7 +1331,10 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
             const double* a = ha + (size_t)p * cap;
             const double* bb = hb + (size_t)p * ld;
             for (int j = done; j < upto; ++j) {
-                const double e = tridiag_lowest(j + 1, a, bb + 1, nullptr);
+                const std::vector<double>& rz = ritz[p];
+                const double e = j == 0 ? tridiag_lowest(1, a, bb + 1, nullptr)
+                                        : tridiag_lowest_next(j + 1, a, bb + 1, rz[j - 1],
+                                                              j > 1 ? std::fabs(rz[j - 1] - rz[j - 2]) : INFINITY);
                 ritz[p].push_back(e);
                 m[p] = j + 1;
                 if (j > 0) dE[p] = std::fabs(ritz[p][j] - ritz[p][j - 1]);
@@ -1283,18 +1353,17 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
             }
         };
         {
-            const int nt = std::min(nthr, nlive);
+            live_ids.clear();
+            for (int p = 0; p < P; ++p)
+                if (live[p]) live_ids.push_back(p);
+            const int nl = (int)live_ids.size();
+            const int nt = std::min(nthr, (nl + 3) / 4);  // at least four problems per thread
             if (nt <= 1) {
-                for (int p = 0; p < P; ++p)
-                    if (live[p]) test(p, svecs[0]);
+                for (int p : live_ids) test(p, svecs[0]);
             } else {
-                std::vector<std::thread> th;
-                for (int t = 0; t < nt; ++t)
-                    th.emplace_back([&, t] {
-                        for (int p = t; p < P; p += nt)
-                            if (live[p]) test(p, svecs[t]);
-                    });
-                for (auto& t : th) t.join();
+                b->hpool->run(nt, [&](int t) {
+                    for (int i = t; i < nl; i += nt) test(live_ids[i], svecs[t]);
+                });
             }
         }
         bool changed = false;
