@@ -41,12 +41,13 @@ __device__ __forceinline__ double2 ntload(const double2* p) {
 }
 
 // ---- deterministic grid reduction -------------------------------------------
-// Block partial -> partials[blockIdx]; the last block (ticket) sums them in
-// order and hands the total to `fin`.
-template <class Fin>
-__device__ __forceinline__ void grid_reduce(double v, double* partials, unsigned* ticket, Fin fin) {
+// Block sum of v -> partials[blockIdx].  The grid total is formed by the
+// next kernel on the stream (wave_sum, fixed order): no atomics or
+// device-scope fences, whose cost grows with the number of workgroups (a
+// ticket taken by each of 4,700 workgroups made the fused step of an
+// 18.8k-row matrix ten times slower than its product).
+__device__ __forceinline__ void block_partial(double v, double* partials) {
     __shared__ double sh[16];  // up to 1024 threads
-    __shared__ bool last;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -54,23 +55,38 @@ __device__ __forceinline__ void grid_reduce(double v, double* partials, unsigned
         double t = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
         partials[blockIdx.x] = t;
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double t = 0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) t += __builtin_nontemporal_load(&partials[b]);
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
+}
+
+// sum of part[0 .. cnt) by the whole workgroup in a fixed order, returned
+// to every thread: each thread adds its strided share with four independent
+// loads in flight (thousands of partials are one memory latency, not a chain
+// of them), then the wavefronts' butterflies and a fixed-order sum of the
+// wavefront totals.  Contains __syncthreads: call from all threads.
+__device__ __forceinline__ double block_total(const double* __restrict__ part, int cnt) {
+    __shared__ double sh[17];
+    const int nt = (int)blockDim.x;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int i = threadIdx.x;
+    for (; i + 3 * nt < cnt; i += 4 * nt) {
+        a0 += part[i];
+        a1 += part[i + nt];
+        a2 += part[i + 2 * nt];
+        a3 += part[i + 3 * nt];
+    }
+    for (; i < cnt; i += nt) a0 += part[i];
+    double v = (a0 + a1) + (a2 + a3);
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();  // sh may still be read by a previous call
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double s = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
-        fin(s);
-        *ticket = 0;
+        double t = 0;
+        for (int w = 0; w < nt / 64; ++w) t += sh[w];
+        sh[16] = t;
     }
+    __syncthreads();
+    return sh[16];
 }
 
 struct Scal {  // device scalars per iteration
@@ -118,8 +134,7 @@ template <int LPR>
 __global__ void __launch_bounds__(kThreads) k_lz1(int64_t n, const int64_t* __restrict__ rowptr,
                                                    const int32_t* __restrict__ col, const double2* __restrict__ val,
                                                    const double2* __restrict__ W, const double2* __restrict__ Wp,
-                                                   double2* __restrict__ u, Scal sc, int j, double* partials,
-                                                   unsigned* ticket) {
+                                                   double2* __restrict__ u, Scal sc, int j, double* partials) {
     constexpr int RPB = kThreads / LPR;
     const int lane = threadIdx.x % LPR;
     const double ib = sc.invb[j];
@@ -139,17 +154,20 @@ __global__ void __launch_bounds__(kThreads) k_lz1(int64_t n, const int64_t* __re
             acc += w.x * ur + w.y * ui;
         }
     }
-    grid_reduce(acc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
+    block_partial(acc, partials);
 }
 
-// K2: Wn = u - alpha[j] invb[j] W; beta[j+1] = |Wn|; for s < ns:
-// psi_s += coef[s*ldc + j] invb[j] conj(W)   (psi_s = psi + s*n)
+// K2: alpha[j] = invb[j] (sum of K1's npart1 partials); Wn = u - alpha[j]
+// invb[j] W; partials of |Wn|^2; for s < ns: psi_s += coef[s*ldc + j] invb[j]
+// conj(W)   (psi_s = psi + s*n)
 __global__ void __launch_bounds__(kThreads) k_lz2(int64_t n, const double2* __restrict__ u,
                                                    const double2* __restrict__ W, double2* __restrict__ Wn, Scal sc,
                                                    int j, int ns, double2* __restrict__ psi,
-                                                   const double* __restrict__ coef, int ldc, double* partials,
-                                                   unsigned* ticket) {
-    const double a = sc.alpha[j] * sc.invb[j];
+                                                   const double* __restrict__ coef, int ldc,
+                                                   const double* __restrict__ part1, int npart1, double* part2) {
+    const double alpha = block_total(part1, npart1) * sc.invb[j];
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc.alpha[j] = alpha;
+    const double a = alpha * sc.invb[j];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
         const double2 w = W[i], x = u[i];
@@ -164,11 +182,23 @@ __global__ void __launch_bounds__(kThreads) k_lz2(int64_t n, const double2* __re
             psi[(int64_t)s * n + i] = p;
         }
     }
-    grid_reduce(acc, partials, ticket, [&](double s) {
-        const double b = sqrt(s);
+    block_partial(acc, part2);
+}
+
+// K3: beta[j+1] = sqrt(sum of K2's partials), invb[j+1]; one workgroup
+__global__ void __launch_bounds__(kThreads) k_lz3(Scal sc, int j, const double* __restrict__ part2, int npart2) {
+    const double t = block_total(part2, npart2);
+    if (threadIdx.x == 0) {
+        const double b = sqrt(t);
         sc.beta[j + 1] = b;
         sc.invb[j + 1] = 1.0 / b;
-    });
+    }
+}
+
+// *out = sum of part[0 .. cnt); one workgroup
+__global__ void __launch_bounds__(kThreads) k_psum(const double* __restrict__ part, int cnt, double* out) {
+    const double t = block_total(part, cnt);
+    if (threadIdx.x == 0) *out = t;
 }
 
 // psi += coef[l] invb[l] conj(W_l) for one vector (last replayed vector)
@@ -196,14 +226,13 @@ __global__ void __launch_bounds__(kThreads) k_combine(int64_t n, int m, const do
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_norm2(int64_t n, const double2* __restrict__ x, double* out,
-                                                     double* partials, unsigned* ticket) {
+__global__ void __launch_bounds__(kThreads) k_norm2(int64_t n, const double2* __restrict__ x, double* partials) {
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
         const double2 v = x[i];
         acc += v.x * v.x + v.y * v.y;
     }
-    grid_reduce(acc, partials, ticket, [&](double s) { *out = s; });
+    block_partial(acc, partials);
 }
 
 __global__ void k_scale(int64_t n, double2* __restrict__ x, const double* __restrict__ nrm2) {
@@ -218,14 +247,13 @@ __global__ void k_scale(int64_t n, double2* __restrict__ x, const double* __rest
 
 // Re<x|y>
 __global__ void __launch_bounds__(kThreads) k_redot(int64_t n, const double2* __restrict__ x,
-                                                     const double2* __restrict__ y, double* out, double* partials,
-                                                     unsigned* ticket) {
+                                                     const double2* __restrict__ y, double* partials) {
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
         const double2 a = x[i], b = y[i];
         acc += a.x * b.x + a.y * b.y;
     }
-    grid_reduce(acc, partials, ticket, [&](double s) { *out = s; });
+    block_partial(acc, partials);
 }
 
 __global__ void __launch_bounds__(kThreads) k_copy(const double2* __restrict__ src, double2* __restrict__ dst,
@@ -262,7 +290,7 @@ __device__ __forceinline__ void seg_fma(const PanelView& P, const double2* __res
 template <int LPR, int PT, int U, bool REAL, bool LZ>
 __global__ void __launch_bounds__(PT) k_pspmv(PanelView P, const double2* __restrict__ x, double2* __restrict__ out,
                                               int64_t rows_per_wg, int chunk, const double2* __restrict__ Wp, Scal sc,
-                                              int j, double* partials, unsigned* ticket) {
+                                              int j, double* partials) {
     extern __shared__ double2 yl[];
     constexpr int NSUB = PT / LPR;
     const int tid = threadIdx.x, sub = tid / LPR, lane = tid % LPR;
@@ -359,7 +387,7 @@ __global__ void __launch_bounds__(PT) k_pspmv(PanelView P, const double2* __rest
         }
         __syncthreads();
     }
-    if (LZ) grid_reduce(dacc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
+    if (LZ) block_partial(dacc, partials);
 }
 
 // ---- slice-streaming panel SpMV ------------------------------------------------------
@@ -387,7 +415,7 @@ __device__ __forceinline__ double wave_incl_scan(double v, int lane) {
 template <bool REAL, bool LZ, int WPB, int MAXS>
 __global__ void __launch_bounds__(WPB * 64) k_sps(PanelView P, const double2* __restrict__ x,
                                                   double2* __restrict__ out, const double2* __restrict__ Wp, Scal sc,
-                                                  int j, double* partials, unsigned* ticket) {
+                                                  int j, double* partials) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t n = P.n;
     const int64_t nslices = (n + 63) >> 6;
@@ -474,7 +502,7 @@ __global__ void __launch_bounds__(WPB * 64) k_sps(PanelView P, const double2* __
             }
         }
     }
-    if (LZ) grid_reduce(dacc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
+    if (LZ) block_partial(dacc, partials);
 }
 
 // ---- jagged panel SpMV --------------------------------------------------------------
@@ -485,7 +513,7 @@ __global__ void __launch_bounds__(WPB * 64) k_sps(PanelView P, const double2* __
 template <bool REAL, bool LZ, int WPB, int MAXS>
 __global__ void __launch_bounds__(WPB * 64) k_jag(PanelView P, const double2* __restrict__ x,
                                                   double2* __restrict__ out, const double2* __restrict__ Wp, Scal sc,
-                                                  int j, double* partials, unsigned* ticket) {
+                                                  int j, double* partials) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t n = P.n;
     const int64_t nslices = (n + 63) >> 6;
@@ -550,7 +578,7 @@ __global__ void __launch_bounds__(WPB * 64) k_jag(PanelView P, const double2* __
             }
         }
     }
-    if (LZ) grid_reduce(dacc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
+    if (LZ) block_partial(dacc, partials);
 }
 
 // SpMV kind 5: as k_jag, but lane l of a wave sums rows l, l+64, ..,
@@ -560,7 +588,7 @@ __global__ void __launch_bounds__(WPB * 64) k_jag(PanelView P, const double2* __
 template <bool REAL, bool LZ, int WPB, int R, int MAXS>
 __global__ void __launch_bounds__(WPB * 64) k_jagr(PanelView P, const double2* __restrict__ x,
                                                    double2* __restrict__ out, const double2* __restrict__ Wp, Scal sc,
-                                                   int j, double* partials, unsigned* ticket) {
+                                                   int j, double* partials) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t n = P.n;
     const int64_t nss = (n + 64 * R - 1) / (64 * R);
@@ -619,7 +647,7 @@ __global__ void __launch_bounds__(WPB * 64) k_jagr(PanelView P, const double2* _
                 }
             }
     }
-    if (LZ) grid_reduce(dacc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
+    if (LZ) block_partial(dacc, partials);
 }
 
 // ---- column-tile SpMV (kind 4) ---------------------------------------------------------
@@ -638,7 +666,7 @@ constexpr int kT4Pre = kTileCols / kT4Threads;  // x entries per thread per tile
 template <bool REAL, bool LZ, int R>
 __global__ void __launch_bounds__(kT4Threads) k_t4(TileView V, const double2* __restrict__ x,
                                                    double2* __restrict__ out, const double2* __restrict__ Wp, Scal sc,
-                                                   int j, double* partials, unsigned* ticket) {
+                                                   int j, double* partials) {
     __shared__ double2 xs[kTileCols];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t n = V.n;
@@ -734,7 +762,7 @@ __global__ void __launch_bounds__(kT4Threads) k_t4(TileView V, const double2* __
             }
         }
     }
-    if (LZ) grid_reduce(dacc, partials, ticket, [&](double s) { sc.alpha[j] = s * ib; });
+    if (LZ) block_partial(dacc, partials);
 }
 
 // ---- launch helpers -----------------------------------------------------------
@@ -785,7 +813,7 @@ PanelLaunch panel_launch(const blqh_hamil* H, const Plan& pl) {
 
 template <int LPR, int PT, int U, bool REAL, bool LZ>
 void panel_go(const blqh_hamil* H, const PanelLaunch& L, const double2* x, double2* out, const double2* Wp,
-              const Scal& sc, int j, double* partials, unsigned* ticket, hipStream_t st) {
+              const Scal& sc, int j, double* partials, hipStream_t st) {
     auto kern = k_pspmv<LPR, PT, U, REAL, LZ>;
     static bool attr = false;
     if (!attr) {
@@ -793,19 +821,19 @@ void panel_go(const blqh_hamil* H, const PanelLaunch& L, const double2* x, doubl
         attr = true;
     }
     hipLaunchKernelGGL(kern, dim3(L.grid), dim3(PT), L.smem, st, view(H->pan, H->n), x, out, L.rows_per_wg, L.chunk,
-                       Wp, sc, j, partials, ticket);
+                       Wp, sc, j, partials);
 }
 
 template <bool LZ>
 void panel_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2* out, const double2* Wp,
-                    const Scal& sc, int j, double* partials, unsigned* ticket, hipStream_t st) {
+                    const Scal& sc, int j, double* partials, hipStream_t st) {
     BLQH_REQUIRE(!H->pan.jagged, BLQH_EINTERNAL, "row-segment SpMV on a jagged panel layout");
     const PanelLaunch L = panel_launch(H, pl);
     const bool real = H->pan.real;
 #define BLQH_PANEL_CASE(LPR, PT, U)                                                                  \
     if (pl.lpr == LPR && pl.threads == PT && pl.rows == U) {                                          \
-        if (real) panel_go<LPR, PT, U, true, LZ>(H, L, x, out, Wp, sc, j, partials, ticket, st);     \
-        else panel_go<LPR, PT, U, false, LZ>(H, L, x, out, Wp, sc, j, partials, ticket, st);         \
+        if (real) panel_go<LPR, PT, U, true, LZ>(H, L, x, out, Wp, sc, j, partials, st);     \
+        else panel_go<LPR, PT, U, false, LZ>(H, L, x, out, Wp, sc, j, partials, st);         \
         return;                                                                                       \
     }
     BLQH_PANEL_CASE(4, 512, 2) BLQH_PANEL_CASE(8, 512, 2) BLQH_PANEL_CASE(16, 512, 2)
@@ -827,16 +855,16 @@ int sps_grid(const blqh_hamil* H, const Plan& pl) {
 
 template <bool LZ>
 void sps_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2* out, const double2* Wp,
-                  const Scal& sc, int j, double* partials, unsigned* ticket, hipStream_t st) {
+                  const Scal& sc, int j, double* partials, hipStream_t st) {
     BLQH_REQUIRE(!H->pan.jagged, BLQH_EINTERNAL, "slice-streaming SpMV on a jagged panel layout");
     const PanelView V = view(H->pan, H->n);
     const dim3 grid(sps_grid(H, pl)), block(pl.threads);
 #define BLQH_SPS_CASE(WPB, MAXS)                                                                            \
     if (pl.threads == WPB * 64 && pl.rows == MAXS) {                                                        \
         if (H->pan.real)                                                                                    \
-            hipLaunchKernelGGL((k_sps<true, LZ, WPB, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+            hipLaunchKernelGGL((k_sps<true, LZ, WPB, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials); \
         else                                                                                                \
-            hipLaunchKernelGGL((k_sps<false, LZ, WPB, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+            hipLaunchKernelGGL((k_sps<false, LZ, WPB, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials); \
         return;                                                                                             \
     }
     BLQH_SPS_CASE(8, 1) BLQH_SPS_CASE(8, 2) BLQH_SPS_CASE(8, 4) BLQH_SPS_CASE(8, 8)
@@ -847,16 +875,16 @@ void sps_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2
 
 template <bool LZ>
 void jag_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2* out, const double2* Wp,
-                  const Scal& sc, int j, double* partials, unsigned* ticket, hipStream_t st) {
+                  const Scal& sc, int j, double* partials, hipStream_t st) {
     BLQH_REQUIRE(H->pan.jagged && H->pan.rows == 1, BLQH_EINTERNAL, "jagged SpMV without its panel layout");
     const PanelView V = view(H->pan, H->n);
     const dim3 grid(sps_grid(H, pl)), block(pl.threads);
 #define BLQH_JAG_CASE(WPB, MAXS)                                                                            \
     if (pl.threads == WPB * 64 && pl.rows == MAXS) {                                                        \
         if (H->pan.real)                                                                                    \
-            hipLaunchKernelGGL((k_jag<true, LZ, WPB, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+            hipLaunchKernelGGL((k_jag<true, LZ, WPB, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials); \
         else                                                                                                \
-            hipLaunchKernelGGL((k_jag<false, LZ, WPB, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+            hipLaunchKernelGGL((k_jag<false, LZ, WPB, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials); \
         return;                                                                                             \
     }
     BLQH_JAG_CASE(4, 1) BLQH_JAG_CASE(4, 2) BLQH_JAG_CASE(4, 4) BLQH_JAG_CASE(4, 8)
@@ -874,7 +902,7 @@ int jagr_grid(const blqh_hamil* H, const Plan& pl) {
 
 template <bool LZ>
 void jagr_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double2* out, const double2* Wp,
-                   const Scal& sc, int j, double* partials, unsigned* ticket, hipStream_t st) {
+                   const Scal& sc, int j, double* partials, hipStream_t st) {
     BLQH_REQUIRE(H->pan.jagged && H->pan.rows == pl.lpr, BLQH_EINTERNAL,
                  "multi-row jagged SpMV without its panel layout");
     const PanelView V = view(H->pan, H->n);
@@ -882,9 +910,9 @@ void jagr_dispatch(const blqh_hamil* H, const Plan& pl, const double2* x, double
 #define BLQH_JAGR_CASE(WPB, R, MAXS)                                                                            \
     if (pl.threads == WPB * 64 && pl.lpr == R && pl.rows == MAXS) {                                             \
         if (H->pan.real)                                                                                        \
-            hipLaunchKernelGGL((k_jagr<true, LZ, WPB, R, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+            hipLaunchKernelGGL((k_jagr<true, LZ, WPB, R, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials); \
         else                                                                                                    \
-            hipLaunchKernelGGL((k_jagr<false, LZ, WPB, R, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+            hipLaunchKernelGGL((k_jagr<false, LZ, WPB, R, MAXS>), grid, block, 0, st, V, x, out, Wp, sc, j, partials); \
         return;                                                                                                 \
     }
     BLQH_JAGR_CASE(8, 2, 1) BLQH_JAGR_CASE(8, 2, 2) BLQH_JAGR_CASE(8, 2, 4)
@@ -904,7 +932,7 @@ int t4_R(int R) {
 
 template <bool LZ>
 void t4_dispatch(const blqh_hamil* H, const double2* x, double2* out, const double2* Wp, const Scal& sc, int j,
-                 double* partials, unsigned* ticket, hipStream_t st) {
+                 double* partials, hipStream_t st) {
     const Tiles& T = H->til;
     BLQH_REQUIRE(T.built, BLQH_EINTERNAL, "column-tile SpMV without its layout");
     const TileView V = tview(T, H->n);
@@ -912,9 +940,9 @@ void t4_dispatch(const blqh_hamil* H, const double2* x, double2* out, const doub
 #define BLQH_T4_CASE(RR)                                                                                    \
     if (t4_R(T.R) == RR) {                                                                                  \
         if (T.real)                                                                                         \
-            hipLaunchKernelGGL((k_t4<true, LZ, RR>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+            hipLaunchKernelGGL((k_t4<true, LZ, RR>), grid, block, 0, st, V, x, out, Wp, sc, j, partials); \
         else                                                                                                \
-            hipLaunchKernelGGL((k_t4<false, LZ, RR>), grid, block, 0, st, V, x, out, Wp, sc, j, partials, ticket); \
+            hipLaunchKernelGGL((k_t4<false, LZ, RR>), grid, block, 0, st, V, x, out, Wp, sc, j, partials); \
         return;                                                                                             \
     }
     BLQH_T4_CASE(1) BLQH_T4_CASE(2) BLQH_T4_CASE(3) BLQH_T4_CASE(4) BLQH_T4_CASE(6) BLQH_T4_CASE(8)
@@ -925,15 +953,15 @@ void t4_dispatch(const blqh_hamil* H, const double2* x, double2* out, const doub
 
 void launch_spmv_plan(const blqh_hamil* H, const Plan& pl, const double2* x, double2* y, hipStream_t st) {
     if (pl.kind == 5) {
-        jagr_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
+        jagr_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, st);
     } else if (pl.kind == 4) {
-        t4_dispatch<false>(H, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
+        t4_dispatch<false>(H, x, y, nullptr, Scal{}, 0, nullptr, st);
     } else if (pl.kind == 3) {
-        jag_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
+        jag_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, st);
     } else if (pl.kind == 2) {
-        sps_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
+        sps_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, st);
     } else if (pl.kind == 1) {
-        panel_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, nullptr, st);
+        panel_dispatch<false>(H, pl, x, y, nullptr, Scal{}, 0, nullptr, st);
     } else {
         const CsrView A = spmv_csr(H);
         BLQH_LPR_SWITCH(pl.lpr, k_spmv, dim3(spmv_grid(H->n, pl.lpr)), dim3(kThreads), 0, st, H->n, A.rowptr,
@@ -949,21 +977,21 @@ int lz1_grid(const blqh_hamil* H, const Plan& pl) {
 }
 
 void launch_lz1(const blqh_hamil* H, const Plan& pl, const double2* W, const double2* Wp, double2* u, const Scal& sc,
-                int j, double* partials, unsigned* ticket, hipStream_t st) {
+                int j, double* partials, hipStream_t st) {
     if (pl.kind == 5) {
-        jagr_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
+        jagr_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, st);
     } else if (pl.kind == 4) {
-        t4_dispatch<true>(H, W, u, Wp, sc, j, partials, ticket, st);
+        t4_dispatch<true>(H, W, u, Wp, sc, j, partials, st);
     } else if (pl.kind == 3) {
-        jag_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
+        jag_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, st);
     } else if (pl.kind == 2) {
-        sps_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
+        sps_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, st);
     } else if (pl.kind == 1) {
-        panel_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, ticket, st);
+        panel_dispatch<true>(H, pl, W, u, Wp, sc, j, partials, st);
     } else {
         const CsrView A = spmv_csr(H);
         BLQH_LPR_SWITCH(pl.lpr, k_lz1, dim3(spmv_grid(H->n, pl.lpr)), dim3(kThreads), 0, st, H->n, A.rowptr,
-                        A.col, A.val, W, Wp, u, sc, j, partials, ticket);
+                        A.col, A.val, W, Wp, u, sc, j, partials);
     }
 }
 
@@ -1453,7 +1481,7 @@ namespace {
 // Work areas of one solve.  Device: Krylov block (store: nvec = cap + 1
 // vectors) or ring of three, u, psi, the replay start v0, the per-step
 // scalars (3 cap + 8), coefficients (cap + 1), reductions (4), block
-// partials (npart) and the reduction ticket.  Host: arrays of cap + 1
+// partials of K1 and of K2 (npart each).  Host: arrays of cap + 1
 // doubles and the tridiagonal work space (8 cap doubles, cap ints).
 struct DevWork {
     double2* Wb;
@@ -1463,10 +1491,32 @@ struct DevWork {
     double* sc;
     double* coef;
     double* red;
-    double* partials;
+    double* partials;   // K1's
+    double* partials2;  // K2's
     int64_t npart;
-    unsigned* ticket;
 };
+
+// K2 and K3 of one step after the K1 launched with grid `g1` wrote part1.
+void launch_lz23(int64_t n, const double2* u, const double2* W, double2* Wn, const Scal& sc, int j, int ns,
+                 double2* psi, const double* coef, int ldc, const double* part1, int g1, double* part2,
+                 hipStream_t st) {
+    const int gv = vec_grid(n);
+    hipLaunchKernelGGL(k_lz2, dim3(gv), dim3(kThreads), 0, st, n, u, W, Wn, sc, j, ns, psi, coef, ldc, part1, g1,
+                       part2);
+    hipLaunchKernelGGL(k_lz3, dim3(1), dim3(kThreads), 0, st, sc, j, part2, gv);
+}
+
+// *out = |x|^2 and *out = Re<x|y>: block partials, then one wavefront sums them
+void reduce_norm2(int64_t n, const double2* x, double* out, double* part, hipStream_t st) {
+    const int gv = vec_grid(n);
+    hipLaunchKernelGGL(k_norm2, dim3(gv), dim3(kThreads), 0, st, n, x, part);
+    hipLaunchKernelGGL(k_psum, dim3(1), dim3(kThreads), 0, st, part, gv, out);
+}
+void reduce_redot(int64_t n, const double2* x, const double2* y, double* out, double* part, hipStream_t st) {
+    const int gv = vec_grid(n);
+    hipLaunchKernelGGL(k_redot, dim3(gv), dim3(kThreads), 0, st, n, x, y, part);
+    hipLaunchKernelGGL(k_psum, dim3(1), dim3(kThreads), 0, st, part, gv, out);
+}
 struct HostWork {
     double *ha, *hb, *ritz, *svec, *c, *aux, *aux2, *tri;
     int* piv;
@@ -1487,7 +1537,7 @@ void lanczos_core(blqh_hamil* H, const Plan& plan, const blqh_lanczos_opts& o, b
     const int gv = vec_grid(n);
     BLQH_REQUIRE(lz1_grid(H, plan) <= D.npart && spmv_grid(n, 4) <= D.npart && gv <= D.npart, BLQH_EINTERNAL,
                  "partial-sum buffer smaller than the grid");
-    BLQH_HIP(hipMemsetAsync(D.ticket, 0, sizeof(unsigned), st));
+    const int g1 = lz1_grid(H, plan);  // K1's partials, summed by K2
     Scal sc{D.sc, D.sc + cap, D.sc + 2 * cap + 2};
     const double one = 1.0;
     BLQH_HIP(hipMemcpyAsync(sc.beta, &one, 8, hipMemcpyHostToDevice, st));
@@ -1516,10 +1566,10 @@ void lanczos_core(blqh_hamil* H, const Plan& plan, const blqh_lanczos_opts& o, b
             }
             BLQH_HIP(hipEventRecord(prof.ev[2 * ntimed], st));
         }
-        launch_lz1(H, plan, W, Wp, D.u, sc, j, D.partials, D.ticket, st);
+        launch_lz1(H, plan, W, Wp, D.u, sc, j, D.partials, st);
         if (prof_on) BLQH_HIP(hipEventRecord(prof.ev[2 * ntimed++ + 1], st));
-        hipLaunchKernelGGL(k_lz2, dim3(gv), dim3(kThreads), 0, st, n, D.u, W, slot(j + 1), sc, j,
-                           psi_acc ? 1 : 0, psi_acc, cf, cap, D.partials, D.ticket);
+        launch_lz23(n, D.u, W, slot(j + 1), sc, j, psi_acc ? 1 : 0, psi_acc, cf, cap, D.partials, g1, D.partials2,
+                    st);
     };
     auto account = [&]() {
         if (!prof_on) return;
@@ -1616,10 +1666,10 @@ void lanczos_core(blqh_hamil* H, const Plan& plan, const blqh_lanczos_opts& o, b
         res.passes = 3;
     }
     // normalise and Rayleigh quotient
-    hipLaunchKernelGGL(k_norm2, dim3(gv), dim3(kThreads), 0, st, n, D.psi, D.red, D.partials, D.ticket);
+    reduce_norm2(n, D.psi, D.red, D.partials, st);
     hipLaunchKernelGGL(k_scale, dim3(gv), dim3(kThreads), 0, st, n, D.psi, D.red);
     launch_spmv_plan(H, plan, D.psi, D.u, st);
-    hipLaunchKernelGGL(k_redot, dim3(gv), dim3(kThreads), 0, st, n, D.psi, D.u, D.red + 1, D.partials, D.ticket);
+    reduce_redot(n, D.psi, D.u, D.red + 1, D.partials, st);
     BLQH_HIP(hipGetLastError());
     double erq = 0;
     BLQH_HIP(hipMemcpyAsync(&erq, D.red + 1, 8, hipMemcpyDeviceToHost, st));
@@ -1662,15 +1712,14 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
     DevBuf<double2> Wb((size_t)nvec * n), u(n), psi(n), v0(store ? 0 : n);
     DevBuf<double> sc_buf(3 * (size_t)cap + 8), coef(cap + 1), red(4);
     const int gmax = std::max(std::max(lz1_grid(H, plan), spmv_grid(n, 4)), vec_grid(n));
-    DevBuf<double> partials(gmax);
-    DevBuf<unsigned> ticket(1);
+    DevBuf<double> partials(2 * (size_t)gmax);
     std::vector<double> hbuf(7 * ((size_t)cap + 1) + 8 * (size_t)cap);
     std::vector<int> piv(cap);
     double* hp = hbuf.data();
     const size_t L = (size_t)cap + 1;
     const HostWork hw{hp, hp + L, hp + 2 * L, hp + 3 * L, hp + 4 * L, hp + 5 * L, hp + 6 * L, hp + 7 * L, piv.data()};
     const DevWork D{Wb.get(), u.get(), psi.get(), v0.get(), sc_buf.get(), coef.get(), red.get(), partials.get(),
-                    gmax, ticket.get()};
+                    partials.get() + gmax, gmax};
     lanczos_core(H, plan, o, store, D, hw,
                  [&](double2* dst) { device_start(n, o.seed, o.start, dst, st, o.start_mix); }, true, res, psi_out,
                  alpha_out, beta_out, t0);
@@ -1684,7 +1733,7 @@ void blqh_lanczos_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_lanczos_r
 // the library's device initialisation, with the device current) asks the
 // runtime for this device's address and every use goes through that pointer.
 // Calls are serialised by a mutex (they share the array); each call
-// initialises everything it reads (reduction ticket, scalars, psi), so
+// initialises everything it reads (scalars, psi), so
 // nothing carries over from the previous call's contents.
 namespace {
 constexpr int64_t kNoAllocMaxRows = int64_t(1) << 17;
@@ -1835,8 +1884,8 @@ void blqh_lanczos_noalloc_impl(blqh_hamil* H, const blqh_lanczos_opts& o, blqh_l
     double* coef = sc + 3 * (size_t)cap + 8;
     double* red = coef + cap + 1;
     double* partials = red + 4;
-    unsigned* ticket = reinterpret_cast<unsigned*>(sd + kNoAllocScalDoubles);
-    const DevWork D{vec, vec + 3 * n, vec + 4 * n, vec + 5 * n, sc, coef, red, partials, kNoAllocPartials, ticket};
+    const DevWork D{vec, vec + 3 * n, vec + 4 * n, vec + 5 * n, sc, coef, red, partials,
+                    partials + kNoAllocPartials / 2, kNoAllocPartials / 2};
     double* hp = g_noalloc_host;
     const size_t L = (size_t)cap + 1;
     const HostWork hw{hp, hp + L, hp + 2 * L, hp + 3 * L, hp + 4 * L, hp + 5 * L, hp + 6 * L, hp + 7 * L,
@@ -1882,9 +1931,7 @@ void blqh_lanczos_reconstruct_impl(blqh_hamil* H, int m, uint32_t seed, const do
     DevBuf<double2> Wb(3 * (size_t)n), u(n), psi((size_t)ns * n);
     DevBuf<double> sc_buf(3 * (size_t)m + 8), coef((size_t)ns * m), red(2 * ns + 2);
     const int gmax = std::max(std::max(lz1_grid(H, plan), spmv_grid(n, 4)), vec_grid(n));
-    DevBuf<double> partials(gmax);
-    DevBuf<unsigned> ticket(1);
-    BLQH_HIP(hipMemsetAsync(ticket.get(), 0, sizeof(unsigned), st));
+    DevBuf<double> partials(2 * (size_t)gmax);
     Scal sc{sc_buf.get(), sc_buf.get() + m, sc_buf.get() + 2 * m + 2};
     const double one = 1.0;
     BLQH_HIP(hipMemcpyAsync(sc.beta, &one, 8, hipMemcpyHostToDevice, st));
@@ -1903,9 +1950,9 @@ void blqh_lanczos_reconstruct_impl(blqh_hamil* H, int m, uint32_t seed, const do
     for (int j = 0; j + 1 < m; ++j) {
         double2* W = slot(j);
         double2* Wp = j > 0 ? slot(j - 1) : slot(j);
-        launch_lz1(H, plan, W, Wp, u.get(), sc, j, partials.get(), ticket.get(), st);
-        hipLaunchKernelGGL(k_lz2, dim3(gv), dim3(kThreads), 0, st, n, u.get(), W, slot(j + 1), sc, j, ns, psi.get(),
-                           coef.get(), m, partials.get(), ticket.get());
+        launch_lz1(H, plan, W, Wp, u.get(), sc, j, partials.get(), st);
+        launch_lz23(n, u.get(), W, slot(j + 1), sc, j, ns, psi.get(), coef.get(), m, partials.get(), lz1_grid(H, plan),
+                    partials.get() + gmax, st);
     }
     BLQH_HIP(hipGetLastError());
     std::vector<double> ra(m), rb(m + 1);
@@ -1925,12 +1972,10 @@ void blqh_lanczos_reconstruct_impl(blqh_hamil* H, int m, uint32_t seed, const do
     BLQH_REQUIRE(std::isfinite(dev), BLQH_EINTERNAL, "replay produced non-finite coefficients");
     for (int s = 0; s < ns; ++s) {
         double2* p = psi.get() + (size_t)s * n;
-        hipLaunchKernelGGL(k_norm2, dim3(gv), dim3(kThreads), 0, st, n, p, red.get() + 2 * s, partials.get(),
-                           ticket.get());
+        reduce_norm2(n, p, red.get() + 2 * s, partials.get(), st);
         hipLaunchKernelGGL(k_scale, dim3(gv), dim3(kThreads), 0, st, n, p, red.get() + 2 * s);
         launch_spmv_plan(H, plan, p, u.get(), st);
-        hipLaunchKernelGGL(k_redot, dim3(gv), dim3(kThreads), 0, st, n, p, u.get(), red.get() + 2 * s + 1,
-                           partials.get(), ticket.get());
+        reduce_redot(n, p, u.get(), red.get() + 2 * s + 1, partials.get(), st);
     }
     BLQH_HIP(hipGetLastError());
     std::vector<double> hred(2 * ns);

@@ -109,3 +109,32 @@ def test_shared_scale_refusals(bases):
     with pytest.raises(blqh_amd.BlqhError):
         bt.lanes = 8
     bt.close()
+
+
+def test_config3_all_sectors_shared():
+    """The sweep workload at its size: N_phi = 12, N_e = 12, all densities, all
+    144 (J, K) sectors (18,772-18,806 states) at d = 0.6 with Delta_SAS in
+    {0, 0.05, 0.1}: 432 problems in one shared-scale batch, each equal to the
+    single solver on the merged Hamiltonian."""
+    scales = [0.0, 0.05, 0.1]
+    bases = [blqh_amd.Basis(12, 12, -1, J, K) for J in range(12) for K in range(12)]
+    assert all(18772 <= B.n <= 18806 for B in bases)
+    par = blqh_amd.params(12, d=0.6)
+    base = [blqh_amd.Hamiltonian(B, par, terms=1) for B in bases]
+    tun = [blqh_amd.Hamiltonian(B, par, terms=2) for B in bases]
+    print("built 144 sectors", flush=True)
+    bt = blqh_amd.Batch.scaled(base, tun, scales)
+    seeds = [3 + i for i in range(432)]
+    res = bt.lanczos(lam=250, seeds=seeds, check_every=2, want_psi=False)
+    bt.close()
+    print("shared-scale solve done", flush=True)
+    for q in range(144):
+        for g, s in enumerate(scales):
+            p = 3 * q + g
+            H = blqh_amd.Hamiltonian.combine(base[q], tun[q], s)
+            ref = H.lanczos(lam=250, seed=seeds[p], check_every=4, want_psi=False)
+            # a problem that needs more than 250 steps stops unconverged in both
+            assert res[p]["converged"] == ref["converged"], (q, s)
+            assert res[p]["E0"] == pytest.approx(ref["E0"], rel=1e-10, abs=0), (q, s)
+            assert abs(res[p]["iterations"] - ref["iterations"]) <= 2, (q, s)
+    assert sum(r["converged"] for r in res) >= 430
