@@ -366,6 +366,8 @@ def run_sweep_shared(args):
 
     tun = None  # T does not depend on d without theta_B phases: built once
     batch = None
+    cont_psi = None
+    cont = {"solved": 0, "seconds": 0.0, "iters": [], "max_rel_dE0": 0.0, "mismatch": 0}
     plain = None
     solved = solves = 0
     t_solve = t_build = 0.0
@@ -388,6 +390,27 @@ def run_sweep_shared(args):
         log(f"{'warmup' if warm else 'step'} d={d} DSAS={ds_list}: build {tb:.2f} s, shared-scale batched solve "
             f"{dt:.3f} s, {ok}/{len(res)} converged, iterations {min(r['iterations'] for r in res)}.."
             f"{max(r['iterations'] for r in res)}")
+        if args.continuation:
+            # every problem starts from its ground state at the previous d (same
+            # sector and DSAS), checked against the cold solve of this d
+            t = time.perf_counter()
+            starts = [np.conj(p) for p in cont_psi] if cont_psi is not None else None
+            wres = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, starts=starts,
+                                 want_psi=True, start_mix=args.start_mix if starts else 0.0)
+            dtw = time.perf_counter() - t
+            cont_psi = [r["psi"] for r in wres]
+            wok = [r["converged"] == 1 and r["dE"] < 1e-10 for r in wres]
+            rel = [abs(a["E0"] - b["E0"]) / abs(b["E0"]) for a, b in zip(wres, res)]
+            bad = sum(int(x > 1e-10) for x, w in zip(rel, wok) if w)
+            log(f"  continuation: batched solve {dtw:.3f} s, {sum(wok)}/{len(wres)} converged, iterations "
+                f"{min(r['iterations'] for r in wres)}..{max(r['iterations'] for r in wres)}, "
+                f"{bad} above the cold E0 by more than 1e-10")
+            if not warm and starts is not None:
+                cont["solved"] += sum(int(w and x <= 1e-10) for w, x in zip(wok, rel))
+                cont["seconds"] += dtw
+                cont["iters"] += [r["iterations"] for r in wres]
+                cont["max_rel_dE0"] = max(cont["max_rel_dE0"], max(rel))
+                cont["mismatch"] += bad
         if warm and plain is None:
             # plain single-problem solves of the same problems (each autotuned by a first solve)
             hs = [blqh_amd.Hamiltonian.combine(b, tn, x) for b, tn in zip(base, tun) for x in ds_list]
@@ -451,6 +474,12 @@ def run_sweep_shared(args):
             "batched_over_plain": value / plain["rate"] if plain and plain["rate"] else None,
             "plain": plain,
             "end_to_end_rate_incl_build": solved / (t_solve + t_build) if t_solve + t_build > 0 else None,
+            "continuation_rate": cont["solved"] / cont["seconds"] if cont["seconds"] > 0 else None,
+            "continuation_iterations_mean": float(np.mean(cont["iters"])) if cont["iters"] else None,
+            "continuation_max_rel_dE0_vs_cold": cont["max_rel_dE0"] if cont["iters"] else None,
+            "continuation_mismatches": cont["mismatch"] if cont["iters"] else None,
+            "continuation_vs_plain": (cont["solved"] / cont["seconds"]) / plain["rate"]
+            if cont["seconds"] > 0 and plain else None,
             "build_s_per_step": t_build / max(1, args.steps),
             "build_scheme": "H(d,0) per d; T once per sector; the batch merges H(d,0) + T once per d and "
                             "flags T's entries",
@@ -773,7 +802,9 @@ def main():
         out["sweeps"] = {}
         for name in ("sweep_spin_nphi8", "sweep_nphi12"):
             sa = argparse.Namespace(**vars(args))
-            sa.workload, sa.steps, sa.warmup, sa.continuation = name, args.sweep_points, 1, 0
+            # continuation: the batch also solves every point from the previous
+            # point's ground states (the warm leg of "batched + warm")
+            sa.workload, sa.steps, sa.warmup, sa.continuation = name, args.sweep_points, 1, 1
             sa.no_cpu_baseline = args.no_cpu_baseline
             shared = args.shared and len({ds for _, ds in SWEEPS[name]["points"]}) > 1
             r = run_sweep_shared(sa) if shared else run_sweep(sa)
@@ -783,6 +814,10 @@ def main():
                 "solves_timed": c["solves_timed"], "converged": c["converged"],
                 "batched_rate": c["batched_rate"], "plain_rate": c["plain_rate"],
                 "batched_over_plain": c["batched_over_plain"], "iterations_mean": c["iterations_mean"],
+                "batched_warm_rate": c["continuation_rate"], "batched_warm_over_plain": c["continuation_vs_plain"],
+                "warm_iterations_mean": c["continuation_iterations_mean"],
+                "warm_start": "previous parameter point's ground state + 0.1 x seeded unit vector",
+                "warm_solves_above_cold_E0": c["continuation_mismatches"],
                 "end_to_end_rate_incl_build": c["end_to_end_rate_incl_build"],
                 "max_rel_dE0_batched_vs_plain": c["plain"]["max_rel_dE0_vs_batched"] if c["plain"] else None,
                 "roofline": {k: r["roofline"][k] for k in ("achieved", "peak", "unit", "frac", "kernel",

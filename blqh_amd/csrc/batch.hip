@@ -1303,13 +1303,24 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     std::vector<std::vector<double>> svecs(nthr, std::vector<double>(cap));
     std::vector<int> live_ids;
     live_ids.reserve(P);
-    // Replay r+1 is queued before the host tests replay r, so the host test
-    // overlaps the GPU; a problem that converged in replay r runs one more
-    // replay of steps that are never used.
+    // The next replays are queued before the host tests the last ones, so the
+    // host test overlaps the GPU; a problem that converged in the tested
+    // replays runs the queued ones too, whose steps are never used.  While
+    // many problems are live one replay is queued ahead; with a quarter or
+    // fewer a replay is short (a few tens of microseconds per step) and the
+    // host test of each replay would leave the GPU idle, so two, then four
+    // replays go ahead of each test.
+    int queued = 0;  // steps queued so far
     auto enqueue = [&](int first) {
         if (timed) chunk(true, first);
         else BLQH_HIP(hipGraphLaunch(gexec, st));
         BLQH_HIP(hipGetLastError());
+    };
+    auto enqueue_n = [&](int r) {
+        for (int i = 0; i < r && queued < cap; ++i) {
+            enqueue(queued);
+            queued += chk;
+        }
     };
     auto readback = [&]() {
         BLQH_HIP(hipMemcpyAsync(ha, pl.alpha.get(), (size_t)P * cap * 8, hipMemcpyDeviceToHost, st));
@@ -1317,17 +1328,17 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
         BLQH_HIP(hipEventRecord(evc, st));
     };
     int done = 0, nlive = P, slot = 0;
-    enqueue(0);
+    enqueue_n(1);
     readback();
     while (true) {
-        const int upto = std::min(cap, done + chk);
-        const bool more = upto < cap;
-        if (more) enqueue(upto);  // speculative next replay
+        const int upto = std::min(cap, queued);  // steps the last readback covers
+        const bool more = queued < cap;
+        if (more) enqueue_n(4 * nlive > P ? 1 : 16 * nlive > P ? 2 : 4);  // speculative next replays
         BLQH_HIP(hipEventSynchronize(evc));
         // the single-problem solver's convergence test, per live problem
         // (problems split over host threads; each writes only its own entries)
         auto test = [&](int p, std::vector<double>& svec) {
-            steps_run[p] = std::min(cap, upto + (more ? chk : 0));
+            steps_run[p] = std::min(cap, queued);
             const double* a = ha + (size_t)p * cap;
             const double* bb = hb + (size_t)p * ld;
             for (int j = done; j < upto; ++j) {
