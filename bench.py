@@ -40,6 +40,13 @@ MEASURED_TRAFFIC = {
     ("nphi16_all", (3, 64, 512, 2, 8)): (6544086 * 2.0 * 1024 + 36721.5 * 1024,
                                          "profiles/k_jag_nphi16_counters.md"),
 }
+# HBM bytes per batched K1 launch of the sweeps (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, mean over the
+# run's K1 dispatches; profiles/*_counters.md), reported for the same batching mode
+MEASURED_SWEEP_TRAFFIC = {
+    ("sweep_spin_nphi8", "per-point"): (739444.2 * 2.0 * 1024 + 11263.0 * 1024,
+                                        "profiles/k_b1j_sweep_spin_nphi8_counters.md"),
+    ("sweep_nphi12", "shared"): (2438173.0 * 2.0 * 1024 + 69723.4 * 1024, "profiles/k_b1s_sweep_nphi12_counters.md"),
+}
 # fused Lanczos K1 kernel of each SpMV variant (lanczos.hip)
 KERNEL_NAMES = {0: "k_lz1", 1: "k_pspmv<LZ=true>", 2: "k_sps<LZ=true>", 3: "k_jag<LZ=true>", 4: "k_t4<LZ=true>",
                 5: "k_jagr<LZ=true>"}
@@ -150,6 +157,8 @@ def sweep_cpu_baseline(sw, iters_ref=200):
 
 def run_sweep(args):
     import blqh_amd
+    if args.events_all:
+        blqh_amd.profile_enable(True)  # every batched launch event-timed: no graph replays
     sw = SWEEPS[args.workload]
     S, nphi, nel = sw["S"], sw["nphi"], sw["nel"]
     log(f"workload {args.workload}: {sw['label']}")
@@ -260,7 +269,7 @@ def run_sweep(args):
             blqh_amd.profile_reset()
             blqh_amd.profile_enable(True)
             batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False)
-            blqh_amd.profile_enable(False)
+            blqh_amd.profile_enable(args.events_all)
             launches, kms, kbytes = blqh_amd.profile_read()
             nnz_total = sum(H.nnz for H in hs)
         del hs
@@ -315,7 +324,8 @@ def run_sweep(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": (MEASURED_SWEEP_TRAFFIC.get((args.workload, "per-point")) or (None,))[0],
+            "traffic_source": (MEASURED_SWEEP_TRAFFIC.get((args.workload, "per-point")) or (None, None))[1],
             "kernel": ("k_b1j<LZ=true> (batched fused SpMV + Lanczos K1, step-major panels, two wavefronts per "
                        "64-row slice)" if batch.lanes == 64 else
                        f"k_b1<{batch.lanes}, LZ=true> (batched fused SpMV + Lanczos K1, row-major panels)"),
@@ -342,6 +352,8 @@ def run_sweep_shared(args):
     step for all of its DSAS problems).  Plain rate: single-problem solves of
     the warm-up d's merged Hamiltonians H(d,0) + DSAS T, every sector and DSAS."""
     import blqh_amd
+    if args.events_all:
+        blqh_amd.profile_enable(True)  # every batched launch event-timed: no graph replays
     sw = SWEEPS[args.workload]
     S, nphi, nel = sw["S"], sw["nphi"], sw["nel"]
     log(f"workload {args.workload} (shared-scale batches): {sw['label']}")
@@ -436,7 +448,7 @@ def run_sweep_shared(args):
             blqh_amd.profile_reset()
             blqh_amd.profile_enable(True)
             batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False)
-            blqh_amd.profile_enable(False)
+            blqh_amd.profile_enable(args.events_all)
             launches, kms, kbytes = blqh_amd.profile_read()
             nnz_total = sum(H.nnz for H in base) + sum(H.nnz for H in tun)
         del base  # the batch keeps its own copy of H(d,0) + T
@@ -491,7 +503,8 @@ def run_sweep_shared(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": (MEASURED_SWEEP_TRAFFIC.get((args.workload, "shared")) or (None,))[0],
+            "traffic_source": (MEASURED_SWEEP_TRAFFIC.get((args.workload, "shared")) or (None, None))[1],
             "kernel": f"k_b1s<LZ=true, G={G}> (shared-scale batched fused SpMV + Lanczos K1: each sector's "
                       f"entries read once for its {G} DSAS problems)",
             "avg_launch_ms": avg_ms,
@@ -681,6 +694,9 @@ def main():
                     help="default workload: also time the sweep workloads at this many parameter points (0: skip)")
     ap.add_argument("--start-mix", type=float, default=0.1,
                     help="weight of the seeded random unit vector added to warm starts (guards level crossings)")
+    ap.add_argument("--events-all", action="store_true",
+                    help="sweeps: time every batched K1 launch with HIP events (plain launches, no graph "
+                         "replays), e.g. under a hardware-counter profiler")
     ap.add_argument("--shared", type=int, default=1,
                     help="sweeps with several DSAS values per d: one shared-scale batch per d (1) or one batch "
                          "per parameter point (0)")
@@ -820,7 +836,7 @@ def main():
                 "warm_solves_above_cold_E0": c["continuation_mismatches"],
                 "end_to_end_rate_incl_build": c["end_to_end_rate_incl_build"],
                 "max_rel_dE0_batched_vs_plain": c["plain"]["max_rel_dE0_vs_batched"] if c["plain"] else None,
-                "roofline": {k: r["roofline"][k] for k in ("achieved", "peak", "unit", "frac", "kernel",
+                "roofline": {k: r["roofline"][k] for k in ("achieved", "peak", "unit", "frac", "traffic", "kernel",
                                                            "avg_launch_ms", "algorithmic_bytes_per_launch")},
                 "cpu_baseline": r.get("cpu_baseline"),
             }
