@@ -955,6 +955,7 @@ namespace {
 // count must equal T's.
 blqh_hamil* make_template(const blqh_hamil* base, const blqh_hamil* tun) {
     BLQH_REQUIRE(base != nullptr && tun != nullptr, BLQH_EINVAL, "null hamiltonian in batch");
+    BLQH_REQUIRE(base->basis->geo.S >= 2, BLQH_EINVAL, "a shared-scale batch needs bilayer sectors");
     BLQH_REQUIRE(base->n <= (int64_t)kTunBit, BLQH_EINVAL,
                  "a shared-scale batch takes sectors of at most 32768 states");
     auto* H = new blqh_hamil;
