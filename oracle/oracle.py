@@ -218,6 +218,18 @@ class Hamiltonian:
             self.h = None
 
 
+class CsrMatrix(Hamiltonian):
+    """A caller's CSR (rowptr, col, val) with the oracle's SpMV and Lanczos."""
+
+    def __init__(self, rowptr, col, val):
+        self.basis = None
+        self.rowptr = np.ascontiguousarray(rowptr, np.int64)
+        self.col = np.ascontiguousarray(col, np.int64)
+        self.val = np.ascontiguousarray(val, np.complex128)
+        self.n = len(self.rowptr) - 1
+        self.h = lib().orc_csr_wrap(self.n, _p(self.rowptr), _p(self.col), _p(self.val.view(np.float64)))
+
+
 def hamil_rows(basis, par, rows, nthreads=0):
     """Build selected rows only.  Returns (rowptr, col, val)."""
     rows = np.ascontiguousarray(rows, np.int64)
