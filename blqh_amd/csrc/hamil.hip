@@ -272,6 +272,7 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
     __shared__ double vre[CAP], vim[CAP];
     __shared__ unsigned long long sbuf[MODE >= 1 && CAP > 512 ? CAP : 1];
     __shared__ int occ[64];
+    __shared__ int sQ[128];  // open pair hops awaiting evaluation
     __shared__ int s_row, s_ovf;
 
     const int lane = threadIdx.x;
@@ -311,22 +312,54 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
             const uint64_t L = translate(g, L0, ql, signl);
             if (lane < nbits && ((L >> lane) & 1ULL)) occ[popc(L & lowbits(lane))] = lane;
             __syncthreads();
+            // Candidates cnd = pr * T + tt (pair pr, shift tt - nphi + 1).  At
+            // half filling only about a third of them are open (both targets
+            // empty), and a wavefront that evaluated them in place would run
+            // the whole target lookup for nearly every group of 64.  So the
+            // open ones are queued in candidate order (sQ, packed pr << 8 | tt)
+            // and evaluated 64 at a time; the hash receives the terms in the
+            // same order as an in-place loop would give them.
             const int ncand = (A.terms & 1) ? npairs * T : 0;
-            // candidate cnd = pr * T + tt, advanced by 64 without division
-            const int d0 = 64 / T, d1 = 64 % T;
+            const int d0 = 64 / T, d1 = 64 % T;  // advance by 64 candidates without division
             int pr = lane / T, tt = lane % T;
-            for (int cnd = lane; cnd < ncand; cnd += 64) {
-                const int t = tt - (nphi - 1);
-                const unsigned short pw = sPair[pr];
-                int j;
-                double re, im;
-                if (pair_term(ctx, L, signl, Dl, ql, occ[pw & 0xff], occ[pw >> 8], t, j, re, im))
-                    if (!hash_add<CAP>(keys, vre, vim, j, re, im)) ovf = true;
-                pr += d0;
-                tt += d1;
-                if (tt >= T) {
-                    tt -= T;
-                    ++pr;
+            int next = 0, qn = 0;
+            for (;;) {
+                while (qn < 64 && next < ncand) {
+                    bool open = false;
+                    if (next + lane < ncand) {
+                        const unsigned short pw = sPair[pr];
+                        PairHop h;
+                        open = pair_hop(nphi, L, occ[pw & 0xff], occ[pw >> 8], tt - (nphi - 1), h);
+                    }
+                    const unsigned long long bal = __ballot(open);
+                    if (open)
+                        sQ[qn + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u))] =
+                            (pr << 8) | tt;
+                    qn += __popcll(bal);
+                    next += 64;
+                    pr += d0;
+                    tt += d1;
+                    if (tt >= T) {
+                        tt -= T;
+                        ++pr;
+                    }
+                }
+                if (qn == 0) break;
+                __syncthreads();  // queue entries written by other lanes
+                const int take = qn < 64 ? qn : 64;
+                const int e = lane < take ? sQ[lane] : -1;
+                const int carry = lane + 64 < qn ? sQ[lane + 64] : 0;
+                __syncthreads();  // every entry read before the carried ones move to the front
+                if (lane + 64 < qn) sQ[lane] = carry;
+                qn -= take;
+                if (e >= 0) {
+                    const unsigned short pw = sPair[e >> 8];
+                    int j;
+                    double re, im;
+                    if (pair_term(ctx, L, signl, Dl, ql, occ[pw & 0xff], occ[pw >> 8], (e & 0xff) - (nphi - 1), j,
+                                  re, im))
+                        if (!hash_add<CAP>(keys, vre, vim, j, re, im)) ovf = true;
                 }
             }
             for (int h = lane; h < A.nhop; h += 64) {

@@ -100,33 +100,54 @@ BLQH_HD bool locate(const RowCtx& c, uint64_t R0, int ql, int& j, int& qr, int& 
     return true;
 }
 
+// Pair hop n -> n + t, m -> m - t, each orbital shifted within its block
+// (layer / species; |t| < nphi) with wrap-around.
+struct PairHop {
+    int a, b;      // blocks of n and m
+    int n0, xm;    // n's orbital and m's target orbital within their blocks
+    bool wn, wm;   // the electron wraps around its block
+    int nt, mt;    // target orbitals
+    uint64_t word; // L with n, m moved to nt, mt
+};
+
+// Targets of the pair hop of n and m (both occupied in L) by t; false when
+// the hop is blocked (the targets coincide, or another electron of L holds
+// one of them).  The compares replace integer divisions.
+BLQH_HD bool pair_hop(int nphi, uint64_t L, int n, int m, int t, PairHop& h) {
+    h.a = n >= nphi ? (n >= 2 * nphi ? (n >= 3 * nphi ? 3 : 2) : 1) : 0;
+    h.b = m >= nphi ? (m >= 2 * nphi ? (m >= 3 * nphi ? 3 : 2) : 1) : 0;
+    h.n0 = n - h.a * nphi;
+    const int m0 = m - h.b * nphi;
+    int xn = h.n0 + t, xm = m0 - t;
+    h.wn = xn < 0 || xn >= nphi;
+    h.wm = xm < 0 || xm >= nphi;
+    xn += xn < 0 ? nphi : (xn >= nphi ? -nphi : 0);
+    xm += xm < 0 ? nphi : (xm >= nphi ? -nphi : 0);
+    h.xm = xm;
+    h.nt = h.a * nphi + xn;
+    h.mt = h.b * nphi + xm;
+    const uint64_t rest = L ^ ((1ULL << n) | (1ULL << m));
+    const uint64_t tgt = (1ULL << h.nt) | (1ULL << h.mt);
+    h.word = rest | tgt;
+    return h.nt != h.mt && !(rest & tgt);
+}
+
 // Two-body term: pair (n < m) occupied in L, shift t.  Returns false when the
 // hop is blocked or its target has no representative.
 // BLQH_without_spin/lanczos_hamiltonian.cpp:109-292, BLQH_with_spin/lanczos_hamiltonian.cpp:109-712.
 BLQH_HD bool pair_term(const RowCtx& c, uint64_t L, int signl, int Dl, int ql, int n, int m, int t, int& j, double& re,
                        double& im) {
     const int nphi = c.g.nphi;
-    // blocks (layer / species) of n and m and the shifted orbitals, wrapped
-    // within the block (|t| < nphi): compares instead of integer division
-    const int a = n >= nphi ? (n >= 2 * nphi ? (n >= 3 * nphi ? 3 : 2) : 1) : 0;
-    const int b = m >= nphi ? (m >= 2 * nphi ? (m >= 3 * nphi ? 3 : 2) : 1) : 0;
-    const int n0 = n - a * nphi, m0 = m - b * nphi;
-    int xn = n0 + t, xm = m0 - t;
-    const bool wn = xn < 0 || xn >= nphi, wm = xm < 0 || xm >= nphi;
-    xn += xn < 0 ? nphi : (xn >= nphi ? -nphi : 0);
-    xm += xm < 0 ? nphi : (xm >= nphi ? -nphi : 0);
-    const int nt = a * nphi + xn;
-    const int mt = b * nphi + xm;
-    const uint64_t rest = L ^ ((1ULL << n) | (1ULL << m));
-    const uint64_t tgt = (1ULL << nt) | (1ULL << mt);
-    if (nt == mt || (rest & tgt)) return false;
+    PairHop h;
+    if (!pair_hop(nphi, L, n, m, t, h)) return false;
+    const int a = h.a, b = h.b, n0 = h.n0, xm = h.xm, nt = h.nt, mt = h.mt;
     int qr, Dr, sr;
-    if (!locate(c, rest | tgt, ql, j, qr, Dr, sr)) return false;
+    if (!locate(c, h.word, ql, j, qr, Dr, sr)) return false;
     // The monolayer counts occupied orbitals between n and nt (and m and mt)
     // where sign2 counts empty ones (Monolayer/basis.cpp:169-195).  The counts
     // add up to |nt - n| - 1 and |mt - m| - 1, whose sum is even unless
     // exactly one of the two electrons wraps, and then has the parity of nphi.
-    const int sign = sign2(L, n, m, nt, mt) * signl * sr * (c.mono_odd && wn != wm ? -1 : 1);
+    const int sign = sign2(L, n, m, nt, mt) * signl * sr * (c.mono_odd && h.wn != h.wm ? -1 : 1);
     const int sh = c.g.S == 4 ? 1 : 0;  // layer of block a: a / (S / 2)
     const int alpha = (a >> sh) != (b >> sh) ? 1 : 0;
     int dm = xm - n0;
