@@ -2,12 +2,14 @@
 //
 // Pipeline (reference: basis::init, BLQH_without_spin/basis.cpp:61-120 and
 // BLQH_with_spin/basis.cpp:74-141):
-//  1. host: insert the sector words into a std::unordered_map<unsigned long,
-//     long> in the reference's depth-first order and read back the map's
-//     iteration order.  The reference keeps, for every translation orbit, the
-//     member that comes last in that order (it erases a word while iterating
-//     whenever another orbit member is still present), so this order is the
-//     only input the representative choice needs.
+//  1. The reference inserts the sector words into a
+//     std::unordered_map<unsigned long, long> in depth-first order and keeps,
+//     for every translation orbit, the member that comes last in the map's
+//     iteration order (it erases a word while iterating whenever another
+//     orbit member is still present), so that order is the only input the
+//     representative choice needs.  The host derives the container's rehash
+//     schedule; the device generates the words in insertion order and
+//     computes every word's list position level by level (map_positions).
 //  2. device: combinatorial rank of every word -> its position in that order.
 //  3. device: survivor test per word (no orbit member later in the order),
 //     spin-up filter, compaction in rank order = ascending word order (the
@@ -155,13 +157,6 @@ __global__ void k_copy_total(const int64_t* __restrict__ tot, int64_t* __restric
 
 // ---- basis kernels -----------------------------------------------------------
 
-// posOf[rank(order[p])] = p
-__global__ void k_scatter_pos(const uint64_t* __restrict__ order, int64_t M, RankSpec s,
-                              const uint32_t* __restrict__ cnt, int32_t* __restrict__ posOf) {
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < M; p += (int64_t)gridDim.x * blockDim.x)
-        posOf[rank_word(s, cnt, order[p])] = (int32_t)p;
-}
-
 // flags[r] = 1 when word r is the orbit member latest in the hash order (or
 // when no reduction is requested) and passes the spin-up filter.
 __global__ void k_survivor(int64_t M, RankSpec s, const uint32_t* __restrict__ cnt,
@@ -215,6 +210,86 @@ __global__ void k_lut(int64_t n, const uint64_t* __restrict__ id, const int16_t*
             lut[rank_word(s, cnt, y)] = lut_pack(j, q);
         }
     }
+}
+
+// ---- the map's iteration order, level by level -----------------------------
+//
+// libstdc++'s unordered_map keeps its nodes in one singly linked list in which
+// the nodes of a bucket are contiguous.  An insert puts the node at the front
+// of its bucket's run, or at the front of the list when the bucket is empty;
+// a rehash walks the old list from the front and applies the same rule with
+// the new bucket count.  Give every node a time: its position in the list
+// before the last rehash (0, 1, ...), or its insertion index for a node
+// inserted after it (larger than every old position).  Then the list orders
+// the buckets by decreasing earliest time of their nodes and the nodes of a
+// bucket by decreasing time, so after each level of the rehash schedule
+//   pos(x) = (nodes of the buckets whose earliest time is later than that of
+//             x's bucket) + (nodes of x's bucket with a later time than x),
+// which the three kernels below compute in parallel.
+
+// key[i] = i-th word of the depth-first insertion order.  That order is
+// lexicographic in the ascending orbital list, i.e. descending in the
+// bit-reversed word, so it is the reversed sector (srev) unranked from the top.
+__global__ void k_dfs_keys(int64_t M, RankSpec srev, const uint32_t* __restrict__ cnt, int nbits,
+                           uint64_t* __restrict__ key) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x)
+        key[i] = __builtin_bitreverse64(unrank_word(srev, cnt, M - 1 - i)) >> (64 - nbits);
+}
+
+// Level step 1 (nodes [0, en), of which [st, en) are inserted after the
+// rehash to m buckets): time of the new nodes, bucket, bucket size and
+// earliest time.
+__global__ void k_hl_bucket(int64_t st, int64_t en, uint64_t m, const uint64_t* __restrict__ key,
+                            uint32_t* __restrict__ tau, uint32_t* __restrict__ bucket, int32_t* __restrict__ gcount,
+                            uint32_t* __restrict__ gmin) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < en; i += (int64_t)gridDim.x * blockDim.x) {
+        if (i >= st) tau[i] = (uint32_t)i;
+        const uint32_t b = (uint32_t)(key[i] % m);
+        bucket[i] = b;
+        atomicAdd(&gcount[b], 1);
+        atomicMin(&gmin[b], tau[i]);
+    }
+}
+
+// Step 2 (goff = exclusive scan of the bucket sizes): each bucket's nodes,
+// in any order, and the size of every bucket at the slot of its earliest
+// time counted from the latest time down (first_sz[en - 1 - time]).
+// gcount returns to zero.
+__global__ void k_hl_members(int64_t en, const uint32_t* __restrict__ tau, const uint32_t* __restrict__ bucket,
+                             const int64_t* __restrict__ goff, int32_t* __restrict__ gcount,
+                             const uint32_t* __restrict__ gmin, uint32_t* __restrict__ members,
+                             int32_t* __restrict__ first_sz) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < en; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = bucket[i];
+        members[goff[b] + atomicSub(&gcount[b], 1) - 1] = (uint32_t)i;
+        first_sz[en - 1 - tau[i]] = tau[i] == gmin[b] ? (int32_t)(goff[b + 1] - goff[b]) : 0;
+    }
+}
+
+// Step 3 (later = exclusive scan of first_sz: later[en - 1 - t] counts the
+// nodes of the buckets whose earliest time exceeds t): list positions.
+__global__ void k_hl_pos(int64_t en, const uint32_t* __restrict__ tau, const uint32_t* __restrict__ bucket,
+                         const int64_t* __restrict__ goff, const uint32_t* __restrict__ gmin,
+                         const uint32_t* __restrict__ members, const int64_t* __restrict__ later,
+                         uint32_t* __restrict__ tau_out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < en; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = bucket[i], t = tau[i];
+        int64_t p = later[en - 1 - gmin[b]];
+        for (int64_t k = goff[b]; k < goff[b + 1]; ++k) p += tau[members[k]] > t ? 1 : 0;
+        tau_out[i] = (uint32_t)p;
+    }
+}
+
+// posOf[rank(key[i])] = final list position of node i
+__global__ void k_scatter_tau(const uint64_t* __restrict__ key, const uint32_t* __restrict__ tau, int64_t M,
+                              RankSpec s, const uint32_t* __restrict__ cnt, int32_t* __restrict__ posOf) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M; i += (int64_t)gridDim.x * blockDim.x)
+        posOf[rank_word(s, cnt, key[i])] = (int32_t)tau[i];
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 // ---- host helpers ------------------------------------------------------------
@@ -356,6 +431,34 @@ std::vector<uint64_t> hash_iteration_order_map(const blqh_sector& s) {
     return order;
 }
 
+// The sector of the bit-reversed words (bit p -> nbits - 1 - p): the upper
+// layer moves to the high half and orbital x of a block to nphi - 1 - x, so
+// the momentum residual becomes nel (nphi - 1) - J.  smap has no spin-up
+// constraint.
+RankSpec reversed_rank_spec(const blqh_sector& s, const RankSpec& smap) {
+    RankSpec srev = smap;
+    if (smap.maskA) srev.maskA = lowbits(smap.nbits) & ~lowbits(smap.nbits / 2);
+    if (smap.R > 1) srev.Jt = ((s.nel * (s.nphi - 1) - smap.Jt) % s.nphi + s.nphi) % s.nphi;
+    return srev;
+}
+
+// Rehash schedule of the map while n words are inserted one by one: (index
+// of the first insert after the rehash, bucket count), from the container's
+// initial single bucket.
+std::vector<std::pair<int64_t, uint64_t>> hash_levels(int64_t n) {
+    std::__detail::_Prime_rehash_policy policy;  // default max load factor 1
+    std::vector<std::pair<int64_t, uint64_t>> lv{{0, 1}};
+    uint64_t nbkt = 1;
+    for (int64_t i = 0; i < n; ++i) {
+        const std::pair<bool, std::size_t> rh = policy._M_need_rehash(nbkt, (std::size_t)i, 1);
+        if (rh.first) {
+            nbkt = rh.second;
+            lv.push_back({i, nbkt});
+        }
+    }
+    return lv;
+}
+
 void exclusive_scan(const int32_t* in, int64_t* out, int64_t n, hipStream_t st) {
     const int64_t nb = std::max<int64_t>(1, ceil_div(n, kScanBlock));
     DevBuf<int64_t> sums(nb + 1);
@@ -365,6 +468,56 @@ void exclusive_scan(const int32_t* in, int64_t* out, int64_t n, hipStream_t st) 
     hipLaunchKernelGGL(k_copy_total, dim3(1), dim3(1), 0, st, sums.get() + nb, out + n);
     BLQH_HIP(hipGetLastError());
     BLQH_HIP(hipStreamSynchronize(st));  // sums is freed on return
+}
+
+// posOf[rank(w)] = position of sector word w in the map's iteration order
+// (see the level kernels above).  smap/cnt: the sector without the spin-up
+// constraint, M its size.
+void map_positions(const blqh_sector& s, const RankSpec& smap, const uint32_t* cnt, int64_t M, int32_t* posOf,
+                   hipStream_t st) {
+    BLQH_REQUIRE(M < ((int64_t)1 << 31) - 2, BLQH_ERANGE, "sector too large for the hash order");
+    const RankSpec srev = reversed_rank_spec(s, smap);
+    int64_t Mrev = 0;
+    const std::vector<uint32_t> trev = make_rank_table(srev, &Mrev);
+    BLQH_REQUIRE(Mrev == M, BLQH_EINTERNAL, "reversed sector size mismatch");
+    const std::vector<std::pair<int64_t, uint64_t>> lv = hash_levels(M);
+    uint64_t mmax = 1;
+    for (const auto& l : lv) mmax = std::max(mmax, l.second);
+    BLQH_REQUIRE(mmax < ((uint64_t)1 << 32), BLQH_ERANGE, "bucket count beyond 32 bits");
+    DevBuf<uint32_t> dtrev(trev.size());
+    BLQH_HIP(hipMemcpyAsync(dtrev.get(), trev.data(), trev.size() * 4, hipMemcpyHostToDevice, st));
+    DevBuf<uint64_t> key(M);
+    DevBuf<uint32_t> tau(M), tau2(M), bucket(M), members(M), gmin(mmax);
+    DevBuf<int32_t> gcount(mmax), first_sz(M);
+    DevBuf<int64_t> goff(mmax + 1), later(M + 1);
+    hipLaunchKernelGGL(k_dfs_keys, dim3(grid_for(M)), dim3(kBlock), 0, st, M, srev, dtrev.get(), smap.nbits,
+                       key.get());
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for((int64_t)mmax)), dim3(kBlock), 0, st, gcount.get(), (int64_t)mmax, 0);
+    BLQH_HIP(hipGetLastError());
+    uint32_t* cur = tau.get();
+    uint32_t* nxt = tau2.get();
+    for (size_t r = 0; r < lv.size(); ++r) {
+        const int64_t st_r = lv[r].first, en = r + 1 < lv.size() ? lv[r + 1].first : M;
+        const uint64_t m = lv[r].second;
+        if (en == 0) continue;  // rehash before the first insert
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((int64_t)m)), dim3(kBlock), 0, st, gmin.get(), (int64_t)m,
+                           0xFFFFFFFFu);
+        hipLaunchKernelGGL(k_hl_bucket, dim3(grid_for(en)), dim3(kBlock), 0, st, st_r, en, m, key.get(), cur,
+                           bucket.get(), gcount.get(), gmin.get());
+        BLQH_HIP(hipGetLastError());
+        exclusive_scan(gcount.get(), goff.get(), (int64_t)m, st);
+        hipLaunchKernelGGL(k_hl_members, dim3(grid_for(en)), dim3(kBlock), 0, st, en, cur, bucket.get(), goff.get(),
+                           gcount.get(), gmin.get(), members.get(), first_sz.get());
+        BLQH_HIP(hipGetLastError());
+        exclusive_scan(first_sz.get(), later.get(), en, st);
+        hipLaunchKernelGGL(k_hl_pos, dim3(grid_for(en)), dim3(kBlock), 0, st, en, cur, bucket.get(), goff.get(),
+                           gmin.get(), members.get(), later.get(), nxt);
+        BLQH_HIP(hipGetLastError());
+        std::swap(cur, nxt);
+    }
+    hipLaunchKernelGGL(k_scatter_tau, dim3(grid_for(M)), dim3(kBlock), 0, st, key.get(), cur, M, smap, cnt, posOf);
+    BLQH_HIP(hipGetLastError());
+    BLQH_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace blqh
@@ -392,17 +545,12 @@ void blqh_basis_build_impl(const blqh_sector& s, blqh_basis* B) {
     auto t0 = std::chrono::steady_clock::now();
     DevBuf<int32_t> posOf;
     if (reduce) {
-        // host: the map's iteration order after inserting in depth-first order
-        std::vector<uint64_t> order = hash_iteration_order(s);
-        BLQH_REQUIRE((int64_t)order.size() == Mmap, BLQH_EINTERNAL, "enumeration size mismatch");
-        B->seconds_host = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        DevBuf<uint64_t> dorder(Mmap);
+        // every word's position in the map's iteration order after the
+        // depth-first inserts (the rehash schedule on the host, the list
+        // positions level by level on the device)
         posOf.alloc(Mmap);
-        BLQH_HIP(hipMemcpyAsync(dorder.get(), order.data(), Mmap * 8, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_scatter_pos, dim3(grid_for(Mmap)), dim3(kBlock), 0, st, dorder.get(), Mmap, smap,
-                           dmap.get(), posOf.get());
-        BLQH_HIP(hipGetLastError());
-        BLQH_HIP(hipStreamSynchronize(st));
+        map_positions(s, smap, dmap.get(), Mmap, posOf.get(), st);
+        B->seconds_host = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     auto t1 = std::chrono::steady_clock::now();
     DevBuf<int32_t> flags(Mmap);

@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cstdint>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "bits.hpp"
@@ -90,6 +91,9 @@ std::vector<uint32_t> make_rank_table(const RankSpec& spec, int64_t* total);
 void exclusive_scan(const int32_t* in, int64_t* out, int64_t n, hipStream_t st);  // out has n+1 entries
 std::vector<uint64_t> hash_iteration_order(const blqh_sector& s);
 std::vector<uint64_t> hash_iteration_order_map(const blqh_sector& s);  // from a real std::unordered_map
+// rehash schedule of the map over n inserts: (first insert after the rehash, buckets)
+std::vector<std::pair<int64_t, uint64_t>> hash_levels(int64_t n);
+RankSpec reversed_rank_spec(const blqh_sector& s, const RankSpec& smap);  // bit-reversed words
 
 // Event timing of the fused Lanczos SpMV (blqh_profile_*).
 struct Profile {

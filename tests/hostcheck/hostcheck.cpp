@@ -69,6 +69,57 @@ void build(const blqh_sector& s, HostBasis& B) {
         }
     }
 }
+// The device's level-by-level computation of the map's iteration order
+// (basis.hip k_dfs_keys, k_hl_bucket, k_hl_members, k_hl_pos), step by step
+// with sequential loops; returns the words in list order.
+std::vector<uint64_t> level_order(const blqh_sector& s) {
+    const RankSpec smap = make_rank_spec(s, false);
+    int64_t M = 0;
+    (void)make_rank_table(smap, &M);
+    const RankSpec srev = reversed_rank_spec(s, smap);
+    int64_t Mrev = 0;
+    const std::vector<uint32_t> trev = make_rank_table(srev, &Mrev);
+    if (Mrev != M) return {};
+    std::vector<uint64_t> key(M);
+    for (int64_t i = 0; i < M; ++i)
+        key[i] = __builtin_bitreverse64(unrank_word(srev, trev.data(), M - 1 - i)) >> (64 - smap.nbits);
+    const auto lv = hash_levels(M);
+    std::vector<uint32_t> tau(M), tau2(M), bucket(M), members(M), gmin;
+    std::vector<int32_t> gcount, first_sz(M);
+    std::vector<int64_t> goff, later(M + 1);
+    for (size_t r = 0; r < lv.size(); ++r) {
+        const int64_t st = lv[r].first, en = r + 1 < lv.size() ? lv[r + 1].first : M;
+        const uint64_t m = lv[r].second;
+        if (en == 0) continue;
+        gcount.assign(m, 0);
+        gmin.assign(m, 0xFFFFFFFFu);
+        for (int64_t i = 0; i < en; ++i) {
+            if (i >= st) tau[i] = (uint32_t)i;
+            bucket[i] = (uint32_t)(key[i] % m);
+            ++gcount[bucket[i]];
+            gmin[bucket[i]] = std::min(gmin[bucket[i]], tau[i]);
+        }
+        goff.assign(m + 1, 0);
+        for (uint64_t b = 0; b < m; ++b) goff[b + 1] = goff[b] + gcount[b];
+        for (int64_t i = 0; i < en; ++i) {
+            const uint32_t b = bucket[i];
+            members[goff[b] + --gcount[b]] = (uint32_t)i;
+            first_sz[en - 1 - tau[i]] = tau[i] == gmin[b] ? (int32_t)(goff[b + 1] - goff[b]) : 0;
+        }
+        later[0] = 0;
+        for (int64_t u = 0; u < en; ++u) later[u + 1] = later[u] + first_sz[u];
+        for (int64_t i = 0; i < en; ++i) {
+            const uint32_t b = bucket[i], t = tau[i];
+            int64_t p = later[en - 1 - gmin[b]];
+            for (int64_t k = goff[b]; k < goff[b + 1]; ++k) p += tau[members[k]] > t ? 1 : 0;
+            tau2[i] = (uint32_t)p;
+        }
+        tau.swap(tau2);
+    }
+    std::vector<uint64_t> out(M);
+    for (int64_t i = 0; i < M; ++i) out[tau[i]] = key[i];
+    return out;
+}
 }  // namespace
 
 extern "C" {
@@ -203,6 +254,14 @@ void hc_internal_order(const blqh_sector* s, const uint64_t* ids, int64_t n, int
 // of a real std::unordered_map filled the same way; *n = words.
 int hc_hash_order_check(const blqh_sector* s, int64_t* n) {
     const std::vector<uint64_t> a = hash_iteration_order(*s);
+    const std::vector<uint64_t> b = hash_iteration_order_map(*s);
+    *n = (int64_t)a.size();
+    return a == b ? 1 : 0;
+}
+
+// The level-by-level order (the device's algorithm) against a real map.
+int hc_hash_order_levels_check(const blqh_sector* s, int64_t* n) {
+    const std::vector<uint64_t> a = level_order(*s);
     const std::vector<uint64_t> b = hash_iteration_order_map(*s);
     *n = (int64_t)a.size();
     return a == b ? 1 : 0;

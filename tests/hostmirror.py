@@ -26,6 +26,8 @@ def lib():
         _L.hc_internal_order.argtypes = [C.POINTER(blqh_amd.Sector), C.c_void_p, C.c_int64, C.c_void_p]
         _L.hc_hash_order_check.restype = C.c_int
         _L.hc_hash_order_check.argtypes = [C.POINTER(blqh_amd.Sector), C.POINTER(C.c_int64)]
+        _L.hc_hash_order_levels_check.restype = C.c_int
+        _L.hc_hash_order_levels_check.argtypes = [C.POINTER(blqh_amd.Sector), C.POINTER(C.c_int64)]
         _L.hc_mod_by_rcp.restype = None
         _L.hc_mod_by_rcp.argtypes = [C.c_int, C.c_int, C.c_float, C.c_void_p]
         _L.hc_xcd_lists.restype = None
@@ -94,4 +96,11 @@ def hash_order_check(sec):
     """(replayed hash-map order == real std::unordered_map order, words)."""
     n = C.c_int64()
     ok = lib().hc_hash_order_check(C.byref(sec), C.byref(n))
+    return bool(ok), n.value
+
+
+def hash_order_levels_check(sec):
+    """(level-by-level order, the device algorithm == real map order, words)."""
+    n = C.c_int64()
+    ok = lib().hc_hash_order_levels_check(C.byref(sec), C.byref(n))
     return bool(ok), n.value

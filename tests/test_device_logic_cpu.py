@@ -153,6 +153,23 @@ def test_hash_order_replay_equals_the_real_map(case):
     assert n > 0 and ok, (case, n)
 
 
+@pytest.mark.parametrize("case", [
+    (2, 8, 8, 4, -1, 4, 4), (2, 8, 8, -1, -1, 3, 2), (2, 10, 10, 5, -1, 1, 7), (2, 9, 6, -1, -1, 2, 1),
+    (2, 12, 12, -1, -1, 1, 2), (2, 14, 14, -1, -1, 0, 0), (2, 14, 14, 7, -1, 7, 7), (2, 16, 16, 8, -1, 0, 0),
+    (4, 6, 6, 3, 3, 0, 1), (4, 6, 6, -1, -1, 1, 2), (4, 8, 8, 4, 4, 3, 5), (4, 4, 4, 2, 2, 1, 1),
+    (1, 12, 4, -1, -1, 3, 3), (1, 18, 6, -1, -1, 0, 0), (1, 9, 3, -1, -1, 4, 2), (2, 5, 1, -1, -1, 2, 0)],
+    ids=lambda c: "_".join(map(str, c)))
+def test_hash_order_levels_equal_the_real_map(case):
+    """The device computes the map's iteration order without the list: words
+    generated in insertion order from the bit-reversed sector, then list
+    positions level by level of the container's rehash schedule (buckets by
+    decreasing earliest time, nodes of a bucket by decreasing time).  The
+    same steps run sequentially here and must give a real std::unordered_map's
+    order, word for word, on spinless, spinful and monolayer sectors."""
+    ok, n = hostmirror.hash_order_levels_check(blqh_amd.Sector(*case))
+    assert n > 0 and ok, (case, n)
+
+
 MONO = [(9, 3, 4, 2), (9, 3, 1, -1), (12, 4, 3, 3), (12, 6, 1, 2), (10, 5, 2, 1), (8, 4, 1, 1), (7, 3, 2, 0),
         (18, 6, 3, 3), (11, 4, 5, 0), (14, 4, 0, 1)]
 
