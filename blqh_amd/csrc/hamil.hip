@@ -4,15 +4,18 @@
 // BLQH_without_spin/lanczos_hamiltonian.cpp:37-385 (spinless) and
 // BLQH_with_spin/lanczos_hamiltonian.cpp:37-954 (spinful).
 //
-// Row i of H (one wavefront per row):
-//   for every translation ql < D_l of id[i], every occupied pair (n < m) and
+// Row i of H (one wavefront per row) sums the reference's terms: for every
+//   translation L_ql of id[i] (ql < D_l), every occupied pair (n < m) and
 //   every shift t in (-nphi, nphi): n -> nt = n + t, m -> mt = m - t (both
 //   wrapped inside their species block); when both targets are empty the
 //   target word R0 is looked up through its combinatorial rank in the sector
 //   table, which gives the representative row j, the shift qr with
-//   inv_translate(R0, qr) = id[j] and that translation's sign.  The element
-//   FT[ql][qr] * V[alpha][|dm|][|t|] * sign / sqrt(D_l D_r) is accumulated
-//   into an LDS hash keyed by j.  Tunnelling terms are one-body hops between
+//   inv_translate(R0, qr) = id[j] and that translation's sign, and the
+//   element FT[ql][qr] * V[alpha][|dm|][|t|] * sign / sqrt(D_l D_r) goes into
+//   an LDS hash keyed by j.  The kernel visits them per open hop of id[i]:
+//   the translations map those hops one to one onto the hops of every L_ql,
+//   with targets in the same orbit, so one lookup serves the D_l images,
+//   whose sum is inserted once.  Tunnelling terms are one-body hops between
 //   the layers; the diagonal (background, charging, bias, Zeeman) comes last.
 // Entries with |h_ij| > 1e-10 are kept, columns ascending.
 //
@@ -273,6 +276,8 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
     __shared__ unsigned long long sbuf[MODE >= 1 && CAP > 512 ? CAP : 1];
     __shared__ int occ[64];
     __shared__ int sQ[128];  // open pair hops awaiting evaluation
+    __shared__ uint64_t sLw[32];  // translate(L0, ql)
+    __shared__ int sLs[32];       // and its sign
     __shared__ int s_row, s_ovf;
 
     const int lane = threadIdx.x;
@@ -307,18 +312,24 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
         bool ovf = false;
         const uint64_t L0 = A.id[i];
         const int Dl = ctx.K < 0 ? 1 : ctx.period[i];
-        for (int ql = 0; ql < Dl; ++ql) {
-            int signl;
-            const uint64_t L = translate(g, L0, ql, signl);
-            if (lane < nbits && ((L >> lane) & 1ULL)) occ[popc(L & lowbits(lane))] = lane;
-            __syncthreads();
-            // Candidates cnd = pr * T + tt (pair pr, shift tt - nphi + 1).  At
-            // half filling only about a third of them are open (both targets
-            // empty), and a wavefront that evaluated them in place would run
-            // the whole target lookup for nearly every group of 64.  So the
-            // open ones are queued in candidate order (sQ, packed pr << 8 | tt)
-            // and evaluated 64 at a time; the hash receives the terms in the
-            // same order as an in-place loop would give them.
+        if (lane < Dl) {  // Dl <= nphi < 32
+            int sg;
+            sLw[lane] = translate(g, L0, lane, sg);
+            sLs[lane] = sg;
+        }
+        if (lane < nbits && ((L0 >> lane) & 1ULL)) occ[popc(L0 & lowbits(lane))] = lane;
+        __syncthreads();
+        {
+            // Pair hops of the row word L0: candidate cnd = pr * T + tt (pair
+            // pr, shift tt - nphi + 1).  A translation maps the open hops of L0
+            // one to one onto those of every translated word L_ql, with
+            // targets in the same orbit, so each open hop of L0 is located
+            // once and its Dl images are summed in registers (pair_orbit_sum)
+            // before one hash insert.  At half filling only about a third of
+            // the candidates are open and a wavefront evaluating them in place
+            // would run the lookup for nearly every group of 64, so the open
+            // ones are first queued in candidate order (sQ, packed
+            // pr << 8 | tt) and evaluated 64 at a time.
             const int ncand = (A.terms & 1) ? npairs * T : 0;
             const int d0 = 64 / T, d1 = 64 % T;  // advance by 64 candidates without division
             int pr = lane / T, tt = lane % T;
@@ -329,7 +340,7 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
                     if (next + lane < ncand) {
                         const unsigned short pw = sPair[pr];
                         PairHop h;
-                        open = pair_hop(nphi, L, occ[pw & 0xff], occ[pw >> 8], tt - (nphi - 1), h);
+                        open = pair_hop(nphi, L0, occ[pw & 0xff], occ[pw >> 8], tt - (nphi - 1), h);
                     }
                     const unsigned long long bal = __ballot(open);
                     if (open)
@@ -357,19 +368,20 @@ __global__ void __launch_bounds__(64) k_build(BuildArgs A) {
                     const unsigned short pw = sPair[e >> 8];
                     int j;
                     double re, im;
-                    if (pair_term(ctx, L, signl, Dl, ql, occ[pw & 0xff], occ[pw >> 8], (e & 0xff) - (nphi - 1), j,
-                                  re, im))
+                    if (pair_orbit_sum(ctx, sLw, sLs, Dl, occ[pw & 0xff], occ[pw >> 8], (e & 0xff) - (nphi - 1), j,
+                                       re, im))
                         if (!hash_add<CAP>(keys, vre, vim, j, re, im)) ovf = true;
                 }
             }
+        }
+        for (int ql = 0; ql < Dl; ++ql)
             for (int h = lane; h < A.nhop; h += 64) {
                 int j;
                 double re, im;
-                if (hop_term(ctx, L, signl, Dl, ql, A.hops[h], j, re, im))
+                if (hop_term(ctx, sLw[ql], sLs[ql], Dl, ql, A.hops[h], j, re, im))
                     if (!hash_add<CAP>(keys, vre, vim, j, re, im)) ovf = true;
             }
-            __syncthreads();
-        }
+        __syncthreads();
         // diagonal
         if (lane == 0 && (A.terms & 1)) {
             const double dg = diag_term(g, ctx.spinless != 0, L0, A.Ecn, A.d, A.dv, A.dz);

@@ -74,6 +74,19 @@ BLQH_HD bool is_survivor(const Geo& g, const RankSpec& s, const T* cnt, const in
 // shift q with inv_translate(y, q) == id[j] as a word.
 BLQH_HD int32_t lut_pack(int64_t j, int q) { return (int32_t)((j << 5) | q); }
 
+// Row j of the orbit of sector word Rs, the table's shift q1
+// (inv_translate(Rs, q1) == id[j]) and j's period Dr.
+BLQH_HD bool locate_word(const RowCtx& c, uint64_t Rs, int& j, int& q1, int& Dr) {
+    const int64_t rk = rank_word(c.spec, c.cnt, Rs);
+    if (rk < 0 || rk >= c.M) return false;  // defensive: unreachable for sector words
+    const int32_t e = c.lut[rk];
+    if (e < 0) return false;
+    j = e >> 5;
+    q1 = e & 31;
+    Dr = c.K < 0 ? 1 : c.period[j];
+    return true;
+}
+
 // Representative of a hop target.  L = translate(id[i], ql) carries momentum
 // J + ql*nel, so R0 (same momentum) is first rotated back by ql into the
 // sector; the table then gives (j, q1) and the reference's shift is
@@ -81,21 +94,15 @@ BLQH_HD int32_t lut_pack(int64_t j, int q) { return (int32_t)((j << 5) | q); }
 // id[j] (lanczos_hamiltonian.cpp:147-165).  Its sign is recomputed for qr:
 // rotating by the period itself may carry a fermion sign.
 BLQH_HD bool locate(const RowCtx& c, uint64_t R0, int ql, int& j, int& qr, int& Dr, int& sr) {
-    int s0;
+    int s0, q1;
     const uint64_t Rs = ql ? inv_translate(c.g, R0, ql, s0) : R0;
-    const int64_t rk = rank_word(c.spec, c.cnt, Rs);
-    if (rk < 0 || rk >= c.M) return false;  // defensive: unreachable for sector words
-    const int32_t e = c.lut[rk];
-    if (e < 0) return false;
-    j = e >> 5;
+    if (!locate_word(c, Rs, j, q1, Dr)) return false;
     if (c.K < 0) {
-        Dr = 1;
         qr = 0;
         sr = 1;
         return true;
     }
-    Dr = c.period[j];
-    qr = mod_small(ql + (e & 31), Dr);
+    qr = mod_small(ql + q1, Dr);
     (void)inv_translate(c.g, R0, qr, sr);
     return true;
 }
@@ -132,17 +139,18 @@ BLQH_HD bool pair_hop(int nphi, uint64_t L, int n, int m, int t, PairHop& h) {
     return h.nt != h.mt && !(rest & tgt);
 }
 
-// Two-body term: pair (n < m) occupied in L, shift t.  Returns false when the
-// hop is blocked or its target has no representative.
+// Two-body element of the open pair hop h of L (pair n < m, shift t) whose
+// target lies in the orbit of row j (sector-table shift q1, period Dr).
 // BLQH_without_spin/lanczos_hamiltonian.cpp:109-292, BLQH_with_spin/lanczos_hamiltonian.cpp:109-712.
-BLQH_HD bool pair_term(const RowCtx& c, uint64_t L, int signl, int Dl, int ql, int n, int m, int t, int& j, double& re,
-                       double& im) {
+BLQH_HD void pair_value(const RowCtx& c, uint64_t L, int signl, int Dl, int ql, int n, int m, int t, const PairHop& h,
+                        int q1, int Dr, double& re, double& im) {
     const int nphi = c.g.nphi;
-    PairHop h;
-    if (!pair_hop(nphi, L, n, m, t, h)) return false;
     const int a = h.a, b = h.b, n0 = h.n0, xm = h.xm, nt = h.nt, mt = h.mt;
-    int qr, Dr, sr;
-    if (!locate(c, h.word, ql, j, qr, Dr, sr)) return false;
+    int qr = 0, sr = 1;
+    if (c.K >= 0) {
+        qr = mod_small(ql + q1, Dr);
+        (void)inv_translate(c.g, h.word, qr, sr);
+    }
     // The monolayer counts occupied orbitals between n and nt (and m and mt)
     // where sign2 counts empty ones (Monolayer/basis.cpp:169-195).  The counts
     // add up to |nt - n| - 1 and |mt - m| - 1, whose sum is even unless
@@ -166,6 +174,63 @@ BLQH_HD bool pair_term(const RowCtx& c, uint64_t L, int signl, int Dl, int ql, i
         const double cs = Cv * sign;
         re = (cs * ft.x) / sq;
         im = (cs * ft.y) / sq;
+    }
+}
+
+// Two-body term: pair (n < m) occupied in L = translate(id[i], ql), shift t.
+// Returns false when the hop is blocked or its target has no representative.
+BLQH_HD bool pair_term(const RowCtx& c, uint64_t L, int signl, int Dl, int ql, int n, int m, int t, int& j, double& re,
+                       double& im) {
+    PairHop h;
+    if (!pair_hop(c.g.nphi, L, n, m, t, h)) return false;
+    int s0, q1, Dr;
+    const uint64_t Rs = ql ? inv_translate(c.g, h.word, ql, s0) : h.word;
+    if (!locate_word(c, Rs, j, q1, Dr)) return false;
+    pair_value(c, L, signl, Dl, ql, n, m, t, h, q1, Dr, re, im);
+    return true;
+}
+
+// Orbital x moved by k (0 <= k < nphi) within its block, with wrap-around
+// (the translation's action on one orbital).
+BLQH_HD int rot_in_block(int nphi, int x, int k) {
+    const int a = x >= nphi ? (x >= 2 * nphi ? (x >= 3 * nphi ? 3 : 2) : 1) : 0;
+    int y = x - a * nphi + k;
+    y -= y >= nphi ? nphi : 0;
+    return a * nphi + y;
+}
+
+// The pair hop (n0 < m0 of the row word L0 = Ls[0], shift t) and its images
+// in the translated words Ls[ql] = translate(L0, ql) (signs sgl[ql]), ql < Dl.
+// A translation maps the open hops of L0 one to one onto those of Ls[ql] (the
+// image moves the rotated orbitals; when the rotation swaps their order the
+// image is the pair (m, n) with shift -t), and maps every hop target onto the
+// same orbit, so the target is located once.  Returns the row j and the sum
+// of the Dl elements, ql ascending, each computed as pair_term computes the
+// image's term in Ls[ql]; false when the hop is blocked or its target has no
+// representative.
+BLQH_HD bool pair_orbit_sum(const RowCtx& c, const uint64_t* Ls, const int* sgl, int Dl, int n0, int m0, int t, int& j,
+                            double& re, double& im) {
+    const int nphi = c.g.nphi;
+    PairHop h;
+    if (!pair_hop(nphi, Ls[0], n0, m0, t, h)) return false;
+    int q1, Dr;
+    if (!locate_word(c, h.word, j, q1, Dr)) return false;
+    re = 0.0;
+    im = 0.0;
+    for (int ql = 0; ql < Dl; ++ql) {
+        const int k = ql * c.g.step;
+        int n = rot_in_block(nphi, n0, k), m = rot_in_block(nphi, m0, k), tq = t;
+        if (n > m) {
+            const int x = n;
+            n = m;
+            m = x;
+            tq = -t;
+        }
+        if (ql) (void)pair_hop(nphi, Ls[ql], n, m, tq, h);  // open: the image of an open hop
+        double r, i;
+        pair_value(c, Ls[ql], sgl[ql], Dl, ql, n, m, tq, h, q1, Dr, r, i);
+        re += r;
+        im += i;
     }
     return true;
 }

@@ -169,8 +169,14 @@ void hc_xcd_lists(int P, const int64_t* item0, const int64_t* nnz, const int32_t
 }
 int hc_sign1(uint64_t c, int n, int nt) { return sign1(c, n, nt); }
 
-// Full Hamiltonian through the mirror.  Returns nnz (or -nnz when cap is too small).
-int64_t hc_hamil(const blqh_sector* s, const blqh_params* p, int64_t* rowptr, int64_t* col, double* val, int64_t cap) {
+// Full Hamiltonian through the mirror.  Returns nnz (or -nnz when cap is too
+// small).  ql_major = 0: the build kernel's order (each open pair hop of the
+// row word with its translated images summed by pair_orbit_sum, then the
+// tunnelling hops per translation, then the diagonal); 1: the reference's
+// loop order (translation, then pair and shift, through pair_term), which
+// checks that regrouping.
+static int64_t hamil_mirror(const blqh_sector* s, const blqh_params* p, int64_t* rowptr, int64_t* col, double* val,
+                            int64_t cap, int ql_major) {
     HostBasis B;
     build(*s, B);
     HostTables T = host_tables(*s, *p);
@@ -191,34 +197,55 @@ int64_t hc_hamil(const blqh_sector* s, const blqh_params* p, int64_t* rowptr, in
     const int64_t n = (int64_t)B.id.size();
     std::vector<int64_t> rp(n + 1, 0), cl;
     std::vector<double> vl;
+    auto occ_of = [&](uint64_t L, int* occ) {
+        int k = 0;
+        for (int b = 0; b < c.g.S * nphi; ++b)
+            if ((L >> b) & 1ULL) occ[k++] = b;
+    };
     for (int64_t i = 0; i < n; ++i) {
         std::map<int, std::pair<double, double>> acc;
         const uint64_t L0 = B.id[i];
         const int Dl = c.K < 0 ? 1 : B.period[i];
-        for (int ql = 0; ql < Dl; ++ql) {
-            int signl;
-            const uint64_t L = translate(c.g, L0, ql, signl);
-            int occ[64], k = 0;
-            for (int b = 0; b < c.g.S * nphi; ++b)
-                if ((L >> b) & 1ULL) occ[k++] = b;
+        uint64_t Ls[64];
+        int sgl[64];
+        for (int ql = 0; ql < Dl; ++ql) Ls[ql] = translate(c.g, L0, ql, sgl[ql]);
+        auto add = [&](int j, double re, double im) {
+            acc[j].first += re;
+            acc[j].second += im;
+        };
+        if (ql_major) {
+            for (int ql = 0; ql < Dl; ++ql) {
+                int occ[64];
+                occ_of(Ls[ql], occ);
+                for (int cnd = 0; cnd < npairs * Tn; ++cnd) {
+                    const int pr = cnd / Tn, t = cnd - pr * Tn - (nphi - 1);
+                    const unsigned short pw = pair_decode(nel, pr);
+                    int j;
+                    double re, im;
+                    if (pair_term(c, Ls[ql], sgl[ql], Dl, ql, occ[pw & 0xff], occ[pw >> 8], t, j, re, im)) add(j, re, im);
+                }
+                for (const HopTerm& h : T.hops) {
+                    int j;
+                    double re, im;
+                    if (hop_term(c, Ls[ql], sgl[ql], Dl, ql, h, j, re, im)) add(j, re, im);
+                }
+            }
+        } else {
+            int occ[64];
+            occ_of(L0, occ);
             for (int cnd = 0; cnd < npairs * Tn; ++cnd) {
                 const int pr = cnd / Tn, t = cnd - pr * Tn - (nphi - 1);
                 const unsigned short pw = pair_decode(nel, pr);
                 int j;
                 double re, im;
-                if (pair_term(c, L, signl, Dl, ql, occ[pw & 0xff], occ[pw >> 8], t, j, re, im)) {
-                    acc[j].first += re;
-                    acc[j].second += im;
-                }
+                if (pair_orbit_sum(c, Ls, sgl, Dl, occ[pw & 0xff], occ[pw >> 8], t, j, re, im)) add(j, re, im);
             }
-            for (const HopTerm& h : T.hops) {
-                int j;
-                double re, im;
-                if (hop_term(c, L, signl, Dl, ql, h, j, re, im)) {
-                    acc[j].first += re;
-                    acc[j].second += im;
+            for (int ql = 0; ql < Dl; ++ql)
+                for (const HopTerm& h : T.hops) {
+                    int j;
+                    double re, im;
+                    if (hop_term(c, Ls[ql], sgl[ql], Dl, ql, h, j, re, im)) add(j, re, im);
                 }
-            }
         }
         acc[(int)i].first += diag_term(c.g, c.spinless != 0, L0, T.Ec * nel, p->d, p->delta_v, p->delta_z);
         for (auto& kv : acc)
@@ -235,6 +262,15 @@ int64_t hc_hamil(const blqh_sector* s, const blqh_params* p, int64_t* rowptr, in
     std::memcpy(col, cl.data(), nnz * 8);
     std::memcpy(val, vl.data(), nnz * 16);
     return nnz;
+}
+
+int64_t hc_hamil(const blqh_sector* s, const blqh_params* p, int64_t* rowptr, int64_t* col, double* val, int64_t cap) {
+    return hamil_mirror(s, p, rowptr, col, val, cap, 0);
+}
+
+int64_t hc_hamil_ql_major(const blqh_sector* s, const blqh_params* p, int64_t* rowptr, int64_t* col, double* val,
+                          int64_t cap) {
+    return hamil_mirror(s, p, rowptr, col, val, cap, 1);
 }
 
 // The SpMV formats' internal order (panel.hip internal_order) of a basis given

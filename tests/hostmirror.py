@@ -20,6 +20,8 @@ def lib():
         _L.hc_hamil.restype = C.c_int64
         _L.hc_hamil.argtypes = [C.POINTER(blqh_amd.Sector), C.POINTER(blqh_amd.Params), C.c_void_p, C.c_void_p,
                                 C.c_void_p, C.c_int64]
+        _L.hc_hamil_ql_major.restype = C.c_int64
+        _L.hc_hamil_ql_major.argtypes = _L.hc_hamil.argtypes
         _L.hc_sector_words.restype = C.c_int64
         _L.hc_sector_words.argtypes = [C.POINTER(blqh_amd.Sector), C.c_int, C.c_void_p, C.c_int64]
         _L.hc_internal_order.restype = None
@@ -53,6 +55,17 @@ def hamil(sec, par, n, cap):
     col = np.empty(cap, np.int64)
     val = np.empty(2 * cap)
     nnz = lib().hc_hamil(C.byref(sec), C.byref(par), _p(rp), _p(col), _p(val), cap)
+    assert nnz >= 0, "capacity too small"
+    return rp, col[:nnz].copy(), val[:2 * nnz].view(np.complex128).copy()
+
+
+def hamil_ql_major(sec, par, n, cap):
+    """The mirror's Hamiltonian in the reference's loop order (translation,
+    then pair and shift) instead of the build kernel's grouping."""
+    rp = np.empty(n + 1, np.int64)
+    col = np.empty(cap, np.int64)
+    val = np.empty(2 * cap)
+    nnz = lib().hc_hamil_ql_major(C.byref(sec), C.byref(par), _p(rp), _p(col), _p(val), cap)
     assert nnz >= 0, "capacity too small"
     return rp, col[:nnz].copy(), val[:2 * nnz].view(np.complex128).copy()
 
