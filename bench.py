@@ -20,6 +20,7 @@ solves against plain single-problem solves, under "sweeps".
 """
 import argparse
 import json
+from concurrent.futures import ThreadPoolExecutor
 import os
 import sys
 import time
@@ -296,6 +297,30 @@ def run_sweep(args):
             launches, kms, kbytes = blqh_amd.profile_read()
             nnz_total = sum(H.nnz for H in hs)
         del hs
+    # pipelined leg: the timed points again, each point's Hamiltonians built on
+    # a second host thread (its own library stream) while the previous point
+    # solves, so the builds overlap the batched solves on the GPU
+    pipe = None
+    if args.pipeline and len(order) > nwarm:
+        pts_t = order[nwarm:]
+        ex = ThreadPoolExecutor(1)
+        t = time.perf_counter()
+        fut = ex.submit(build, *pts_t[0])
+        psolved = 0
+        for k, (d, ds) in enumerate(pts_t):
+            hs, _ = fut.result()
+            if k + 1 < len(pts_t):
+                fut = ex.submit(build, *pts_t[k + 1])
+            prev = batch.hams
+            batch.rebind(hs)
+            res = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False)
+            del prev, hs
+            psolved += sum(int(r["converged"] == 1 and r["dE"] < 1e-10) for r in res)
+        wall = time.perf_counter() - t
+        ex.shutdown()
+        pipe = {"points": len(pts_t), "converged": psolved, "seconds": wall, "rate": psolved / wall}
+        log(f"pipelined builds and solves: {len(pts_t)} points, {psolved} solves in {wall:.3f} s "
+            f"({psolved / wall:.1f}/s end to end)")
     avg_ms = kms / max(launches, 1)
     achieved = (kbytes / max(launches, 1)) / (avg_ms * 1e-3) / 1e9 if launches else 0.0
     value = solved / t_solve if t_solve > 0 else 0.0
@@ -329,6 +354,8 @@ def run_sweep(args):
             "batched_over_plain": value / plain["rate"] if plain and plain["rate"] else None,
             "plain": plain,
             "end_to_end_rate_incl_build": solved / (t_solve + t_build) if t_solve + t_build > 0 else None,
+            "end_to_end_rate_pipelined": pipe["rate"] if pipe else None,
+            "pipelined": pipe,
             "continuation_rate": cont["solved"] / cont["seconds"] if cont["seconds"] > 0 else None,
             "continuation_iterations_mean": float(np.mean(cont["iters"])) if cont["iters"] else None,
             "continuation_max_rel_dE0_vs_cold": cont["max_rel_dE0"] if cont["iters"] else None,
@@ -475,6 +502,32 @@ def run_sweep_shared(args):
             launches, kms, kbytes = blqh_amd.profile_read()
             nnz_total = sum(H.nnz for H in base) + sum(H.nnz for H in tun)
         del base  # the batch keeps its own copy of H(d,0) + T
+    # pipelined leg: the timed d values again, each d's H(d,0) built on a
+    # second host thread (its own library stream) while the previous d solves
+    pipe = None
+    if args.pipeline and len(order) > nwarm:
+        pts_t = order[nwarm:]
+        ex = ThreadPoolExecutor(1)
+
+        def build_base(d):
+            return [blqh_amd.Hamiltonian(B, par(d), terms=1) for B in bases]
+
+        t = time.perf_counter()
+        fut = ex.submit(build_base, pts_t[0])
+        psolved = 0
+        for k, d in enumerate(pts_t):
+            base = fut.result()
+            if k + 1 < len(pts_t):
+                fut = ex.submit(build_base, pts_t[k + 1])
+            batch.rebind_scaled(base, tun)
+            del base
+            res = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False)
+            psolved += sum(int(r["converged"] == 1 and r["dE"] < 1e-10) for r in res)
+        wall = time.perf_counter() - t
+        ex.shutdown()
+        pipe = {"points": len(pts_t) * G, "converged": psolved, "seconds": wall, "rate": psolved / wall}
+        log(f"pipelined builds and solves: {len(pts_t)} d values, {psolved} solves in {wall:.3f} s "
+            f"({psolved / wall:.1f}/s end to end)")
     avg_ms = kms / max(launches, 1)
     achieved = (kbytes / max(launches, 1)) / (avg_ms * 1e-3) / 1e9 if launches else 0.0
     value = solved / t_solve if t_solve > 0 else 0.0
@@ -509,6 +562,8 @@ def run_sweep_shared(args):
             "batched_over_plain": value / plain["rate"] if plain and plain["rate"] else None,
             "plain": plain,
             "end_to_end_rate_incl_build": solved / (t_solve + t_build) if t_solve + t_build > 0 else None,
+            "end_to_end_rate_pipelined": pipe["rate"] if pipe else None,
+            "pipelined": pipe,
             "continuation_rate": cont["solved"] / cont["seconds"] if cont["seconds"] > 0 else None,
             "continuation_iterations_mean": float(np.mean(cont["iters"])) if cont["iters"] else None,
             "continuation_max_rel_dE0_vs_cold": cont["max_rel_dE0"] if cont["iters"] else None,
@@ -723,6 +778,9 @@ def main():
     ap.add_argument("--shared", type=int, default=1,
                     help="sweeps with several DSAS values per d: one shared-scale batch per d (1) or one batch "
                          "per parameter point (0)")
+    ap.add_argument("--pipeline", type=int, default=1,
+                    help="sweeps: also time the timed points with each point's builds on a second host thread, "
+                         "overlapping the previous point's solve (1) or not (0)")
     ap.add_argument("--continuation", type=int, default=0,
                     help="sweeps: also solve every point from the previous point's ground states (1) or not (0)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -858,6 +916,7 @@ def main():
                 "warm_start": "previous parameter point's ground state + 0.1 x seeded unit vector",
                 "warm_solves_above_cold_E0": c["continuation_mismatches"],
                 "end_to_end_rate_incl_build": c["end_to_end_rate_incl_build"],
+                "end_to_end_rate_pipelined": c["end_to_end_rate_pipelined"],
                 "max_rel_dE0_batched_vs_plain": c["plain"]["max_rel_dE0_vs_batched"] if c["plain"] else None,
                 "roofline": {k: r["roofline"][k] for k in ("achieved", "peak", "unit", "frac", "traffic", "kernel",
                                                            "avg_launch_ms", "algorithmic_bytes_per_launch")},

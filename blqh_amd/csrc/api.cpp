@@ -4,6 +4,7 @@
 
 #include <cstring>
 #include <list>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -100,10 +101,26 @@ void init_device(int dev) {
     BLQH_HIP(hipGetDeviceProperties(&prop, dev));
     g_rt.device = dev;
     g_rt.num_cus = prop.multiProcessorCount;
-    BLQH_HIP(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
     blqh::noalloc_setup(dev);  // device address of the allocation-free solver's static scratch
     g_rt_ok = true;
 }
+
+// Library streams: one per host thread (non-blocking, default priority),
+// taken on the thread's first library call that needs one and handed back to
+// a free list when the thread ends.  Streams are never destroyed: the HIP
+// runtime may already be gone when the main thread's thread-locals end.
+std::mutex g_streams_mu;
+std::vector<hipStream_t>* g_free_streams = new std::vector<hipStream_t>;  // never destroyed
+struct ThreadStream {
+    hipStream_t s = nullptr;
+    ~ThreadStream() {
+        if (!s) return;
+        std::lock_guard<std::mutex> lk(g_streams_mu);
+        g_free_streams->push_back(s);
+        s = nullptr;
+    }
+};
+thread_local ThreadStream t_stream;
 }  // namespace
 
 namespace blqh {
@@ -111,53 +128,98 @@ Runtime& runtime() {
     if (!g_rt_ok) init_device(-1);
     return g_rt;
 }
-hipStream_t stream() { return runtime().stream; }
+hipStream_t stream() {
+    if (t_stream.s) return t_stream.s;
+    const Runtime& rt = runtime();
+    BLQH_HIP(hipSetDevice(rt.device));  // a new host thread starts on device 0
+    std::lock_guard<std::mutex> lk(g_streams_mu);
+    if (!g_free_streams->empty()) {
+        t_stream.s = g_free_streams->back();
+        g_free_streams->pop_back();
+    } else {
+        BLQH_HIP(hipStreamCreateWithFlags(&t_stream.s, hipStreamNonBlocking));
+    }
+    return t_stream.s;
+}
 
 namespace {
+// Released device blocks, oldest first, with an event recorded on the
+// releasing thread's stream: a block goes to a new owner once every kernel
+// queued before its release has finished (hipEventSynchronize on that event),
+// so another thread's queued work is not waited for.  Every size is kept
+// (hipFree waits for the whole device, which would stall a solve on another
+// thread); at most a quarter of device memory and kMaxBlocks blocks.
+struct CachedBlock {
+    size_t bytes;
+    void* p;
+    hipEvent_t ev;
+};
 struct BlockCache {
     std::mutex mu;
-    std::list<std::pair<size_t, void*>> blocks;  // oldest first
+    std::list<CachedBlock> blocks;              // oldest first
+    std::multimap<size_t, std::list<CachedBlock>::iterator> by_size;
+    std::vector<hipEvent_t> spare_events;
     size_t bytes = 0, cap = 0;
 };
+constexpr size_t kMaxBlocks = 4096;
 BlockCache& block_cache() {
     static BlockCache* c = new BlockCache;  // never destroyed: no hipFree after the runtime shuts down
     return *c;
 }
+void cache_erase(BlockCache& c, std::list<CachedBlock>::iterator it) {  // caller holds c.mu
+    auto r = c.by_size.equal_range(it->bytes);
+    for (auto m = r.first; m != r.second; ++m)
+        if (m->second == it) {
+            c.by_size.erase(m);
+            break;
+        }
+    c.bytes -= it->bytes;
+    c.spare_events.push_back(it->ev);
+    c.blocks.erase(it);
+}
+void cache_drop_oldest(BlockCache& c) {  // caller holds c.mu
+    void* p = c.blocks.front().p;
+    cache_erase(c, c.blocks.begin());
+    (void)hipFree(p);
+}
 void cache_drop_all(BlockCache& c) {  // caller holds c.mu
-    for (auto& b : c.blocks) (void)hipFree(b.second);
-    c.blocks.clear();
-    c.bytes = 0;
+    while (!c.blocks.empty()) cache_drop_oldest(c);
 }
 }  // namespace
 
 void* dev_alloc(size_t bytes) {
+    (void)stream();  // binds the calling thread to the library's device
     BlockCache& c = block_cache();
-    if (bytes >= kCacheMin) {
-        void* hit = nullptr;
-        {
-            std::lock_guard<std::mutex> lk(c.mu);
-            for (auto it = c.blocks.begin(); it != c.blocks.end(); ++it)
-                if (it->first == bytes) {
-                    hit = it->second;
-                    c.bytes -= bytes;
-                    c.blocks.erase(it);
-                    break;
-                }
-        }
-        if (hit) {
-            BLQH_HIP(hipStreamSynchronize(stream()));
-            return hit;
+    void* hit = nullptr;
+    hipEvent_t ev = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        auto m = c.by_size.find(bytes);
+        if (m != c.by_size.end()) {
+            hit = m->second->p;
+            ev = m->second->ev;
+            cache_erase(c, m->second);  // the event goes to the spares only after the wait below
+            c.spare_events.pop_back();
         }
     }
+    if (hit) {
+        const hipError_t e = hipEventSynchronize(ev);
+        {
+            std::lock_guard<std::mutex> lk(c.mu);
+            c.spare_events.push_back(ev);
+        }
+        BLQH_HIP(e);
+        return hit;
+    }
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
+    hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 1));
     if (e != hipSuccess) {
         (void)hipGetLastError();
         {
             std::lock_guard<std::mutex> lk(c.mu);
             cache_drop_all(c);
         }
-        e = hipMalloc(&p, bytes);
+        e = hipMalloc(&p, std::max<size_t>(bytes, 1));
     }
     if (e != hipSuccess)
         throw Error(BLQH_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
@@ -167,22 +229,34 @@ void* dev_alloc(size_t bytes) {
 void dev_free(void* p, size_t bytes) noexcept {
     if (!p) return;
     BlockCache& c = block_cache();
-    if (bytes >= kCacheMin && g_rt_ok) {
+    if (g_rt_ok) {
+        hipStream_t st = nullptr;
+        try {
+            st = stream();
+        } catch (...) {
+            st = nullptr;
+        }
         std::lock_guard<std::mutex> lk(c.mu);
         if (c.cap == 0) {
             size_t fr = 0, tot = 0;
             if (hipMemGetInfo(&fr, &tot) == hipSuccess) c.cap = tot / 4;
         }
-        if (bytes <= c.cap) {
-            while (c.bytes + bytes > c.cap && !c.blocks.empty()) {
-                (void)hipFree(c.blocks.front().second);
-                c.bytes -= c.blocks.front().first;
-                c.blocks.pop_front();
-            }
-            c.blocks.emplace_back(bytes, p);
+        hipEvent_t ev = nullptr;
+        if (!c.spare_events.empty()) {
+            ev = c.spare_events.back();
+            c.spare_events.pop_back();
+        } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            ev = nullptr;
+        }
+        if (st && ev && bytes <= c.cap && hipEventRecord(ev, st) == hipSuccess) {
+            while ((c.bytes + bytes > c.cap || c.blocks.size() >= kMaxBlocks) && !c.blocks.empty())
+                cache_drop_oldest(c);
+            c.blocks.push_back({bytes, p, ev});
+            c.by_size.emplace(bytes, std::prev(c.blocks.end()));
             c.bytes += bytes;
             return;
         }
+        if (ev) c.spare_events.push_back(ev);
     }
     (void)hipFree(p);
 }

@@ -29,15 +29,14 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::blqh::Error((code), (msg)); \
     } while (0)
 
-// Device memory of DevBuf.  Blocks of kCacheMin bytes or more are kept after
-// release and handed out again for the same size: sweeps free and
-// re-allocate the same matrix sizes at every parameter point, and hipMalloc /
-// hipFree of large blocks (page mapping; hipFree also waits for the device)
-// cost more than the kernels that fill them.  The cache holds at most a
-// quarter of device memory (oldest blocks go first) and is emptied when an
-// allocation fails.  A reused block is handed out after the library stream
-// has drained, so no kernel queued before its release still reads it.
-constexpr size_t kCacheMin = size_t(1) << 20;
+// Device memory of DevBuf.  Released blocks are kept and handed out again
+// for the same size: sweeps free and re-allocate the same matrix sizes at
+// every parameter point, hipMalloc of large blocks costs more than the kernels
+// that fill them, and hipFree waits for the whole device (a solve running on
+// another host thread's stream included).  A block is handed out again once
+// the work queued on the releasing thread's stream before its release is
+// done.  The cache holds at most a quarter of device memory and 4096 blocks
+// (oldest go first) and is emptied when an allocation fails.
 void* dev_alloc(size_t bytes);                 // throws Error(BLQH_ENOMEM)
 void dev_free(void* p, size_t bytes) noexcept;
 size_t dev_cached_bytes();                     // free for the library: counted as free memory
@@ -76,10 +75,12 @@ struct DevBuf {
 // The library's stream and device (one GPU per process).
 struct Runtime {
     int device = -1;
-    hipStream_t stream = nullptr;
     int num_cus = 0;
 };
 Runtime& runtime();   // initialises on first use
+// The calling host thread's library stream (each thread has its own, so
+// builds on one thread can overlap solves on another).  A call returns after
+// its work is done, or leaves it queued on this stream only.
 hipStream_t stream();
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -356,8 +356,9 @@ blqh_mlp* blqh_mlp_create_impl(int nbits, uint32_t seed) {
         M->v.alloc(n);
         M->lsum.alloc(1);
         copy_sync(M->P.get(), h.data(), n * 4, hipMemcpyHostToDevice);
-        BLQH_HIP(hipMemset(M->m.get(), 0, n * 4));
-        BLQH_HIP(hipMemset(M->v.get(), 0, n * 4));
+        BLQH_HIP(hipMemsetAsync(M->m.get(), 0, n * 4, stream()));
+        BLQH_HIP(hipMemsetAsync(M->v.get(), 0, n * 4, stream()));
+        BLQH_HIP(hipStreamSynchronize(stream()));
     } catch (...) {
         delete M;
         throw;
@@ -375,8 +376,9 @@ void blqh_mlp_get_params_impl(const blqh_mlp* M, float* out) {
 
 void blqh_mlp_set_params_impl(blqh_mlp* M, const float* in) {
     copy_sync(M->P.get(), in, M->D.n * 4, hipMemcpyHostToDevice);
-    BLQH_HIP(hipMemset(M->m.get(), 0, M->D.n * 4));
-    BLQH_HIP(hipMemset(M->v.get(), 0, M->D.n * 4));
+    BLQH_HIP(hipMemsetAsync(M->m.get(), 0, M->D.n * 4, stream()));
+    BLQH_HIP(hipMemsetAsync(M->v.get(), 0, M->D.n * 4, stream()));
+    BLQH_HIP(hipStreamSynchronize(stream()));
     M->adam_t = 0;
 }
 

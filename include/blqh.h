@@ -7,8 +7,12 @@
  *    blqh_last_error() returns the calling thread's last message.
  *  - Handles are opaque and owned by the library; output buffers are owned by
  *    the caller.  Complex arrays are interleaved (re, im) doubles.
- *  - One process drives one GPU through one HIP stream.  A handle must not be
- *    used from two threads at once.
+ *  - One process drives one GPU.  Each host thread queues its library work
+ *    on a HIP stream of its own (non-blocking, default priority), so work
+ *    issued from two threads can overlap on the GPU (a build on one, a
+ *    solve on the other).  A call returns when its results are complete,
+ *    except blqh_spmv_device, which leaves its work queued on the calling
+ *    thread's stream.  A handle must not be used from two threads at once.
  *  - No exceptions cross this boundary.
  */
 #ifndef BLQH_H
@@ -66,8 +70,8 @@ typedef struct blqh_hamil blqh_hamil;
 
 const char* blqh_last_error(void);
 
-/* Select the GPU (device index) and create the library stream.  Optional:
- * every other entry point initialises device 0 on first use. */
+/* Select the GPU (device index).  Optional: every other entry point
+ * initialises device 0 on first use. */
 int blqh_init(int device);
 
 /* ---- basis: basis::init + id[] / basis_C[]  (BLQH_without_spin/basis.cpp:61-120,
@@ -366,6 +370,7 @@ int blqh_device_copy(const void* src_dev, void* dst_dev, int64_t bytes);
 /* Device allocation helpers for callers without a HIP runtime of their own. */
 int blqh_device_alloc(void** ptr, int64_t bytes);
 int blqh_device_free(void* ptr);
+/* Wait for the work the calling thread has queued (its library stream). */
 int blqh_synchronize(void);
 /* Release the device blocks the library keeps for reuse.  Engine memory
  * management, no reference counterpart: blocks of 1 MiB or more freed by the
