@@ -36,9 +36,9 @@ HBM_PEAK_GBS = 8000.0
 # passes, scaled by the same counters on a plain 1 GiB copy (profiles/);
 # reported only when the live run uses the profiled workload and variant
 MEASURED_TRAFFIC = {
-    # k_jag<true, true, 8, 8>, internal row order, greedy steps: 6,544,086 KiB FETCH_SIZE x 2.0 + 36,721.5 KiB
-    # WRITE_SIZE x 1.0 per dispatch (profiles/k_jag_nphi16_counters.md)
-    ("nphi16_all", (3, 64, 512, 2, 8)): (6544086 * 2.0 * 1024 + 36721.5 * 1024,
+    # k_jag<true, true, 8, 8>, internal row order, plain step order: 6,477,611.5 KiB FETCH_SIZE x 2.0 +
+    # 36,706.9 KiB WRITE_SIZE x 1.0 per dispatch (profiles/k_jag_nphi16_counters.md)
+    ("nphi16_all", (3, 64, 512, 2, 8)): (6477611.5 * 2.0 * 1024 + 36706.9 * 1024,
                                          "profiles/k_jag_nphi16_counters.md"),
 }
 # HBM bytes per batched K1 launch of the sweeps (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, mean over the

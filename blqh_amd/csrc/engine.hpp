@@ -127,6 +127,7 @@ bool build_tiles(blqh_hamil* H, int G);
 // temporary and synchronises the library stream.
 // rows > 1: step-major with that many rows per lane (plain order), for SpMV kind 5.
 void set_panel_layout(blqh_hamil* H, bool jagged, void* scratch = nullptr, size_t scratch_bytes = 0, int rows = 1);
+bool greedy_step_order();  // single-matrix jagged panels in the greedy step order (BLQH_JAGGED_ORDER=greedy)
 size_t panel_layout_scratch_bytes(const blqh_hamil* H);
 // The same for many matrices in one launch (those already in the layout are skipped).
 void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scratch, size_t scratch_bytes,

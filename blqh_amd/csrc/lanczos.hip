@@ -1834,7 +1834,7 @@ Plan noalloc_plan(const blqh_hamil* H) {
             case 0: ok = true; break;
             case 1:
             case 2: ok = P.built && !P.jagged; break;
-            case 3: ok = P.built && P.jagged && P.greedy && P.rows == 1; break;
+            case 3: ok = P.built && P.jagged && P.greedy == greedy_step_order() && P.rows == 1; break;
             case 4: ok = H->til.built; break;
             case 5: ok = P.built && P.jagged && P.rows == p.lpr; break;
             default: break;

@@ -492,14 +492,11 @@ constexpr size_t kAlign = 256;
 
 size_t greedy_lds_bytes(int cap) { return (size_t)cap * (8 + 2 + 2 + 2 + 1); }
 
-// Most entries per run that the greedy jagged order can hold in LDS (0: plain
-// order, BLQH_JAGGED_ORDER=plain): 8192 (120 KiB of LDS per workgroup) where
-// the device allows it, else 4096 (60 KiB).
+// Most entries per run that the greedy jagged order can hold in LDS: 8192
+// (120 KiB of LDS per workgroup) where the device allows it, else 4096 (60 KiB).
 int greedy_cap_max() {
     static int cap = -1;
     if (cap >= 0) return cap;
-    const char* env = std::getenv("BLQH_JAGGED_ORDER");
-    if (env && std::strcmp(env, "plain") == 0) return cap = 0;
     int dev = 0, maxlds = 0;
     BLQH_HIP(hipGetDevice(&dev));
     BLQH_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
@@ -512,9 +509,10 @@ int greedy_cap_max() {
     return cap;
 }
 
-// Runs of up to kGreedySmall entries are ordered with 30 KiB of LDS per
-// workgroup (several workgroups per CU); larger ones in a second launch.
-constexpr int kGreedySmall = 2048;
+// Runs are ordered in size classes (kGreedyFirst, doubling to kGreedySmall
+// entries, then the rest up to the cap), each launch with the LDS its class
+// needs (3.75 KiB for 256 entries, 30 KiB for 2048).
+constexpr int kGreedyFirst = 256, kGreedySmall = 2048;
 
 // LDS capacity for the large runs of these jobs: the smallest power of two
 // >= 64 holding their largest run, at most greedy_cap_max() (larger runs keep
@@ -607,13 +605,17 @@ void relayout_pass(const std::vector<blqh_hamil*>& todo, bool jagged, bool greed
     } else if (cap > 0) {
         err.alloc(1);
         BLQH_HIP(hipMemsetAsync(err.get(), 0, sizeof(int), st));
+        // one launch per size class of runs, each with the LDS of its largest
+        // run: the kernel is bound by LDS latency and barriers, so smaller
+        // classes run with more workgroups per CU
         const int gg = (int)std::max<int64_t>(1, std::min<int64_t>(max_runs, 8192));
-        const int small = std::min(cap, kGreedySmall);
-        hipLaunchKernelGGL(k_relayout_greedy, dim3(gg, (unsigned)jobs.size()), dim3(64), greedy_lds_bytes(small), st,
-                           dj, 0, small, 0, err.get());
-        if (cap > small)
-            hipLaunchKernelGGL(k_relayout_greedy, dim3(gg, (unsigned)jobs.size()), dim3(64), greedy_lds_bytes(cap),
-                               st, dj, small, cap, 0, err.get());
+        int lo = 0;
+        for (int c = kGreedyFirst; lo < cap; c = c < kGreedySmall ? c * 2 : cap) {
+            const int hi = std::min(c, cap);
+            hipLaunchKernelGGL(k_relayout_greedy, dim3(gg, (unsigned)jobs.size()), dim3(64), greedy_lds_bytes(hi),
+                               st, dj, lo, hi, 0, err.get());
+            lo = hi;
+        }
         if (max_e > (unsigned)cap)  // runs too large for LDS keep the plain order
             hipLaunchKernelGGL(k_relayout_runs, dim3(gx, (unsigned)jobs.size()), dim3(kBlk), 0, st, dj, 1,
                                (int64_t)cap);
@@ -677,8 +679,18 @@ void set_panel_layouts(blqh_hamil* const* hs, int count, bool jagged, void* scra
     if (!todo.empty()) relayout_pass(todo, jagged, greedy, rows, scratch, scratch_bytes);
 }
 
+// Step order of a single matrix's jagged panels: the plain order unless
+// BLQH_JAGGED_ORDER=greedy (read at every layout change).  With the internal
+// row order in place the greedy order no longer pays: measured on the 2.35M-row
+// N_phi = 16 sector, its fused K1 is 3.06 ms against 2.93 ms in the plain
+// order, and the greedy relayout costs 2.85 s once per matrix.
+bool greedy_step_order() {
+    const char* env = std::getenv("BLQH_JAGGED_ORDER");
+    return env && std::strcmp(env, "greedy") == 0;
+}
+
 void set_panel_layout(blqh_hamil* H, bool jagged, void* scratch, size_t scratch_bytes, int rows) {
-    set_panel_layouts(&H, 1, jagged, scratch, scratch_bytes, true, rows);
+    set_panel_layouts(&H, 1, jagged, scratch, scratch_bytes, greedy_step_order(), rows);
 }
 
 size_t panel_layout_scratch_bytes(const blqh_hamil* H) {

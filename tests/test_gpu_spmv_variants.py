@@ -50,7 +50,7 @@ def test_variant_matches_csr_product(built, plan):
 
 
 def test_layout_round_trip(built):
-    """Kind 3 reads the panels in greedy step-major order, kind 5 in step-major
+    """Kind 3 reads the panels in step-major order, kind 5 in step-major
     order with R rows per lane, kinds 1-2 in row-major order; switching back
     and forth reorders the entries on the device and must give bit-identical
     products each time, whatever the order of the switches."""
@@ -120,3 +120,37 @@ def test_rows_per_lane_on_small_matrices(R, u):
         e = H.lanczos(lam=300, seed=1)["E0"]
         H.set_spmv(0, 16)
         assert e == pytest.approx(H.lanczos(lam=300, seed=1)["E0"], rel=1e-11)
+
+
+def test_greedy_step_order_on_request(built, monkeypatch):
+    """BLQH_JAGGED_ORDER=greedy puts a single matrix's step-major panels in the
+    greedy step order (at each step the lanes are served line by line, the x
+    cache line wanted by most first); the plain step order is the default.
+    Both give the CSR product, each order reproduces its own products bit for
+    bit after a switch away and back, and the CSR is untouched."""
+    name, B, H, A = built
+    print(f"{name} greedy step order", flush=True)
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=H.n) + 1j * rng.normal(size=H.n)
+    ref = A @ x
+    scale = np.abs(A).max() * np.abs(x).max() * np.diff(A.indptr).max()
+    monkeypatch.setenv("BLQH_JAGGED_ORDER", "greedy")
+    H.set_spmv(2, 64, 512, 2, 2)
+    H.set_spmv(3, 64, 512, 2, 2)
+    yg = H.spmv(x)
+    monkeypatch.delenv("BLQH_JAGGED_ORDER")
+    H.set_spmv(2, 64, 512, 2, 2)
+    H.set_spmv(3, 64, 512, 2, 2)
+    yp = H.spmv(x)
+    np.testing.assert_allclose(yg, ref, rtol=0, atol=2e-15 * scale)
+    np.testing.assert_allclose(yp, ref, rtol=0, atol=2e-15 * scale)
+    monkeypatch.setenv("BLQH_JAGGED_ORDER", "greedy")
+    H.set_spmv(2, 64, 512, 2, 2)
+    H.set_spmv(3, 64, 512, 2, 2)
+    assert np.array_equal(H.spmv(x), yg)
+    monkeypatch.delenv("BLQH_JAGGED_ORDER")
+    H.set_spmv(5, 4, 512, 2, 2)
+    H.set_spmv(3, 64, 512, 2, 2)
+    assert np.array_equal(H.spmv(x), yp)
+    rp, col, val = H.export()
+    assert np.array_equal(rp, A.indptr) and np.array_equal(col, A.indices) and np.array_equal(val, A.data)
