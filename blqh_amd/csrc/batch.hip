@@ -33,6 +33,7 @@
 #include <thread>
 
 #include "engine.hpp"
+#include "mtstart.hpp"
 #include "xcdplan.hpp"
 
 namespace blqh {
@@ -556,6 +557,51 @@ __global__ void k_bnext(int32_t* jbase, int count) {
 }
 
 // psi_r = sum_l coef_l conj(W_l[r]) over the problem's m vectors; |psi|^2 partials
+// Seeded start vectors of a batch on the device: one workgroup per problem
+// draws its std::mt19937 entries (mt_start_block) into gen; k_bstart_scale
+// then writes dst[r] = gen[iperm[r]] / s (the member's internal row order;
+// identity without one), s = sqrt of the problem's entry sum taken on the
+// host as start_vector does: the same values as the host's start vectors.
+struct BStart {
+    int64_t n;
+    uint32_t seed;
+    double2* gen;
+    double2* dst;
+    const int32_t* iperm;
+};
+
+__global__ void __launch_bounds__(1024) k_bmt_start(const BStart* __restrict__ jobs, double* __restrict__ nrm2) {
+    const BStart J = jobs[blockIdx.x];
+    mt_start_block(J.n, J.seed, J.gen, nrm2 + blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kBT) k_bstart_scale(const BStart* __restrict__ jobs, const double* __restrict__ s) {
+    const BStart J = jobs[blockIdx.y];
+    const double sj = s[blockIdx.y];
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < J.n; r += (int64_t)gridDim.x * blockDim.x) {
+        double2 z = J.gen[J.iperm ? J.iperm[r] : r];
+        z.x /= sj;
+        z.y /= sj;
+        J.dst[r] = z;
+    }
+}
+
+// Ground states back to the reference row order for the copy to the host:
+// dst[iperm[r]] = src[r] (a member in its internal order), dst[r] = src[r]
+// otherwise; one job per problem (blockIdx.y).
+struct BPsi {
+    int64_t n;
+    const int32_t* iperm;
+    const double2* src;
+    double2* dst;
+};
+
+__global__ void __launch_bounds__(kBT) k_bpsi_ref(const BPsi* __restrict__ jobs) {
+    const BPsi J = jobs[blockIdx.y];
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < J.n; r += (int64_t)gridDim.x * blockDim.x)
+        J.dst[J.iperm ? J.iperm[r] : r] = J.src[r];
+}
+
 __global__ void __launch_bounds__(kBT) k_bcombine(BState S, const double* __restrict__ coef,
                                                   const int32_t* __restrict__ mvec, double2* __restrict__ psi) {
     __shared__ double sh[kBT / 64];
@@ -686,6 +732,7 @@ struct blqh_batch {
     hipGraphExec_t gexec = nullptr;
     int g_chk = -1, g_lpr = -1, g_cap = -1, g_kinds = -1;
     char* pin = nullptr;
+    double2* psi_pin = nullptr;  // pinned staging of the ground states (b->rows entries, on first use)
     DevBuf<char> relayout;       // scratch for reordering members' panels
     // Shared-scale groups (blqh_batch_create_scaled); G = 0: every member
     // has its own matrix.  Problem p = s G + g is H0[s] + scale[g] T[s];
@@ -703,6 +750,7 @@ struct blqh_batch {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         if (pin) (void)hipHostFree(pin);
+        if (psi_pin) (void)hipHostFree(psi_pin);
         for (blqh_hamil* H : tpl) delete H;
     }
 };
@@ -1170,33 +1218,33 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     BLQH_HIP(hipMemcpyAsync(pl.probs.get(), hp.data(), P * sizeof(BProb), hipMemcpyHostToDevice, st));
 
     phase("setup");
-    // start vectors (host mt19937 or caller's), beta_0 = 1
+    // start vectors, beta_0 = 1: the caller's on the host (normalised, mixed,
+    // permuted), the seeded ones drawn on the device (k_bmt_start)
     {
         std::vector<std::vector<double2>> v0(P);
+        std::vector<char> seeded(P, 0);
+        for (int p = 0; p < P; ++p) seeded[p] = !(starts && starts[p]);
         auto make = [&](int p0, int p1) {
             for (int p = p0; p < p1; ++p) {
+                if (seeded[p]) continue;
                 const int64_t n = hp[p].n;
-                if (starts && starts[p]) {
-                    v0[p].assign((const double2*)starts[p], (const double2*)starts[p] + n);
-                    double s = 0;
-                    for (auto& z : v0[p]) s += z.x * z.x + z.y * z.y;
-                    if (!(s > 0) || !std::isfinite(s)) { v0[p].clear(); continue; }
-                    if (o.start_mix > 0) {  // start/|start| + start_mix * seeded unit vector
-                        std::vector<double2> r;
-                        start_vector(n, seeds ? seeds[p] : o.seed, r);
-                        const double zs = 1.0 / std::sqrt(s);
-                        s = 0;
-                        for (int64_t i = 0; i < n; ++i) {
-                            double2& z = v0[p][i];
-                            z = make_double2(z.x * zs + o.start_mix * r[i].x, z.y * zs + o.start_mix * r[i].y);
-                            s += z.x * z.x + z.y * z.y;
-                        }
+                v0[p].assign((const double2*)starts[p], (const double2*)starts[p] + n);
+                double s = 0;
+                for (auto& z : v0[p]) s += z.x * z.x + z.y * z.y;
+                if (!(s > 0) || !std::isfinite(s)) { v0[p].clear(); continue; }
+                if (o.start_mix > 0) {  // start/|start| + start_mix * seeded unit vector
+                    std::vector<double2> r;
+                    start_vector(n, seeds ? seeds[p] : o.seed, r);
+                    const double zs = 1.0 / std::sqrt(s);
+                    s = 0;
+                    for (int64_t i = 0; i < n; ++i) {
+                        double2& z = v0[p][i];
+                        z = make_double2(z.x * zs + o.start_mix * r[i].x, z.y * zs + o.start_mix * r[i].y);
+                        s += z.x * z.x + z.y * z.y;
                     }
-                    s = std::sqrt(s);
-                    for (auto& z : v0[p]) { z.x /= s; z.y /= s; }
-                } else {
-                    start_vector(n, seeds ? seeds[p] : o.seed, v0[p]);
                 }
+                s = std::sqrt(s);
+                for (auto& z : v0[p]) { z.x /= s; z.y /= s; }
                 const blqh_hamil* Hp = b->hs[p];
                 if (Hp->ord.on) {  // the member's SpMV formats are in its internal order
                     std::vector<double2> w(n);
@@ -1205,18 +1253,54 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
                 }
             }
         };
-        const int nt = std::max(1, std::min<int>(P, (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
-        std::vector<std::thread> th;
-        for (int t = 0; t < nt; ++t) th.emplace_back(make, (int)((int64_t)P * t / nt), (int)((int64_t)P * (t + 1) / nt));
-        for (auto& t : th) t.join();
+        int ncaller = 0;
+        for (int p = 0; p < P; ++p) ncaller += seeded[p] ? 0 : 1;
+        if (ncaller > 0) {
+            const int nt = std::max(1, std::min<int>(ncaller, (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+            std::vector<std::thread> th;
+            for (int t = 0; t < nt; ++t)
+                th.emplace_back(make, (int)((int64_t)P * t / nt), (int)((int64_t)P * (t + 1) / nt));
+            for (auto& t : th) t.join();
+        }
         for (int p = 0; p < P; ++p)
-            BLQH_REQUIRE(!v0[p].empty(), BLQH_EINVAL, "start vector is zero or not finite");
+            BLQH_REQUIRE(seeded[p] || !v0[p].empty(), BLQH_EINVAL, "start vector is zero or not finite");
         for (int p = 0; p < P; ++p)
-            BLQH_HIP(hipMemcpyAsync(hp[p].W, v0[p].data(), hp[p].n * sizeof(double2), hipMemcpyHostToDevice, st));
+            if (!seeded[p])
+                BLQH_HIP(hipMemcpyAsync(hp[p].W, v0[p].data(), hp[p].n * sizeof(double2), hipMemcpyHostToDevice, st));
+        std::vector<BStart> jobs;
+        int64_t nmax = 0;
+        for (int p = 0; p < P; ++p)
+            if (seeded[p]) {
+                const blqh_hamil* Hp = b->hs[p];
+                const bool perm = Hp->ord.on;
+                jobs.push_back({hp[p].n, seeds ? seeds[p] : o.seed, perm ? pl.u.get() + hp[p].voff : hp[p].W, hp[p].W,
+                                perm ? Hp->ord.iperm.get() : nullptr});
+                nmax = std::max(nmax, hp[p].n);
+            }
+        DevBuf<BStart> djobs;
+        DevBuf<double> dnrm;
+        if (!jobs.empty()) {
+            const int J = (int)jobs.size();
+            djobs.alloc(J);
+            dnrm.alloc(2 * (size_t)J);
+            BLQH_HIP(hipMemcpyAsync(djobs.get(), jobs.data(), J * sizeof(BStart), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_bmt_start, dim3(J), dim3(1024), 0, st, djobs.get(), dnrm.get());
+            BLQH_HIP(hipGetLastError());
+            std::vector<double> nrm(J);
+            BLQH_HIP(hipMemcpyAsync(nrm.data(), dnrm.get(), J * sizeof(double), hipMemcpyDeviceToHost, st));
+            BLQH_HIP(hipStreamSynchronize(st));
+            for (double& x : nrm) x = std::sqrt(x);
+            BLQH_HIP(hipMemcpyAsync(dnrm.get() + J, nrm.data(), J * sizeof(double), hipMemcpyHostToDevice, st));
+            const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nmax, kBT), 256));
+            for (int j0 = 0; j0 < J; j0 += 65535)
+                hipLaunchKernelGGL(k_bstart_scale, dim3(gx, (unsigned)std::min(65535, J - j0)), dim3(kBT), 0, st,
+                                   djobs.get() + j0, dnrm.get() + J + j0);
+            BLQH_HIP(hipGetLastError());
+        }
         std::vector<double> ones((size_t)P * ld, 1.0);
         BLQH_HIP(hipMemcpyAsync(pl.beta.get(), ones.data(), ones.size() * 8, hipMemcpyHostToDevice, st));
         BLQH_HIP(hipMemcpyAsync(pl.invb.get(), ones.data(), ones.size() * 8, hipMemcpyHostToDevice, st));
-        BLQH_HIP(hipStreamSynchronize(st));  // v0 and ones are freed at scope exit
+        BLQH_HIP(hipStreamSynchronize(st));  // v0, ones, the job table and the norms are freed at scope exit
     }
     phase("start");
     std::vector<int32_t> act(P, 1);
@@ -1419,14 +1503,25 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     BLQH_HIP(hipStreamSynchronize(st));
     std::vector<double> w((size_t)P * cap, 0.0), E0(P);
     std::vector<int32_t> hm(P);
-    for (int p = 0; p < P; ++p) {
+    for (int p = 0; p < P; ++p)
         BLQH_REQUIRE(m[p] >= 1 && std::isfinite(ritz[p].back()), BLQH_EINTERNAL,
                      "Lanczos produced non-finite coefficients");
-        std::vector<double> c(m[p]);
+    auto pass2 = [&](int p, std::vector<double>& c) {
+        c.resize(m[p]);
         E0[p] = tridiag_lowest(m[p], ha + (size_t)p * cap, hb + (size_t)p * ld + 1, c.data());
         for (int l = 0; l < m[p]; ++l) w[(size_t)p * cap + l] = c[l] * hinvb[(size_t)p * ld + l];
         hm[p] = m[p];
         if (!std::isfinite(resid[p])) resid[p] = hb[(size_t)p * ld + m[p]] * std::fabs(c[m[p] - 1]);
+    };
+    {
+        const int nt = std::min(nthr, (P + 3) / 4);  // per-problem results only: any split gives the same values
+        if (nt <= 1) {
+            for (int p = 0; p < P; ++p) pass2(p, svecs[0]);
+        } else {
+            b->hpool->run(nt, [&](int t) {
+                for (int p = t; p < P; p += nt) pass2(p, svecs[t]);
+            });
+        }
     }
     BLQH_HIP(hipMemcpyAsync(pl.coef.get(), w.data(), w.size() * 8, hipMemcpyHostToDevice, st));
     BLQH_HIP(hipMemcpyAsync(pl.mvec.get(), hm.data(), P * 4, hipMemcpyHostToDevice, st));
@@ -1446,16 +1541,44 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     launch_b1<false>(b, S, 0, pl.psi.get(), pl.u.get(), pl.part2.get(), st);
     BLQH_HIP(hipGetLastError());
     BLQH_HIP(hipMemcpyAsync(hp2.data(), pl.part2.get(), b->nitems * 8, hipMemcpyDeviceToHost, st));
+    // ground states: every wanted one in the reference order into u (free
+    // after the last SpMV), one copy of u into pinned staging, then host
+    // copies into the caller's arrays on the pool threads
+    std::vector<BPsi> pj;
+    int64_t pmax = 0;
     for (int p = 0; p < P; ++p)
         if (psi_out && psi_out[p]) {
-            const double2* src = pl.psi.get() + hp[p].voff;
-            if (b->hs[p]->ord.on) {  // back to the reference order, through u (free after the last SpMV)
-                order_to_reference(b->hs[p], src, pl.u.get() + hp[p].voff, st);
-                src = pl.u.get() + hp[p].voff;
-            }
-            BLQH_HIP(hipMemcpyAsync(psi_out[p], src, hp[p].n * sizeof(double2), hipMemcpyDeviceToHost, st));
+            const blqh_hamil* Hp = b->hs[p];
+            pj.push_back({hp[p].n, Hp->ord.on ? Hp->ord.iperm.get() : nullptr, pl.psi.get() + hp[p].voff,
+                          pl.u.get() + hp[p].voff});
+            pmax = std::max(pmax, hp[p].n);
         }
+    DevBuf<BPsi> dpj;
+    if (!pj.empty()) {
+        if (!b->psi_pin) BLQH_HIP(hipHostMalloc((void**)&b->psi_pin, (size_t)b->rows * sizeof(double2), hipHostMallocDefault));
+        dpj.alloc(pj.size());
+        BLQH_HIP(hipMemcpyAsync(dpj.get(), pj.data(), pj.size() * sizeof(BPsi), hipMemcpyHostToDevice, st));
+        const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(pmax, kBT), 256));
+        for (size_t j0 = 0; j0 < pj.size(); j0 += 65535)
+            hipLaunchKernelGGL(k_bpsi_ref, dim3(gx, (unsigned)std::min<size_t>(65535, pj.size() - j0)), dim3(kBT), 0,
+                               st, dpj.get() + j0);
+        BLQH_HIP(hipGetLastError());
+        BLQH_HIP(hipMemcpyAsync(b->psi_pin, pl.u.get(), (size_t)b->rows * sizeof(double2), hipMemcpyDeviceToHost, st));
+    }
     BLQH_HIP(hipStreamSynchronize(st));
+    if (!pj.empty()) {
+        auto put = [&](int p) {
+            if (psi_out[p]) std::memcpy(psi_out[p], b->psi_pin + hp[p].voff, hp[p].n * sizeof(double2));
+        };
+        const int nt = std::min(nthr, (P + 3) / 4);
+        if (nt <= 1) {
+            for (int p = 0; p < P; ++p) put(p);
+        } else {
+            b->hpool->run(nt, [&](int t) {
+                for (int p = t; p < P; p += nt) put(p);
+            });
+        }
+    }
     if (timed) {
         for (int k = 0; k < ntimed; ++k) {
             float ms = 0;
