@@ -215,7 +215,7 @@ def run_sweep(args):
                 cache["tun"] = [blqh_amd.Hamiltonian(B, par(d, 0.0), terms=2) for B in bases]
             hs = [blqh_amd.Hamiltonian.combine(b, tn, ds) for b, tn in zip(cache["base"], cache["tun"])]
         else:
-            hs = list(cache["base"])
+            hs = [H.prepare() for H in cache["base"]]  # solved directly: its SpMV formats count as build
         return hs, time.time() - t
 
     # convergence checks: plain solves every --check-every steps, the batch

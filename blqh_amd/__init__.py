@@ -79,6 +79,7 @@ _SIGS = {
     "blqh_hamil_build": (C.c_int, [C.c_void_p, C.POINTER(Params), C.POINTER(C.c_void_p)]),
     "blqh_hamil_build_terms": (C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int32, C.POINTER(C.c_void_p)]),
     "blqh_hamil_combine": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_void_p)]),
+    "blqh_hamil_prepare": (C.c_int, [C.c_void_p]),
     "blqh_hamil_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "blqh_hamil_export": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "blqh_hamil_destroy": (None, [C.c_void_p]),
@@ -290,6 +291,12 @@ class Hamiltonian:
         par = Params.from_buffer_copy(base.par)
         par.delta_sas = delta_sas
         return cls(base.basis, par, _handle=h)
+
+    def prepare(self):
+        """Build the SpMV formats now (a terms-1 or terms-2 part otherwise
+        builds them on its first SpMV, solve or batch)."""
+        _check(lib().blqh_hamil_prepare(self.h))
+        return self
 
     def export(self):
         rowptr = np.empty(self.n + 1, np.int64)

@@ -344,6 +344,11 @@ int blqh_hamil_build_terms(const blqh_basis* b, const blqh_params* p, int32_t te
     });
 }
 
+int blqh_hamil_prepare(blqh_hamil* h) {
+    if (!h) return fail(BLQH_EINVAL, "null hamiltonian");
+    return guarded([&] { blqh::ensure_panels(h); });
+}
+
 int blqh_hamil_combine(const blqh_hamil* base, const blqh_hamil* tun, double delta_sas, blqh_hamil** out) {
     if (!base || !tun || !out) return fail(BLQH_EINVAL, "null argument");
     *out = nullptr;
@@ -392,6 +397,10 @@ int blqh_hamil_get_spmv(const blqh_hamil* h, int32_t* kind, int32_t* lanes, int3
 
 int blqh_hamil_set_spmv(blqh_hamil* h, int32_t kind, int32_t lanes, int32_t threads, int32_t wpc, int32_t rows) {
     if (!h) return fail(BLQH_EINVAL, "null hamiltonian");
+    if (kind >= 1) {  // the panel kinds need the SpMV formats
+        const int rc = guarded([&] { blqh::ensure_panels(h); });
+        if (rc != BLQH_OK) return rc;
+    }
     bool ok = false;
     if (kind == -1) ok = true;
     if (kind == 0) ok = lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64;

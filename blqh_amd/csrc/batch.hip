@@ -938,6 +938,7 @@ blqh_batch* blqh_batch_create_impl(blqh_hamil* const* hs, int count, int lanes) 
         for (int p = 0; p < count; ++p) {
             blqh_hamil* H = hs[p];
             BLQH_REQUIRE(H != nullptr, BLQH_EINVAL, "null hamiltonian in batch");
+            ensure_panels(H);
             BLQH_REQUIRE(H->pan.built && H->n > 0, BLQH_EINVAL, "batched solve needs a non-empty panel layout");
             BProb d{};
             d.n = H->n;
@@ -982,7 +983,8 @@ void blqh_batch_rebind_impl(blqh_batch* b, blqh_hamil* const* hs, int count) {
     BLQH_REQUIRE(count == (int)b->hs.size(), BLQH_EINVAL, "rebind needs as many Hamiltonians as the batch has");
     int64_t nnz = 0;
     for (int p = 0; p < count; ++p) {
-        BLQH_REQUIRE(hs[p] != nullptr && hs[p]->pan.built, BLQH_EINVAL, "null hamiltonian in batch");
+        BLQH_REQUIRE(hs[p] != nullptr, BLQH_EINVAL, "null hamiltonian in batch");
+        ensure_panels(hs[p]);
         BLQH_REQUIRE(hs[p]->n == b->h_probs[p].n && hs[p]->pan.npan == b->h_probs[p].npan, BLQH_EINVAL,
                      "rebind needs Hamiltonians of the same sizes, in the same order");
         nnz += hs[p]->nnz;
@@ -1009,6 +1011,7 @@ blqh_hamil* make_template(const blqh_hamil* base, const blqh_hamil* tun) {
     auto* H = new blqh_hamil;
     try {
         blqh_hamil_combine_impl(base, tun, 1.0, H);
+        ensure_panels(H);
         BLQH_REQUIRE(H->pan.built && H->pan.npan == 1, BLQH_EINTERNAL, "template is not a one-panel matrix");
         set_panel_layout(H, false);  // row-major: the marking kernel walks rows
         const blqh_basis* B = H->basis;

@@ -114,6 +114,11 @@ struct HostTables {
 };
 HostTables host_tables(const blqh_sector& s, const blqh_params& p);
 void build_panels(blqh_hamil* H);
+// The SpMV formats of H (internal-order CSR, panels): built with a full
+// Hamiltonian or a merge, on first use (or blqh_hamil_prepare) for the parts
+// of a merge (terms 1 and 2), which often only feed blqh_hamil_combine or a
+// shared-scale batch and never need them.
+void ensure_panels(blqh_hamil* H);
 // Column-tile layout for SpMV kind 4 on `G` workgroups; returns false (and
 // builds nothing) when the matrix does not fit it.
 bool build_tiles(blqh_hamil* H, int G);

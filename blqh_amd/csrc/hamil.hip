@@ -763,7 +763,9 @@ void blqh_hamil_build_impl(const blqh_basis* B, const blqh_params& p_in, blqh_ha
     BLQH_REQUIRE((single ? e1[2] == 0 : e1[2] == novf) && e2[2] == 0, BLQH_EINTERNAL,
                  "count and fill passes disagree");
     phase(3);
-    build_panels(H);
+    // the SpMV formats (internal order, panels): now for a full Hamiltonian,
+    // on first use for the parts of a merge, which may never need them
+    if (terms == 3) build_panels(H);
     phase(4);
     H->seconds_build = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (timing)

@@ -1157,6 +1157,7 @@ void tune(blqh_hamil* H) {
 // entry order the chosen kernel reads; the column-tile copy is released when
 // another variant won.
 void autotune(blqh_hamil* H) {
+    ensure_panels(H);
     if (H->spmv_kind < 0) {
         tune(H);
         if (H->spmv_kind != 4 && H->til.built) {

@@ -878,6 +878,10 @@ void order_to_reference(const blqh_hamil* H, const double2* src, double2* dst, h
     BLQH_HIP(hipGetLastError());
 }
 
+void ensure_panels(blqh_hamil* H) {
+    if (!H->pan.built) build_panels(H);
+}
+
 void build_panels(blqh_hamil* H) {
     build_order(H);
     const CsrView A = spmv_csr(H);

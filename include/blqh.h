@@ -99,9 +99,14 @@ int blqh_hamil_build(const blqh_basis* b, const blqh_params* p, blqh_hamil** out
  * blqh_hamil_build with that Delta_SAS: same rows, columns and Coulomb and
  * diagonal values; tunnelling values to rounding (Delta_SAS * sum instead of
  * the sum of Delta_SAS * terms).  The result owns its arrays; base and tun may
- * be destroyed afterwards. */
+ * be destroyed afterwards.  A terms-1 or terms-2 part gets its SpMV formats
+ * (internal row order, panels) on its first SpMV, solve or batch, or from
+ * blqh_hamil_prepare; a part that only feeds a merge never builds them. */
 int blqh_hamil_build_terms(const blqh_basis* b, const blqh_params* p, int32_t terms, blqh_hamil** out);
 int blqh_hamil_combine(const blqh_hamil* base, const blqh_hamil* tun, double delta_sas, blqh_hamil** out);
+/* Build the SpMV formats of h now (no-op when present; no reference
+ * counterpart): lets a caller put that cost on a build thread. */
+int blqh_hamil_prepare(blqh_hamil* h);
 int blqh_hamil_info(const blqh_hamil* h, int64_t* nrows, int64_t* nnz);
 /* rowptr[nrows+1], col[nnz], val[2*nnz]; any pointer may be NULL. */
 int blqh_hamil_export(const blqh_hamil* h, int64_t* rowptr, int32_t* col, double* val);

@@ -70,3 +70,28 @@ def test_combine_rejects_mismatched_inputs():
     full = blqh_amd.Hamiltonian(B, blqh_amd.params(8, d=2.5, delta_sas=0.1))
     comb = blqh_amd.Hamiltonian.combine(base0, tun0, 0.1)
     assert np.array_equal(full.export()[1], comb.export()[1])
+
+
+def test_parts_build_their_spmv_formats_on_first_use():
+    """A terms-1 part (used by a merge, often never solved) builds its SpMV
+    formats on its first product, or on prepare(): the same products either
+    way, equal to the full build at Delta_SAS = 0, and prepare() twice is a
+    no-op."""
+    B = blqh_amd.Basis(10, 10, -1, 3, 2)
+    par = blqh_amd.params(10, d=1.0)
+    full = blqh_amd.Hamiltonian(B, par)
+    lazy = blqh_amd.Hamiltonian(B, par, terms=1)
+    ready = blqh_amd.Hamiltonian(B, par, terms=1).prepare().prepare()
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=B.n) + 1j * rng.normal(size=B.n)
+    y_lazy = lazy.spmv(x)  # first use: SpMV formats, then the autotune
+    plan = lazy.get_spmv()
+    ready.set_spmv(*plan)
+    full.set_spmv(*plan)
+    y_ready = ready.spmv(x)
+    y_full = full.spmv(x)
+    assert np.array_equal(y_lazy, y_ready)
+    np.testing.assert_allclose(y_lazy, y_full, rtol=0, atol=1e-13 * np.abs(y_full).max())
+    r1 = lazy.lanczos(lam=200, seed=3, tol=1e-10, want_psi=False)
+    r2 = full.lanczos(lam=200, seed=3, tol=1e-10, want_psi=False)
+    assert r1["E0"] == pytest.approx(r2["E0"], rel=1e-12)
