@@ -27,6 +27,7 @@
 #include <cstdlib>
 #include <condition_variable>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <memory>
 #include <mutex>
@@ -1040,6 +1041,38 @@ blqh_hamil* make_template(const blqh_hamil* base, const blqh_hamil* tun) {
     }
     return H;
 }
+
+// Templates of nsec sectors, built on up to 8 host threads: one sector's
+// merge, panels and marking are small launches with host waits between
+// them, so several sectors on their own streams keep the GPU busy.  On an
+// error every built template is freed and the first error is raised.
+std::vector<blqh_hamil*> make_templates(const blqh_hamil* const* base, const blqh_hamil* const* tun, int nsec) {
+    std::vector<blqh_hamil*> out(nsec, nullptr);
+    const int nt = std::max(1, std::min(nsec, 8));
+    std::vector<std::exception_ptr> err(nt);
+    auto work = [&](int t) {
+        try {
+            for (int q = t; q < nsec; q += nt) out[q] = make_template(base[q], tun[q]);
+        } catch (...) {
+            err[t] = std::current_exception();
+        }
+    };
+    if (nt == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+        work(0);
+        for (auto& x : th) x.join();
+    }
+    for (auto& e : err)
+        if (e) {
+            for (blqh_hamil* H : out) delete H;
+            std::rethrow_exception(e);
+        }
+    return out;
+}
+
 }  // namespace
 
 blqh_batch* blqh_batch_create_scaled_impl(const blqh_hamil* const* base, const blqh_hamil* const* tun, int nsec,
@@ -1052,7 +1085,7 @@ blqh_batch* blqh_batch_create_scaled_impl(const blqh_hamil* const* base, const b
     try {
         const int G = nscale;
         b->G = G;
-        for (int q = 0; q < nsec; ++q) b->tpl.push_back(make_template(base[q], tun[q]));
+        b->tpl = make_templates(base, tun, nsec);
         const int P = nsec * G;
         b->hs.resize(P);
         b->item0.assign(P + 1, 0);
@@ -1114,13 +1147,7 @@ void blqh_batch_rebind_scaled_impl(blqh_batch* b, const blqh_hamil* const* base,
     for (int q = 0; q < nsec; ++q)
         BLQH_REQUIRE(base[q] != nullptr && base[q]->n == b->tpl[q]->n, BLQH_EINVAL,
                      "rebind needs sectors of the same sizes, in the same order");
-    std::vector<blqh_hamil*> fresh;
-    try {
-        for (int q = 0; q < nsec; ++q) fresh.push_back(make_template(base[q], tun[q]));
-    } catch (...) {
-        for (blqh_hamil* H : fresh) delete H;
-        throw;
-    }
+    std::vector<blqh_hamil*> fresh = make_templates(base, tun, nsec);
     BLQH_HIP(hipStreamSynchronize(stream()));  // no queued work reads the old templates
     for (blqh_hamil* H : b->tpl) delete H;
     b->tpl.swap(fresh);

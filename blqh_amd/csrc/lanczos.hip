@@ -816,11 +816,11 @@ template <int LPR, int PT, int U, bool REAL, bool LZ>
 void panel_go(const blqh_hamil* H, const PanelLaunch& L, const double2* x, double2* out, const double2* Wp,
               const Scal& sc, int j, double* partials, hipStream_t st) {
     auto kern = k_pspmv<LPR, PT, U, REAL, LZ>;
-    static bool attr = false;
-    if (!attr) {
+    static const bool attr = [&] {  // once per instantiation, thread-safe
         BLQH_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
-        attr = true;
-    }
+        return true;
+    }();
+    (void)attr;
     hipLaunchKernelGGL(kern, dim3(L.grid), dim3(PT), L.smem, st, view(H->pan, H->n), x, out, L.rows_per_wg, L.chunk,
                        Wp, sc, j, partials);
 }

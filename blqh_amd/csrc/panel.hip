@@ -495,17 +495,18 @@ size_t greedy_lds_bytes(int cap) { return (size_t)cap * (8 + 2 + 2 + 2 + 1); }
 // Most entries per run that the greedy jagged order can hold in LDS: 8192
 // (120 KiB of LDS per workgroup) where the device allows it, else 4096 (60 KiB).
 int greedy_cap_max() {
-    static int cap = -1;
-    if (cap >= 0) return cap;
-    int dev = 0, maxlds = 0;
-    BLQH_HIP(hipGetDevice(&dev));
-    BLQH_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    cap = 4096;
-    if ((size_t)maxlds >= greedy_lds_bytes(8192) &&
-        hipFuncSetAttribute((const void*)k_relayout_greedy, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)greedy_lds_bytes(8192)) == hipSuccess)
-        cap = 8192;
-    (void)hipGetLastError();
+    static const int cap = [] {  // once, thread-safe
+        int dev = 0, maxlds = 0;
+        BLQH_HIP(hipGetDevice(&dev));
+        BLQH_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        int c = 4096;
+        if ((size_t)maxlds >= greedy_lds_bytes(8192) &&
+            hipFuncSetAttribute((const void*)k_relayout_greedy, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)greedy_lds_bytes(8192)) == hipSuccess)
+            c = 8192;
+        (void)hipGetLastError();
+        return c;
+    }();
     return cap;
 }
 
