@@ -22,7 +22,7 @@ BLQH_with_spin/lanczos_hamiltonian.cpp:105-908).
    rotation invariant, so every level of the S_z = m + 1 sector is a level
    of the S_z = m sector (m >= 0).  A missing or mis-weighted inter-spin
    pair block breaks it.
-4. The printed Lanczos energy of a complex sector (F11 of the survey): the
+4. The printed Lanczos energy of a complex sector: the
    reference prints Re<psi|H psi> with psi = sum_l c_l conj(v_l)
    (lanczos_hamiltonian.cpp:624-632,646-656), i.e. psi = conj(u0), which is
    not E0 when H is complex.  The oracle's converged value equals
