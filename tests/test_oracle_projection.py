@@ -143,7 +143,7 @@ def test_complex_sector_printed_energy(oracle_mod, J, K):
     printed = np.real(np.vdot(u, Hd @ u))
     r = H.lanczos(200, seed=2)
     assert r["E_rq"] == pytest.approx(printed, rel=1e-10)
-    # F11: for a complex sector the printed value is not the ground-state energy
+    # for a complex sector the printed value is not the ground-state energy
     assert abs(printed - w[0]) > 1e-6 * abs(w[0])
 
 
