@@ -309,6 +309,25 @@ def run_sweep(args):
             launches, kms, kbytes = blqh_amd.profile_read()
             nnz_total = sum(H.nnz for H in hs)
         del hs
+    # multi-point leg: the timed points' problems (every sector at every
+    # timed point) in one batch, so that no point's last steps run alone
+    multi = None
+    if args.multi_point and len(order) - nwarm > 1 and len(set(order[nwarm:])) == len(order) - nwarm:
+        pts_t = order[nwarm:]
+        hs_all = []
+        for d, ds in pts_t:
+            hs_all += build(d, ds)[0]
+        mb = blqh_amd.Batch(hs_all)
+        mseeds = seeds * len(pts_t)
+        mb.lanczos(lam=args.lam, seeds=mseeds, tol=1e-10, check_every=bchk, want_psi=False)  # warm-up
+        t = time.perf_counter()
+        mres = mb.lanczos(lam=args.lam, seeds=mseeds, tol=1e-10, check_every=bchk, want_psi=False)
+        dtm = time.perf_counter() - t
+        mok = sum(int(r["converged"] == 1 and r["dE"] < 1e-10) for r in mres)
+        multi = {"points": len(pts_t), "solves": len(mres), "converged": mok, "seconds": dtm, "rate": mok / dtm}
+        log(f"multi-point batch: {len(pts_t)} points, {len(mres)} problems in one batch: {dtm:.3f} s, "
+            f"{mok}/{len(mres)} converged ({mok / dtm:.1f}/s)")
+        del mb, hs_all
     # pipelined leg: the timed points again, each point's Hamiltonians built on
     # a second host thread (its own library stream) while the previous point
     # solves, so the builds overlap the batched solves on the GPU
@@ -368,6 +387,8 @@ def run_sweep(args):
             "end_to_end_rate_incl_build": solved / (t_solve + t_build) if t_solve + t_build > 0 else None,
             "end_to_end_rate_pipelined": pipe["rate"] if pipe else None,
             "pipelined": pipe,
+            "batched_multipoint_rate": multi["rate"] if multi else None,
+            "multipoint": multi,
             "continuation_rate": cont["solved"] / cont["seconds"] if cont["seconds"] > 0 else None,
             "continuation_iterations_mean": float(np.mean(cont["iters"])) if cont["iters"] else None,
             "continuation_max_rel_dE0_vs_cold": cont["max_rel_dE0"] if cont["iters"] else None,
@@ -791,6 +812,8 @@ def main():
     ap.add_argument("--shared", type=int, default=1,
                     help="sweeps with several DSAS values per d: one shared-scale batch per d (1) or one batch "
                          "per parameter point (0)")
+    ap.add_argument("--multi-point", type=int, default=1,
+                    help="per-point sweeps: also solve the timed points' problems in one batch (1) or not (0)")
     ap.add_argument("--pipeline", type=int, default=1,
                     help="sweeps: also time the timed points with each point's builds on a second host thread, "
                          "overlapping the previous point's solve (1) or not (0)")
@@ -930,6 +953,7 @@ def main():
                 "warm_solves_above_cold_E0": c["continuation_mismatches"],
                 "end_to_end_rate_incl_build": c["end_to_end_rate_incl_build"],
                 "end_to_end_rate_pipelined": c["end_to_end_rate_pipelined"],
+                "batched_multipoint_rate": c.get("batched_multipoint_rate"),
                 "max_rel_dE0_batched_vs_plain": c["plain"]["max_rel_dE0_vs_batched"] if c["plain"] else None,
                 "roofline": {k: r["roofline"][k] for k in ("achieved", "peak", "unit", "frac", "traffic", "kernel",
                                                            "avg_launch_ms", "algorithmic_bytes_per_launch")},
