@@ -535,6 +535,25 @@ def run_sweep_shared(args):
             launches, kms, kbytes = blqh_amd.profile_read()
             nnz_total = sum(H.nnz for H in base) + sum(H.nnz for H in tun)
         del base  # the batch keeps its own copy of H(d,0) + T
+    # multi-point leg: the timed d values' groups in one shared-scale batch
+    multi = None
+    if args.multi_point and len(order) - nwarm > 1:
+        pts_t = order[nwarm:]
+        base_all = []
+        for d in pts_t:
+            base_all += build_map(lambda B, d=d: blqh_amd.Hamiltonian(B, par(d), terms=1), bases)
+        mb = blqh_amd.Batch.scaled(base_all, tun * len(pts_t), ds_list)
+        del base_all
+        mseeds = seeds * len(pts_t)
+        mb.lanczos(lam=args.lam, seeds=mseeds, tol=1e-10, check_every=bchk, want_psi=False)  # warm-up
+        t = time.perf_counter()
+        mres = mb.lanczos(lam=args.lam, seeds=mseeds, tol=1e-10, check_every=bchk, want_psi=False)
+        dtm = time.perf_counter() - t
+        mok = sum(int(r["converged"] == 1 and r["dE"] < 1e-10) for r in mres)
+        multi = {"points": len(pts_t) * G, "solves": len(mres), "converged": mok, "seconds": dtm, "rate": mok / dtm}
+        log(f"multi-point batch: {len(pts_t)} d values, {len(mres)} problems in one batch: {dtm:.3f} s, "
+            f"{mok}/{len(mres)} converged ({mok / dtm:.1f}/s)")
+        del mb
     # pipelined leg: the timed d values again, each d's H(d,0) built on a
     # second host thread (its own library stream) while the previous d solves
     pipe = None
@@ -597,6 +616,8 @@ def run_sweep_shared(args):
             "end_to_end_rate_incl_build": solved / (t_solve + t_build) if t_solve + t_build > 0 else None,
             "end_to_end_rate_pipelined": pipe["rate"] if pipe else None,
             "pipelined": pipe,
+            "batched_multipoint_rate": multi["rate"] if multi else None,
+            "multipoint": multi,
             "continuation_rate": cont["solved"] / cont["seconds"] if cont["seconds"] > 0 else None,
             "continuation_iterations_mean": float(np.mean(cont["iters"])) if cont["iters"] else None,
             "continuation_max_rel_dE0_vs_cold": cont["max_rel_dE0"] if cont["iters"] else None,
@@ -813,7 +834,7 @@ def main():
                     help="sweeps with several DSAS values per d: one shared-scale batch per d (1) or one batch "
                          "per parameter point (0)")
     ap.add_argument("--multi-point", type=int, default=1,
-                    help="per-point sweeps: also solve the timed points' problems in one batch (1) or not (0)")
+                    help="sweeps: also solve the timed points' problems in one batch (1) or not (0)")
     ap.add_argument("--pipeline", type=int, default=1,
                     help="sweeps: also time the timed points with each point's builds on a second host thread, "
                          "overlapping the previous point's solve (1) or not (0)")
