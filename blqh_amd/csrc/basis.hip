@@ -346,7 +346,9 @@ void validate_sector(const blqh_sector& s) {
 namespace blqh {
 
 // Iteration order of the reference's std::unordered_map<unsigned long, long>
-// after inserting the map sector (no spin-up filter yet) in depth-first order.
+// after inserting the map sector (no spin-up filter yet) in depth-first order,
+// on the host.  The basis takes the order from the device computation
+// (map_positions); this sequential replay is the CPU tests' check of it.
 //
 // The map is not built: its node list is replayed with index arrays.  The
 // container keeps every node in one singly linked list; bucket b (key mod
