@@ -240,9 +240,9 @@ def run_sweep(args):
     t_solve = t_build = 0.0
     iters, e_by_point = [], {}
     # continuation starts: every sector starts from its ground state at the
-    # previous point (conj(psi): psi = sum c_l conj(v_l) is the eigenvector of
-    # conj(H)); checked against the cold batched solve of the same point
-    cont_psi = None
+    # previous point, which the batch keeps on the device (conj(psi): psi =
+    # sum c_l conj(v_l) is the eigenvector of conj(H)); run before the cold
+    # batched solve of the same point, which checks it
     cont = {"solved": 0, "seconds": 0.0, "iters": [], "max_rel_dE0": 0.0, "mismatch": 0}
     for k, (d, ds) in enumerate(order):
         hs, tb = build(d, ds)
@@ -253,8 +253,16 @@ def run_sweep(args):
             batch = blqh_amd.Batch(hs)  # autotunes the batched SpMV once for the sweep
         else:
             batch.rebind(hs)  # same sectors, next parameter point: pools and replay kept
-        res = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False)
         dt = time.perf_counter() - t
+        wres = None
+        if args.continuation and k > 0:
+            t = time.perf_counter()
+            wres = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False,
+                                 start_mix=args.start_mix, continue_prev=True)
+            dtw = time.perf_counter() - t
+        t = time.perf_counter()
+        res = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False)
+        dt += time.perf_counter() - t
         t = time.time()
         del prev  # the previous point's Hamiltonians (device frees)
         tb += time.time() - t
@@ -262,20 +270,14 @@ def run_sweep(args):
         log(f"{'warmup' if warm else 'point'} d={d} DSAS={ds}: build {tb:.2f} s, batched solve {dt:.3f} s, "
             f"{ok}/{len(res)} converged, iterations {min(r['iterations'] for r in res)}.."
             f"{max(r['iterations'] for r in res)}")
-        if args.continuation:
-            t = time.perf_counter()
-            starts = [np.conj(p) for p in cont_psi] if cont_psi is not None else None
-            wres = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk,
-                                 starts=starts, want_psi=True, start_mix=args.start_mix if starts else 0.0)
-            dtw = time.perf_counter() - t
-            cont_psi = [r["psi"] for r in wres]
+        if wres is not None:
             wok = [r["converged"] == 1 and r["dE"] < 1e-10 for r in wres]
             rel = [abs(a["E0"] - b["E0"]) / abs(b["E0"]) for a, b in zip(wres, res)]
             bad = sum(int(x > 1e-10) for x, w in zip(rel, wok) if w)
             log(f"  continuation: batched solve {dtw:.3f} s, {sum(wok)}/{len(wres)} converged, iterations "
                 f"{min(r['iterations'] for r in wres)}..{max(r['iterations'] for r in wres)}, "
                 f"max |E0 - cold E0|/|E0| = {max(rel):.1e}")
-            if not warm and starts is not None:
+            if not warm:
                 cont["solved"] += sum(int(w and x <= 1e-10) for w, x in zip(wok, rel))
                 cont["seconds"] += dtw
                 cont["iters"] += [r["iterations"] for r in wres]
@@ -461,7 +463,6 @@ def run_sweep_shared(args):
 
     tun = None  # T does not depend on d without theta_B phases: built once
     batch = None
-    cont_psi = None
     cont = {"solved": 0, "seconds": 0.0, "iters": [], "max_rel_dE0": 0.0, "mismatch": 0}
     plain = None
     solved = solves = 0
@@ -478,6 +479,15 @@ def run_sweep_shared(args):
         else:
             batch.rebind_scaled(base, tun)
         tb = time.time() - t
+        wres = None
+        if args.continuation and k > 0:
+            # every problem starts from its ground state at the previous d (same
+            # sector and DSAS, kept on the device), checked against the cold
+            # solve of this d below
+            t = time.perf_counter()
+            wres = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False,
+                                 start_mix=args.start_mix, continue_prev=True)
+            dtw = time.perf_counter() - t
         t = time.perf_counter()
         res = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False)
         dt = time.perf_counter() - t
@@ -485,22 +495,14 @@ def run_sweep_shared(args):
         log(f"{'warmup' if warm else 'step'} d={d} DSAS={ds_list}: build {tb:.2f} s, shared-scale batched solve "
             f"{dt:.3f} s, {ok}/{len(res)} converged, iterations {min(r['iterations'] for r in res)}.."
             f"{max(r['iterations'] for r in res)}")
-        if args.continuation:
-            # every problem starts from its ground state at the previous d (same
-            # sector and DSAS), checked against the cold solve of this d
-            t = time.perf_counter()
-            starts = [np.conj(p) for p in cont_psi] if cont_psi is not None else None
-            wres = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, starts=starts,
-                                 want_psi=True, start_mix=args.start_mix if starts else 0.0)
-            dtw = time.perf_counter() - t
-            cont_psi = [r["psi"] for r in wres]
+        if wres is not None:
             wok = [r["converged"] == 1 and r["dE"] < 1e-10 for r in wres]
             rel = [abs(a["E0"] - b["E0"]) / abs(b["E0"]) for a, b in zip(wres, res)]
             bad = sum(int(x > 1e-10) for x, w in zip(rel, wok) if w)
             log(f"  continuation: batched solve {dtw:.3f} s, {sum(wok)}/{len(wres)} converged, iterations "
                 f"{min(r['iterations'] for r in wres)}..{max(r['iterations'] for r in wres)}, "
                 f"{bad} above the cold E0 by more than 1e-10")
-            if not warm and starts is not None:
+            if not warm:
                 cont["solved"] += sum(int(w and x <= 1e-10) for w, x in zip(wok, rel))
                 cont["seconds"] += dtw
                 cont["iters"] += [r["iterations"] for r in wres]

@@ -94,6 +94,8 @@ _SIGS = {
     "blqh_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "blqh_batch_lanczos": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.c_void_p, C.c_void_p,
                                      C.POINTER(LanczosResult), C.c_void_p]),
+    "blqh_batch_lanczos_continue": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.c_void_p,
+                                              C.POINTER(LanczosResult), C.c_void_p]),
     "blqh_batch_rebind": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32]),
     "blqh_batch_create_scaled": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32,
                                            C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_void_p)]),
@@ -494,10 +496,13 @@ class Batch:
         _check(self._lib.blqh_batch_set_lanes(self.h, v))
 
     def lanczos(self, lam=200, seeds=0, tol=1e-10, res_tol=1e-7, check_every=5, starts=None, want_psi=True,
-                start_mix=0.0):
+                start_mix=0.0, continue_prev=False):
         """Ground states of every problem; `seeds` is one seed or one per problem,
         `starts` None or a list of start vectors (None entries: seeded),
-        `start_mix` the weight of the seeded unit vector added to each start."""
+        `start_mix` the weight of the seeded unit vector added to each start.
+        continue_prev: start every problem from conj(psi) of the previous
+        solve on this batch, kept on the device (blqh_batch_lanczos_continue;
+        the same lam as that solve, no `starts`)."""
         P = len(self.hams)
         opts = LanczosOpts(lam, 0, tol, check_every, 1, None, res_tol, 0, start_mix)
         sd = np.asarray(seeds, np.uint32)
@@ -523,7 +528,12 @@ class Batch:
         po = None
         if want_psi:
             po = (C.c_void_p * P)(*[x.ctypes.data for x in psis])
-        _check(self._lib.blqh_batch_lanczos(self.h, C.byref(opts), sd.ctypes.data, st, res, po))
+        if continue_prev:
+            if starts is not None:
+                raise ValueError("continue_prev takes no start vectors")
+            _check(self._lib.blqh_batch_lanczos_continue(self.h, C.byref(opts), sd.ctypes.data, res, po))
+        else:
+            _check(self._lib.blqh_batch_lanczos(self.h, C.byref(opts), sd.ctypes.data, st, res, po))
         out = []
         for i in range(P):
             d = {k: getattr(res[i], k) for k, _ in LanczosResult._fields_}

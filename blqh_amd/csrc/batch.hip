@@ -587,6 +587,42 @@ __global__ void __launch_bounds__(kBT) k_bstart_scale(const BStart* __restrict__
     }
 }
 
+// Continuation starts: the previous solve's ground state, kept in the
+// reference row order (prev), conjugated (psi = sum_l c_l conj(v_l) is the
+// eigenvector of conj(H), so conj(psi) is that of H), plus mix times the
+// seeded unit vector: dst[r] = conj(prev[ip(r)]) + mix / |gen| * gen[ip(r)]
+// (ip = the member's internal order, the identity without one).  One
+// 1024-thread workgroup per problem; nrm2 gets |dst|^2, the same fixed-order
+// sum on every call.
+struct BCont {
+    int64_t n;
+    const double2* prev;
+    const double2* gen;  // seeded entries, reference order (read when mix > 0)
+    double2* dst;
+    const int32_t* iperm;
+    double mix;          // start_mix / |gen|
+};
+
+__global__ void __launch_bounds__(1024) k_bcont(const BCont* __restrict__ jobs, double* __restrict__ nrm2) {
+    __shared__ double sh[1024 / 64];
+    const BCont J = jobs[blockIdx.x];
+    double acc = 0.0;
+    for (int64_t r = threadIdx.x; r < J.n; r += 1024) {
+        const int64_t q = J.iperm ? J.iperm[r] : r;
+        const double2 a = J.prev[q];
+        double2 z = make_double2(a.x, -a.y);
+        if (J.mix > 0) {
+            const double2 g = J.gen[q];
+            z.x += J.mix * g.x;
+            z.y += J.mix * g.y;
+        }
+        J.dst[r] = z;
+        acc += z.x * z.x + z.y * z.y;
+    }
+    const double t = block_sum<1024>(acc, sh);
+    if (threadIdx.x == 0) nrm2[blockIdx.x] = t;
+}
+
 // Ground states back to the reference row order for the copy to the host:
 // dst[iperm[r]] = src[r] (a member in its internal order), dst[r] = src[r]
 // otherwise; one job per problem (blockIdx.y).
@@ -734,6 +770,9 @@ struct blqh_batch {
     int g_chk = -1, g_lpr = -1, g_cap = -1, g_kinds = -1;
     char* pin = nullptr;
     double2* psi_pin = nullptr;  // pinned staging of the ground states (b->rows entries, on first use)
+    // pl.u holds every problem's ground state of the last finished solve in
+    // the reference row order (the start of a continuation solve)
+    bool cont_ok = false;
     DevBuf<char> relayout;       // scratch for reordering members' panels
     // Shared-scale groups (blqh_batch_create_scaled); G = 0: every member
     // has its own matrix.  Problem p = s G + g is H0[s] + scale[g] T[s];
@@ -1179,8 +1218,13 @@ void blqh_batch_set_lanes_impl(blqh_batch* b, int lanes) {
 }
 
 void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const uint32_t* seeds,
-                             const double* const* starts, blqh_lanczos_result* res, double* const* psi_out) {
+                             const double* const* starts, blqh_lanczos_result* res, double* const* psi_out,
+                             bool cont) {
     BLQH_REQUIRE(o.start_mix >= 0 && std::isfinite(o.start_mix), BLQH_EINVAL, "start_mix must be finite and >= 0");
+    if (cont)
+        BLQH_REQUIRE(b->cont_ok && b->pool_cap == o.lambda && !starts, BLQH_EINVAL,
+                     "a continuation solve needs a finished solve of the same lambda on this batch");
+    b->cont_ok = false;  // the steps below overwrite pl.u
     auto t0 = std::chrono::steady_clock::now();
     // BLQH_BATCH_TIMING=1 prints the host-side phases of every solve to stderr
     const bool phase_log = std::getenv("BLQH_BATCH_TIMING") != nullptr;
@@ -1250,7 +1294,56 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     phase("setup");
     // start vectors, beta_0 = 1: the caller's on the host (normalised, mixed,
     // permuted), the seeded ones drawn on the device (k_bmt_start)
-    {
+    if (cont) {
+        // conj(previous ground state) + start_mix * seeded unit vector, drawn
+        // into the second Krylov slot (free until the first step)
+        std::vector<BStart> gj;
+        std::vector<BCont> cj(P);
+        for (int p = 0; p < P; ++p) {
+            const blqh_hamil* Hp = b->hs[p];
+            double2* gen = hp[p].W + hp[p].n;
+            cj[p] = {hp[p].n, pl.u.get() + hp[p].voff, gen, hp[p].W, Hp->ord.on ? Hp->ord.iperm.get() : nullptr, 0.0};
+            if (o.start_mix > 0) gj.push_back({hp[p].n, seeds ? seeds[p] : o.seed, gen, gen, nullptr});
+        }
+        DevBuf<BStart> dgj;
+        DevBuf<BCont> dcj(P);
+        DevBuf<double> dnrm(2 * (size_t)P);
+        std::vector<double> nrm(P);
+        if (!gj.empty()) {
+            dgj.alloc(P);
+            BLQH_HIP(hipMemcpyAsync(dgj.get(), gj.data(), P * sizeof(BStart), hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_bmt_start, dim3(P), dim3(1024), 0, st, dgj.get(), dnrm.get());
+            BLQH_HIP(hipGetLastError());
+            BLQH_HIP(hipMemcpyAsync(nrm.data(), dnrm.get(), P * sizeof(double), hipMemcpyDeviceToHost, st));
+            BLQH_HIP(hipStreamSynchronize(st));
+            for (int p = 0; p < P; ++p) cj[p].mix = o.start_mix / std::sqrt(nrm[p]);
+        }
+        BLQH_HIP(hipMemcpyAsync(dcj.get(), cj.data(), P * sizeof(BCont), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_bcont, dim3(P), dim3(1024), 0, st, dcj.get(), dnrm.get());
+        BLQH_HIP(hipGetLastError());
+        BLQH_HIP(hipMemcpyAsync(nrm.data(), dnrm.get(), P * sizeof(double), hipMemcpyDeviceToHost, st));
+        BLQH_HIP(hipStreamSynchronize(st));
+        std::vector<BStart> sj(P);
+        for (int p = 0; p < P; ++p) {
+            BLQH_REQUIRE(nrm[p] > 0 && std::isfinite(nrm[p]), BLQH_EINVAL, "continuation start is zero or not finite");
+            nrm[p] = std::sqrt(nrm[p]);
+            sj[p] = {hp[p].n, 0u, hp[p].W, hp[p].W, nullptr};
+        }
+        BLQH_HIP(hipMemcpyAsync(dnrm.get() + P, nrm.data(), P * sizeof(double), hipMemcpyHostToDevice, st));
+        DevBuf<BStart> dsj(P);
+        BLQH_HIP(hipMemcpyAsync(dsj.get(), sj.data(), P * sizeof(BStart), hipMemcpyHostToDevice, st));
+        int64_t nmax = 0;
+        for (int p = 0; p < P; ++p) nmax = std::max(nmax, hp[p].n);
+        const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(nmax, kBT), 256));
+        for (int j0 = 0; j0 < P; j0 += 65535)
+            hipLaunchKernelGGL(k_bstart_scale, dim3(gx, (unsigned)std::min(65535, P - j0)), dim3(kBT), 0, st,
+                               dsj.get() + j0, dnrm.get() + P + j0);
+        BLQH_HIP(hipGetLastError());
+        std::vector<double> ones((size_t)P * ld, 1.0);
+        BLQH_HIP(hipMemcpyAsync(pl.beta.get(), ones.data(), ones.size() * 8, hipMemcpyHostToDevice, st));
+        BLQH_HIP(hipMemcpyAsync(pl.invb.get(), ones.data(), ones.size() * 8, hipMemcpyHostToDevice, st));
+        BLQH_HIP(hipStreamSynchronize(st));  // job tables, norms and ones are freed at scope exit
+    } else {
         std::vector<std::vector<double2>> v0(P);
         std::vector<char> seeded(P, 0);
         for (int p = 0; p < P; ++p) seeded[p] = !(starts && starts[p]);
@@ -1571,32 +1664,36 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     launch_b1<false>(b, S, 0, pl.psi.get(), pl.u.get(), pl.part2.get(), st);
     BLQH_HIP(hipGetLastError());
     BLQH_HIP(hipMemcpyAsync(hp2.data(), pl.part2.get(), b->nitems * 8, hipMemcpyDeviceToHost, st));
-    // ground states: every wanted one in the reference order into u (free
-    // after the last SpMV), one copy of u into pinned staging, then host
-    // copies into the caller's arrays on the pool threads
+    // ground states: all of them in the reference order into u (free after
+    // the last SpMV; the start of a continuation solve), then, when any is
+    // wanted, one copy of u into pinned staging and host copies into the
+    // caller's arrays on the pool threads
     std::vector<BPsi> pj;
     int64_t pmax = 0;
-    for (int p = 0; p < P; ++p)
-        if (psi_out && psi_out[p]) {
-            const blqh_hamil* Hp = b->hs[p];
-            pj.push_back({hp[p].n, Hp->ord.on ? Hp->ord.iperm.get() : nullptr, pl.psi.get() + hp[p].voff,
-                          pl.u.get() + hp[p].voff});
-            pmax = std::max(pmax, hp[p].n);
-        }
-    DevBuf<BPsi> dpj;
-    if (!pj.empty()) {
-        if (!b->psi_pin) BLQH_HIP(hipHostMalloc((void**)&b->psi_pin, (size_t)b->rows * sizeof(double2), hipHostMallocDefault));
-        dpj.alloc(pj.size());
-        BLQH_HIP(hipMemcpyAsync(dpj.get(), pj.data(), pj.size() * sizeof(BPsi), hipMemcpyHostToDevice, st));
+    bool want = false;
+    for (int p = 0; p < P; ++p) {
+        const blqh_hamil* Hp = b->hs[p];
+        pj.push_back({hp[p].n, Hp->ord.on ? Hp->ord.iperm.get() : nullptr, pl.psi.get() + hp[p].voff,
+                      pl.u.get() + hp[p].voff});
+        pmax = std::max(pmax, hp[p].n);
+        want = want || (psi_out && psi_out[p]);
+    }
+    DevBuf<BPsi> dpj(pj.size());
+    BLQH_HIP(hipMemcpyAsync(dpj.get(), pj.data(), pj.size() * sizeof(BPsi), hipMemcpyHostToDevice, st));
+    {
         const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(pmax, kBT), 256));
         for (size_t j0 = 0; j0 < pj.size(); j0 += 65535)
             hipLaunchKernelGGL(k_bpsi_ref, dim3(gx, (unsigned)std::min<size_t>(65535, pj.size() - j0)), dim3(kBT), 0,
                                st, dpj.get() + j0);
         BLQH_HIP(hipGetLastError());
+    }
+    if (want) {
+        if (!b->psi_pin) BLQH_HIP(hipHostMalloc((void**)&b->psi_pin, (size_t)b->rows * sizeof(double2), hipHostMallocDefault));
         BLQH_HIP(hipMemcpyAsync(b->psi_pin, pl.u.get(), (size_t)b->rows * sizeof(double2), hipMemcpyDeviceToHost, st));
     }
     BLQH_HIP(hipStreamSynchronize(st));
-    if (!pj.empty()) {
+    b->cont_ok = true;
+    if (want) {
         auto put = [&](int p) {
             if (psi_out[p]) std::memcpy(psi_out[p], b->psi_pin + hp[p].voff, hp[p].n * sizeof(double2));
         };

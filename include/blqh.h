@@ -243,6 +243,17 @@ int blqh_batch_create(blqh_hamil* const* hs, int32_t count, int32_t lanes, blqh_
 int blqh_batch_rebind(blqh_batch* b, blqh_hamil* const* hs, int32_t count);
 int blqh_batch_lanczos(blqh_batch* b, const blqh_lanczos_opts* o, const uint32_t* seeds,
                        const double* const* starts, blqh_lanczos_result* res, double* const* psi);
+/* Continuation solve (no counterpart in the reference, which starts every
+ * point of its sweeps from a seeded vector; for parameter sweeps whose
+ * callers would otherwise copy each ground state out and back in as the next
+ * start).  Every problem starts from conj(psi) of the previous finished solve
+ * on this batch (kept on the device in the reference row order, across
+ * blqh_batch_rebind / blqh_batch_rebind_scaled) plus o->start_mix times its
+ * seeded unit vector, normalised: the start blqh_batch_lanczos builds from
+ * starts[p] = conj(previous psi[p]).  Needs the same o->lambda as that solve
+ * (BLQH_EINVAL otherwise, and on a batch that has not finished a solve). */
+int blqh_batch_lanczos_continue(blqh_batch* b, const blqh_lanczos_opts* o, const uint32_t* seeds,
+                                blqh_lanczos_result* res, double* const* psi);
 /* Lanes per row of the batched SpMV (4, 8, 16 or 32 over row-major panels;
  * 64: one lane per row over step-major panels; autotuned at create). */
 int blqh_batch_get_lanes(const blqh_batch* b, int32_t* lanes);

@@ -190,3 +190,39 @@ def test_many_members_one_sector_each():
         assert r[i]["converged"] == 1 and s["converged"] == 1, i
         assert r[i]["E0"] == pytest.approx(s["E0"], rel=1e-10, abs=0), i
         assert r2[i]["E0"] == r[i]["E0"] and r2[i]["iterations"] == r[i]["iterations"], i
+
+
+@pytest.mark.parametrize("mix", [0.0, 0.05])
+def test_continuation_matches_host_starts(hams, mix):
+    """blqh_batch_lanczos_continue after a rebind starts every problem from
+    conj(psi) of the previous solve kept on the device; it must agree with
+    the same solve started from the host copies (starts = conj(psi), same
+    start_mix and seeds), and it is refused before any solve and for another
+    lambda."""
+    seeds = [11 + i for i in range(len(PROBLEMS))]
+    seeds2 = [40 + i for i in range(len(PROBLEMS))]
+    bt = blqh_amd.Batch([H for _, _, H in hams])
+    with pytest.raises(blqh_amd.BlqhError):
+        bt.lanczos(lam=250, seeds=seeds, check_every=4, continue_prev=True)
+    cold = bt.lanczos(lam=250, seeds=seeds, check_every=4)
+    new = []
+    for (name, S, nphi, nel, nu, nsu, J, K, d, dsas), (_, B, _) in zip(PROBLEMS, hams):
+        new.append(blqh_amd.Hamiltonian(B, blqh_amd.params(nphi, d=d + 0.25, delta_sas=dsas)))
+    bt.rebind(new)
+    with pytest.raises(blqh_amd.BlqhError):
+        bt.lanczos(lam=200, seeds=seeds2, check_every=4, continue_prev=True)
+    cont = bt.lanczos(lam=250, seeds=seeds2, check_every=4, start_mix=mix, continue_prev=True)
+    host = blqh_amd.Batch(new, lanes=bt.lanes).lanczos(
+        lam=250, seeds=seeds2, check_every=4, starts=[np.conj(c["psi"]) for c in cold], start_mix=mix)
+    for (name, _, H), a, b in zip(hams, cont, host):
+        print(f"continuation {name} mix={mix}: it {a['iterations']} host {b['iterations']}", flush=True)
+        assert a["converged"] == 1 and b["converged"] == 1, name
+        assert a["E0"] == pytest.approx(b["E0"], rel=1e-10, abs=0), name
+        assert abs(a["iterations"] - b["iterations"]) <= 4, name
+        assert abs(np.vdot(a["psi"], b["psi"])) >= 1 - 1e-10, name
+    # and again from the continuation's own states (no rebind): a few steps
+    again = bt.lanczos(lam=250, seeds=seeds2, check_every=2, start_mix=mix, continue_prev=True, want_psi=False)
+    for a, b in zip(again, cont):
+        assert a["converged"] == 1
+        assert a["E0"] == pytest.approx(b["E0"], rel=1e-10, abs=0)
+    bt.close()

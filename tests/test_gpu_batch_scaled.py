@@ -74,6 +74,27 @@ def test_shared_scales_match_single_solver(bases):
     bt.close()
 
 
+def test_shared_scale_continuation(bases):
+    """A continuation solve of a shared-scale batch after rebind_scaled
+    agrees with the same solve from host start vectors conj(psi)."""
+    secs = SECTORS[1:3]
+    G = len(SCALES)
+    pairs = [_pair(sec, 1.0, bases) for sec in secs]
+    bt = blqh_amd.Batch.scaled([p[1] for p in pairs], [p[2] for p in pairs], SCALES)
+    seeds = [5 + i for i in range(len(secs) * G)]
+    cold = bt.lanczos(lam=300, seeds=seeds, check_every=3)
+    pairs2 = [_pair(sec, 1.3, bases) for sec in secs]
+    bt.rebind_scaled([p[1] for p in pairs2], [p[2] for p in pairs2])
+    cont = bt.lanczos(lam=300, seeds=seeds, check_every=3, start_mix=0.05, continue_prev=True)
+    host = bt.lanczos(lam=300, seeds=seeds, check_every=3, start_mix=0.05, starts=[np.conj(c["psi"]) for c in cold])
+    for p, (a, b) in enumerate(zip(cont, host)):
+        assert a["converged"] == 1 and b["converged"] == 1, p
+        assert a["E0"] == pytest.approx(b["E0"], rel=1e-10, abs=0), p
+        assert abs(np.vdot(a["psi"], b["psi"])) >= 1 - 1e-10, p
+    _check(secs, pairs2, cont, G, seeds)
+    bt.close()
+
+
 @pytest.mark.parametrize("G", [1, 2, 4])
 def test_group_sizes(bases, G):
     scales = [0.0, 0.05, 0.1, 0.2][:G]
