@@ -258,7 +258,8 @@ def run_sweep(args):
         if args.continuation and k > 0:
             t = time.perf_counter()
             wres = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False,
-                                 start_mix=args.start_mix, continue_prev=True)
+                                 start_mix=args.start_mix, continue_prev=True,
+                                 excited_mix=args.excited_mix)
             dtw = time.perf_counter() - t
         t = time.perf_counter()
         res = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False)
@@ -486,7 +487,8 @@ def run_sweep_shared(args):
             # solve of this d below
             t = time.perf_counter()
             wres = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False,
-                                 start_mix=args.start_mix, continue_prev=True)
+                                 start_mix=args.start_mix, continue_prev=True,
+                                 excited_mix=args.excited_mix)
             dtw = time.perf_counter() - t
         t = time.perf_counter()
         res = batch.lanczos(lam=args.lam, seeds=seeds, tol=1e-10, check_every=bchk, want_psi=False)
@@ -840,6 +842,9 @@ def main():
     ap.add_argument("--pipeline", type=int, default=1,
                     help="sweeps: also time the timed points with each point's builds on a second host thread, "
                          "overlapping the previous point's solve (1) or not (0)")
+    ap.add_argument("--excited-mix", type=float, default=1.0,
+                    help="continuation solves: weight of the previous solve's second and third Ritz vectors "
+                         "added to the start (guards level crossings)")
     ap.add_argument("--continuation", type=int, default=0,
                     help="sweeps: also solve every point from the previous point's ground states (1) or not (0)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -972,7 +977,8 @@ def main():
                 "batched_over_plain": c["batched_over_plain"], "iterations_mean": c["iterations_mean"],
                 "batched_warm_rate": c["continuation_rate"], "batched_warm_over_plain": c["continuation_vs_plain"],
                 "warm_iterations_mean": c["continuation_iterations_mean"],
-                "warm_start": "previous parameter point's ground state + 0.1 x seeded unit vector",
+                "warm_start": (f"previous parameter point's ground state + {sa.start_mix} x seeded unit vector + "
+                               f"{sa.excited_mix} x its second and third Ritz vectors (normalised sum), kept on the device"),
                 "warm_solves_above_cold_E0": c["continuation_mismatches"],
                 "end_to_end_rate_incl_build": c["end_to_end_rate_incl_build"],
                 "end_to_end_rate_pipelined": c["end_to_end_rate_pipelined"],

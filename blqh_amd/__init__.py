@@ -94,7 +94,7 @@ _SIGS = {
     "blqh_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "blqh_batch_lanczos": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.c_void_p, C.c_void_p,
                                      C.POINTER(LanczosResult), C.c_void_p]),
-    "blqh_batch_lanczos_continue": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.c_void_p,
+    "blqh_batch_lanczos_continue": (C.c_int, [C.c_void_p, C.POINTER(LanczosOpts), C.c_void_p, C.c_double,
                                               C.POINTER(LanczosResult), C.c_void_p]),
     "blqh_batch_rebind": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_int32]),
     "blqh_batch_create_scaled": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32,
@@ -496,13 +496,14 @@ class Batch:
         _check(self._lib.blqh_batch_set_lanes(self.h, v))
 
     def lanczos(self, lam=200, seeds=0, tol=1e-10, res_tol=1e-7, check_every=5, starts=None, want_psi=True,
-                start_mix=0.0, continue_prev=False):
+                start_mix=0.0, continue_prev=False, excited_mix=0.0):
         """Ground states of every problem; `seeds` is one seed or one per problem,
         `starts` None or a list of start vectors (None entries: seeded),
         `start_mix` the weight of the seeded unit vector added to each start.
         continue_prev: start every problem from conj(psi) of the previous
         solve on this batch, kept on the device (blqh_batch_lanczos_continue;
-        the same lam as that solve, no `starts`)."""
+        the same lam as that solve, no `starts`); excited_mix adds that
+        weight of the previous solve's second and third Ritz vectors."""
         P = len(self.hams)
         opts = LanczosOpts(lam, 0, tol, check_every, 1, None, res_tol, 0, start_mix)
         sd = np.asarray(seeds, np.uint32)
@@ -531,7 +532,8 @@ class Batch:
         if continue_prev:
             if starts is not None:
                 raise ValueError("continue_prev takes no start vectors")
-            _check(self._lib.blqh_batch_lanczos_continue(self.h, C.byref(opts), sd.ctypes.data, res, po))
+            _check(self._lib.blqh_batch_lanczos_continue(self.h, C.byref(opts), sd.ctypes.data, float(excited_mix),
+                                                         res, po))
         else:
             _check(self._lib.blqh_batch_lanczos(self.h, C.byref(opts), sd.ctypes.data, st, res, po))
         out = []

@@ -34,7 +34,7 @@ void blqh_lanczos_reconstruct_impl(blqh_hamil* H, int m, uint32_t seed, const do
 void blqh_tridiag_vectors_impl(int m, const double* alpha, const double* offdiag, int k0, int count, double* vecs);
 void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const uint32_t* seeds,
                              const double* const* starts, blqh_lanczos_result* res, double* const* psi_out,
-                             bool cont);
+                             bool cont, double excited);
 blqh_mlp* blqh_mlp_create_impl(int nbits, uint32_t seed);
 void blqh_mlp_destroy_impl(blqh_mlp* M);
 int blqh_mlp_nparams_impl(const blqh_mlp* M);
@@ -509,13 +509,13 @@ int blqh_batch_rebind_scaled(blqh_batch* b, const blqh_hamil* const* base, const
 int blqh_batch_lanczos(blqh_batch* b, const blqh_lanczos_opts* o, const uint32_t* seeds, const double* const* starts,
                        blqh_lanczos_result* res, double* const* psi) {
     if (!b || !o || !res) return fail(BLQH_EINVAL, "null argument");
-    return guarded([&] { blqh_batch_lanczos_impl(b, *o, seeds, starts, res, psi, false); });
+    return guarded([&] { blqh_batch_lanczos_impl(b, *o, seeds, starts, res, psi, false, 0.0); });
 }
 
 int blqh_batch_lanczos_continue(blqh_batch* b, const blqh_lanczos_opts* o, const uint32_t* seeds,
-                                blqh_lanczos_result* res, double* const* psi) {
+                                double excited_mix, blqh_lanczos_result* res, double* const* psi) {
     if (!b || !o || !res) return fail(BLQH_EINVAL, "null argument");
-    return guarded([&] { blqh_batch_lanczos_impl(b, *o, seeds, nullptr, res, psi, true); });
+    return guarded([&] { blqh_batch_lanczos_impl(b, *o, seeds, nullptr, res, psi, true, excited_mix); });
 }
 
 int blqh_batch_get_lanes(const blqh_batch* b, int32_t* lanes) {

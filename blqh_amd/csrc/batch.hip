@@ -591,16 +591,19 @@ __global__ void __launch_bounds__(kBT) k_bstart_scale(const BStart* __restrict__
 // reference row order (prev), conjugated (psi = sum_l c_l conj(v_l) is the
 // eigenvector of conj(H), so conj(psi) is that of H), plus mix times the
 // seeded unit vector: dst[r] = conj(prev[ip(r)]) + mix / |gen| * gen[ip(r)]
-// (ip = the member's internal order, the identity without one).  One
+// + xmix * conj(prevx[ip(r)]) (ip = the member's internal order, the
+// identity without one; prevx the previous solve's excited mix).  One
 // 1024-thread workgroup per problem; nrm2 gets |dst|^2, the same fixed-order
 // sum on every call.
 struct BCont {
     int64_t n;
     const double2* prev;
     const double2* gen;  // seeded entries, reference order (read when mix > 0)
+    const double2* prevx;  // the previous solve's excited mix, reference order (read when xmix > 0)
     double2* dst;
     const int32_t* iperm;
     double mix;          // start_mix / |gen|
+    double xmix;
 };
 
 __global__ void __launch_bounds__(1024) k_bcont(const BCont* __restrict__ jobs, double* __restrict__ nrm2) {
@@ -615,6 +618,11 @@ __global__ void __launch_bounds__(1024) k_bcont(const BCont* __restrict__ jobs, 
             const double2 g = J.gen[q];
             z.x += J.mix * g.x;
             z.y += J.mix * g.y;
+        }
+        if (J.xmix > 0) {
+            const double2 g = J.prevx[q];
+            z.x += J.xmix * g.x;
+            z.y -= J.xmix * g.y;
         }
         J.dst[r] = z;
         acc += z.x * z.x + z.y * z.y;
@@ -640,23 +648,28 @@ __global__ void __launch_bounds__(kBT) k_bpsi_ref(const BPsi* __restrict__ jobs)
 }
 
 __global__ void __launch_bounds__(kBT) k_bcombine(BState S, const double* __restrict__ coef,
-                                                  const int32_t* __restrict__ mvec, double2* __restrict__ psi) {
+                                                  const double* __restrict__ coefx, const int32_t* __restrict__ mvec,
+                                                  double2* __restrict__ psi, double2* __restrict__ xs) {
     __shared__ double sh[kBT / 64];
     for (int64_t it = blockIdx.x; it < S.nitems; it += gridDim.x) {
         const int p = S.item_prob[it];
         const BProb P = S.probs[p];
         const int m = mvec[p];
         const double* cf = coef + (int64_t)p * S.cap;
+        const double* cx = coefx + (int64_t)p * S.cap;
         const int64_t r = (it - P.item0) * kItemRows + threadIdx.x;
         double acc = 0.0;
         if (r < P.n) {
-            double re = 0.0, im = 0.0;
+            double re = 0.0, im = 0.0, xr = 0.0, xi = 0.0;
             for (int l = 0; l < m; ++l) {
                 const double2 w = P.W[(int64_t)l * P.n + r];
                 re += cf[l] * w.x;
                 im -= cf[l] * w.y;
+                xr += cx[l] * w.x;
+                xi -= cx[l] * w.y;
             }
             psi[P.voff + r] = make_double2(re, im);
+            xs[P.voff + r] = make_double2(xr, xi);
             acc = re * re + im * im;
         }
         const double t = block_sum(acc, sh);
@@ -745,8 +758,11 @@ private:
 struct Pools {
     DevBuf<BProb> probs;
     DevBuf<int32_t> active, jbase, mvec;
-    DevBuf<double> alpha, beta, invb, part1, part2, coef, scale;
+    DevBuf<double> alpha, beta, invb, part1, part2, coef, coefx, scale;
     DevBuf<double2> u, psi, W;
+    // the previous solve's excited mix (normalised sum of its second and
+    // third lowest Ritz vectors): internal order, then reference order
+    DevBuf<double2> xs, xref;
 };
 }  // namespace
 
@@ -1217,10 +1233,124 @@ void blqh_batch_set_lanes_impl(blqh_batch* b, int lanes) {
     b->lpr = lanes;
 }
 
+namespace {
+// Number of eigenvalues below x of the tridiagonal (a, b) (Sturm sequence).
+int sturm_below(int m, const double* a, const double* b, double x) {
+    int c = 0;
+    double q = 1.0;
+    for (int i = 0; i < m; ++i) {
+        q = a[i] - x - (i ? b[i - 1] * b[i - 1] / q : 0.0);
+        if (q == 0.0) q = -1e-300;
+        if (q < 0) ++c;
+    }
+    return c;
+}
+
+// Eigenvector k (ascending) of the m x m tridiagonal (a, b): the eigenvalue
+// by bisection on Sturm counts, the vector by three steps of inverse
+// iteration, each solve by Gaussian elimination with partial pivoting on the
+// band (the steps of LAPACK's dgttrf / dgttrs).  x gets a unit vector.
+void tridiag_vector_k(int m, const double* a, const double* b, int k, double* x) {
+    double lo = a[0], hi = a[0];
+    for (int i = 0; i < m; ++i) {
+        const double r = (i ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(b[i]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        if (sturm_below(m, a, b, mid) >= k + 1) hi = mid; else lo = mid;
+    }
+    const double e = 0.5 * (lo + hi);
+    const double sh = e - 1e-13 * std::max(1.0, std::fabs(e));
+    std::vector<double> dl(m), dd(m), du(m), du2(m), y(m);
+    std::vector<char> sw(m, 0);
+    for (int i = 0; i < m; ++i) {
+        dd[i] = a[i] - sh;
+        du[i] = i + 1 < m ? b[i] : 0.0;
+        dl[i] = i + 1 < m ? b[i] : 0.0;
+        du2[i] = 0.0;
+    }
+    for (int i = 0; i + 1 < m; ++i) {
+        if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
+            if (dd[i] == 0.0) dd[i] = 1e-300;
+            const double l = dl[i] / dd[i];
+            dl[i] = l;
+            dd[i + 1] -= l * du[i];
+        } else {  // rows i and i + 1 swap
+            const double l = dd[i] / dl[i];
+            dd[i] = dl[i];
+            dl[i] = l;
+            const double t = du[i];
+            du[i] = dd[i + 1];
+            dd[i + 1] = t - l * dd[i + 1];
+            if (i + 2 < m) {
+                du2[i] = du[i + 1];
+                du[i + 1] = -l * du[i + 1];
+            }
+            sw[i] = 1;
+        }
+    }
+    if (dd[m - 1] == 0.0) dd[m - 1] = 1e-300;
+    for (int i = 0; i < m; ++i) x[i] = 1.0 / std::sqrt((double)m);
+    for (int iter = 0; iter < 3; ++iter) {
+        for (int i = 0; i < m; ++i) y[i] = x[i];
+        for (int i = 0; i + 1 < m; ++i) {
+            if (sw[i]) {
+                const double t = y[i];
+                y[i] = y[i + 1];
+                y[i + 1] = t - dl[i] * y[i];
+            } else {
+                y[i + 1] -= dl[i] * y[i];
+            }
+        }
+        for (int i = m - 1; i >= 0; --i) {
+            double v = y[i];
+            if (i + 1 < m) v -= du[i] * x[i + 1];
+            if (i + 2 < m) v -= du2[i] * x[i + 2];
+            x[i] = v / dd[i];
+        }
+        double s = 0;
+        for (int i = 0; i < m; ++i) s += x[i] * x[i];
+        s = 1.0 / std::sqrt(s);
+        for (int i = 0; i < m; ++i) x[i] *= s;
+    }
+}
+
+// Excited mix of a finished Lanczos run: the normalised sum of the second
+// and third lowest Ritz vectors (those that exist), orthogonalised against
+// the ground Ritz vector c0; zero when m = 1.  The start of a continuation
+// solve adds it so that a state that was excited at the previous parameter
+// point and is the ground state at the next one starts with a large weight.
+void excited_mix(int m, const double* a, const double* b, const double* c0, double* out) {
+    for (int l = 0; l < m; ++l) out[l] = 0.0;
+    if (m < 2) return;
+    std::vector<double> v(m);
+    for (int k = 1; k <= std::min(2, m - 1); ++k) {
+        tridiag_vector_k(m, a, b, k, v.data());
+        for (int l = 0; l < m; ++l) out[l] += v[l];
+    }
+    double d = 0, s = 0;
+    for (int l = 0; l < m; ++l) d += out[l] * c0[l];
+    for (int l = 0; l < m; ++l) {
+        out[l] -= d * c0[l];
+        s += out[l] * out[l];
+    }
+    if (!(s > 0) || !std::isfinite(s)) {
+        for (int l = 0; l < m; ++l) out[l] = 0.0;
+        return;
+    }
+    s = 1.0 / std::sqrt(s);
+    for (int l = 0; l < m; ++l) out[l] *= s;
+}
+}  // namespace
+
 void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const uint32_t* seeds,
                              const double* const* starts, blqh_lanczos_result* res, double* const* psi_out,
-                             bool cont) {
+                             bool cont, double excited) {
     BLQH_REQUIRE(o.start_mix >= 0 && std::isfinite(o.start_mix), BLQH_EINVAL, "start_mix must be finite and >= 0");
+    BLQH_REQUIRE(excited >= 0 && std::isfinite(excited), BLQH_EINVAL, "excited_mix must be finite and >= 0");
     if (cont)
         BLQH_REQUIRE(b->cont_ok && b->pool_cap == o.lambda && !starts, BLQH_EINVAL,
                      "a continuation solve needs a finished solve of the same lambda on this batch");
@@ -1270,6 +1400,9 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
         pl.part1.alloc(b->nitems);
         pl.part2.alloc(b->nitems);
         pl.coef.alloc((size_t)P * cap);
+        pl.coefx.alloc((size_t)P * cap);
+        pl.xs.alloc(b->rows);
+        pl.xref.alloc(b->rows);
         pl.scale.alloc(P);
         pl.u.alloc(b->rows);
         pl.psi.alloc(b->rows);
@@ -1302,7 +1435,8 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
         for (int p = 0; p < P; ++p) {
             const blqh_hamil* Hp = b->hs[p];
             double2* gen = hp[p].W + hp[p].n;
-            cj[p] = {hp[p].n, pl.u.get() + hp[p].voff, gen, hp[p].W, Hp->ord.on ? Hp->ord.iperm.get() : nullptr, 0.0};
+            cj[p] = {hp[p].n, pl.u.get() + hp[p].voff, gen, pl.xref.get() + hp[p].voff, hp[p].W,
+                     Hp->ord.on ? Hp->ord.iperm.get() : nullptr, 0.0, excited};
             if (o.start_mix > 0) gj.push_back({hp[p].n, seeds ? seeds[p] : o.seed, gen, gen, nullptr});
         }
         DevBuf<BStart> dgj;
@@ -1624,7 +1758,7 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     std::vector<double> hinvb((size_t)P * ld);
     BLQH_HIP(hipMemcpyAsync(hinvb.data(), pl.invb.get(), hinvb.size() * 8, hipMemcpyDeviceToHost, st));
     BLQH_HIP(hipStreamSynchronize(st));
-    std::vector<double> w((size_t)P * cap, 0.0), E0(P);
+    std::vector<double> w((size_t)P * cap, 0.0), wx((size_t)P * cap, 0.0), E0(P);
     std::vector<int32_t> hm(P);
     for (int p = 0; p < P; ++p)
         BLQH_REQUIRE(m[p] >= 1 && std::isfinite(ritz[p].back()), BLQH_EINTERNAL,
@@ -1633,6 +1767,8 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
         c.resize(m[p]);
         E0[p] = tridiag_lowest(m[p], ha + (size_t)p * cap, hb + (size_t)p * ld + 1, c.data());
         for (int l = 0; l < m[p]; ++l) w[(size_t)p * cap + l] = c[l] * hinvb[(size_t)p * ld + l];
+        excited_mix(m[p], ha + (size_t)p * cap, hb + (size_t)p * ld + 1, c.data(), wx.data() + (size_t)p * cap);
+        for (int l = 0; l < m[p]; ++l) wx[(size_t)p * cap + l] *= hinvb[(size_t)p * ld + l];
         hm[p] = m[p];
         if (!std::isfinite(resid[p])) resid[p] = hb[(size_t)p * ld + m[p]] * std::fabs(c[m[p] - 1]);
     };
@@ -1650,7 +1786,9 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
     BLQH_HIP(hipMemcpyAsync(pl.mvec.get(), hm.data(), P * 4, hipMemcpyHostToDevice, st));
     std::fill(act.begin(), act.end(), 1);
     BLQH_HIP(hipMemcpyAsync(pl.active.get(), act.data(), P * 4, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_bcombine, dim3(b->grid), dim3(kBT), 0, st, S, pl.coef.get(), pl.mvec.get(), pl.psi.get());
+    BLQH_HIP(hipMemcpyAsync(pl.coefx.get(), wx.data(), wx.size() * 8, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_bcombine, dim3(b->grid), dim3(kBT), 0, st, S, pl.coef.get(), pl.coefx.get(), pl.mvec.get(),
+                       pl.psi.get(), pl.xs.get());
     std::vector<double> hp1(b->nitems), hp2(b->nitems), sc(P);
     BLQH_HIP(hipMemcpyAsync(hp1.data(), pl.part1.get(), b->nitems * 8, hipMemcpyDeviceToHost, st));
     BLQH_HIP(hipStreamSynchronize(st));
@@ -1675,6 +1813,8 @@ void blqh_batch_lanczos_impl(blqh_batch* b, const blqh_lanczos_opts& o, const ui
         const blqh_hamil* Hp = b->hs[p];
         pj.push_back({hp[p].n, Hp->ord.on ? Hp->ord.iperm.get() : nullptr, pl.psi.get() + hp[p].voff,
                       pl.u.get() + hp[p].voff});
+        pj.push_back({hp[p].n, Hp->ord.on ? Hp->ord.iperm.get() : nullptr, pl.xs.get() + hp[p].voff,
+                      pl.xref.get() + hp[p].voff});
         pmax = std::max(pmax, hp[p].n);
         want = want || (psi_out && psi_out[p]);
     }

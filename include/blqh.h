@@ -249,11 +249,15 @@ int blqh_batch_lanczos(blqh_batch* b, const blqh_lanczos_opts* o, const uint32_t
  * start).  Every problem starts from conj(psi) of the previous finished solve
  * on this batch (kept on the device in the reference row order, across
  * blqh_batch_rebind / blqh_batch_rebind_scaled) plus o->start_mix times its
- * seeded unit vector, normalised: the start blqh_batch_lanczos builds from
- * starts[p] = conj(previous psi[p]).  Needs the same o->lambda as that solve
- * (BLQH_EINVAL otherwise, and on a batch that has not finished a solve). */
+ * seeded unit vector, normalised: with excited_mix = 0 the start
+ * blqh_batch_lanczos builds from starts[p] = conj(previous psi[p]).
+ * excited_mix > 0 also adds that weight times conj of the previous solve's
+ * excited mix (the normalised sum of its second and third lowest Ritz
+ * vectors, orthogonal to its ground Ritz vector), against level crossings
+ * between the points.  Needs the same o->lambda as that solve (BLQH_EINVAL
+ * otherwise, and on a batch that has not finished a solve). */
 int blqh_batch_lanczos_continue(blqh_batch* b, const blqh_lanczos_opts* o, const uint32_t* seeds,
-                                blqh_lanczos_result* res, double* const* psi);
+                                double excited_mix, blqh_lanczos_result* res, double* const* psi);
 /* Lanes per row of the batched SpMV (4, 8, 16 or 32 over row-major panels;
  * 64: one lane per row over step-major panels; autotuned at create). */
 int blqh_batch_get_lanes(const blqh_batch* b, int32_t* lanes);

@@ -55,5 +55,5 @@ def test_null_batch_arguments_are_refused():
     lib = blqh_amd.lib()
     res = (blqh_amd.LanczosResult * 1)()
     opts = blqh_amd.LanczosOpts(200, 0, 1e-10, 1, 1, None, 1e-7, 0, 0.0)
-    assert lib.blqh_batch_lanczos_continue(None, blqh_amd.C.byref(opts), None, res, None) == -1
+    assert lib.blqh_batch_lanczos_continue(None, blqh_amd.C.byref(opts), None, 0.0, res, None) == -1
     assert lib.blqh_batch_lanczos(None, blqh_amd.C.byref(opts), None, None, res, None) == -1

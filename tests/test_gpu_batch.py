@@ -226,3 +226,31 @@ def test_continuation_matches_host_starts(hams, mix):
         assert a["converged"] == 1
         assert a["E0"] == pytest.approx(b["E0"], rel=1e-10, abs=0)
     bt.close()
+
+
+def test_continuation_with_excited_mix(hams):
+    """excited_mix adds the previous solve's second and third Ritz vectors to
+    the continuation start: the solve still ends at the cold ground state of
+    the new point, and repeats bit for bit."""
+    seeds = [11 + i for i in range(len(PROBLEMS))]
+    bt = blqh_amd.Batch([H for _, _, H in hams])
+    bt.lanczos(lam=250, seeds=seeds, check_every=4, want_psi=False)
+    new = []
+    for (name, S, nphi, nel, nu, nsu, J, K, d, dsas), (_, B, _) in zip(PROBLEMS, hams):
+        new.append(blqh_amd.Hamiltonian(B, blqh_amd.params(nphi, d=d + 0.5, delta_sas=dsas)))
+    cold = blqh_amd.Batch(new, lanes=bt.lanes).lanczos(lam=250, seeds=seeds, check_every=4, want_psi=False)
+    bt.rebind(new)
+    with pytest.raises(blqh_amd.BlqhError):
+        bt.lanczos(lam=250, seeds=seeds, continue_prev=True, excited_mix=-1.0)
+    bt2 = blqh_amd.Batch([H for _, _, H in hams], lanes=bt.lanes)
+    bt2.lanczos(lam=250, seeds=seeds, check_every=4, want_psi=False)
+    bt2.rebind(new)
+    r1 = bt.lanczos(lam=250, seeds=seeds, check_every=4, start_mix=0.1, continue_prev=True, excited_mix=0.5)
+    r2 = bt2.lanczos(lam=250, seeds=seeds, check_every=4, start_mix=0.1, continue_prev=True, excited_mix=0.5)
+    for (name, _, H), a, b, c in zip(hams, r1, r2, cold):
+        print(f"excited mix {name}: it {a['iterations']} cold {c['iterations']}", flush=True)
+        assert a["converged"] == 1, name
+        assert a["E0"] == pytest.approx(c["E0"], rel=1e-10, abs=0), name
+        assert a["E0"] == b["E0"] and a["iterations"] == b["iterations"] and np.array_equal(a["psi"], b["psi"]), name
+    bt.close()
+    bt2.close()
