@@ -14,6 +14,7 @@
 
 #include "../../blqh_amd/csrc/engine.hpp"
 #include "../../blqh_amd/csrc/xcdplan.hpp"
+#include "../../blqh_amd/csrc/ritz.hpp"
 
 using namespace blqh;
 
@@ -307,6 +308,15 @@ int hc_hash_order_levels_check(const blqh_sector* s, int64_t* n) {
 // with a given reciprocal r of d.
 void hc_mod_by_rcp(int n, int d, float r, int32_t* out) {
     for (int s = 0; s < n; ++s) out[s] = mod_by_rcp(s, d, r);
+}
+
+// the batched solver's low Ritz vectors (ritz.hpp)
+void hc_tridiag_vector_k(int m, const double* a, const double* b, int k, double* x) {
+    tridiag_vector_k(m, a, b, k, x);
+}
+
+void hc_excited_mix(int m, const double* a, const double* b, const double* c0, double* out) {
+    excited_mix(m, a, b, c0, out);
 }
 
 }  // extern "C"

@@ -28,6 +28,8 @@ def lib():
         _L.hc_internal_order.argtypes = [C.POINTER(blqh_amd.Sector), C.c_void_p, C.c_int64, C.c_void_p]
         _L.hc_hash_order_check.restype = C.c_int
         _L.hc_hash_order_check.argtypes = [C.POINTER(blqh_amd.Sector), C.POINTER(C.c_int64)]
+        _L.hc_tridiag_vector_k.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        _L.hc_excited_mix.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _L.hc_hash_order_levels_check.restype = C.c_int
         _L.hc_hash_order_levels_check.argtypes = [C.POINTER(blqh_amd.Sector), C.POINTER(C.c_int64)]
         _L.hc_mod_by_rcp.restype = None
@@ -117,3 +119,22 @@ def hash_order_levels_check(sec):
     n = C.c_int64()
     ok = lib().hc_hash_order_levels_check(C.byref(sec), C.byref(n))
     return bool(ok), n.value
+
+
+def tridiag_vector_k(a, b, k):
+    """Eigenvector k (ascending) of the symmetric tridiagonal (a, b), as the batched solver computes it."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.empty(a.size)
+    lib().hc_tridiag_vector_k(a.size, a.ctypes.data, b.ctypes.data, k, x.ctypes.data)
+    return x
+
+
+def excited_mix(a, b, c0):
+    """The batched solver's excited mix of a tridiagonal with ground vector c0."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    c0 = np.ascontiguousarray(c0, np.float64)
+    out = np.empty(a.size)
+    lib().hc_excited_mix(a.size, a.ctypes.data, b.ctypes.data, c0.ctypes.data, out.ctypes.data)
+    return out

@@ -220,3 +220,30 @@ def test_mirror_monolayer_matches_oracle(oracle_mod, nphi, nel, J, K):
     np.testing.assert_array_equal(rp, d["rowptr"])
     np.testing.assert_array_equal(col, d["col"])
     np.testing.assert_allclose(val, d["val"], rtol=0, atol=1e-13 * np.abs(d["val"]).max())
+
+
+@pytest.mark.parametrize("m, seed", [(2, 0), (3, 1), (40, 2), (150, 3)])
+def test_low_ritz_vectors_match_lapack(m, seed):
+    """The batched solver's host Ritz vectors (bisection + pivoted inverse
+    iteration) against LAPACK's tridiagonal eigensolver: eigenvectors k = 0..2
+    up to sign, and the excited mix = (v1 + v2)/sqrt(2) up to the signs of
+    v1 and v2 (unit norm, orthogonal to v0, equal weight on v1 and v2)."""
+    from scipy.linalg import eigh_tridiagonal
+    rng = np.random.default_rng(seed)
+    a = rng.normal(size=m) * 2.0
+    b = np.abs(rng.normal(size=m - 1)) + 0.1  # Lanczos off-diagonals are positive
+    w, V = eigh_tridiagonal(a, b)
+    for k in range(min(3, m)):
+        x = hostmirror.tridiag_vector_k(a, b, k)
+        assert abs(np.linalg.norm(x) - 1) < 1e-12
+        assert abs(abs(x @ V[:, k]) - 1) < 1e-9, (m, k)
+    c0 = hostmirror.tridiag_vector_k(a, b, 0)
+    xm = hostmirror.excited_mix(a, b, c0)
+    assert abs(np.linalg.norm(xm) - 1) < 1e-12
+    assert abs(xm @ V[:, 0]) < 1e-9
+    if m == 2:
+        assert abs(abs(xm @ V[:, 1]) - 1) < 1e-9
+    else:
+        c = np.abs(V[:, 1:3].T @ xm)
+        assert np.allclose(c, 1 / np.sqrt(2), atol=1e-9), c
+    assert np.all(hostmirror.excited_mix(a[:1], b[:0], c0[:1]) == 0)
