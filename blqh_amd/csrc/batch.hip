@@ -19,7 +19,10 @@
 // runs the convergence test of the single-problem solver and clears the flags
 // of the converged ones (data-dependent termination without a host round trip
 // per step).  The Krylov vectors of every problem stay in HBM, so the ground
-// state is one weighted sum per problem afterwards (no replay pass).
+// state is one weighted sum per problem afterwards (no replay pass); the same
+// pass forms the excited mix (second and third Ritz vectors), and both stay
+// on the device in the reference row order as the start of a continuation
+// solve at the next parameter point (k_bcont).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
